@@ -1,0 +1,638 @@
+// fsph_api.hip -- the C ABI of include/fsph.h: context lifetime, transfers,
+// the per-step entry points and the host step driver that runs them in the
+// order of the reference's hydro/newtonian drivers.
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "fsph_internal.h"
+
+namespace fsph {
+
+int fail(Ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_check(Ctx *c, hipError_t e, const char *what) {
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorOutOfMemory)
+    return fail(c, FSPH_E_OOM, "%s: %s", what, hipGetErrorString(e));
+  return fail(c, FSPH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
+  if (bytes <= b.bytes) return 0;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return hip_check(c, e, "hipMalloc");
+  if (keep && b.p && b.bytes) {
+    e = hipMemcpyAsync(p, b.p, b.bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipStreamSynchronize");
+  }
+  if (b.p) {
+    hipStreamSynchronize(c->stream);
+    hipFree(b.p);
+  }
+  b.p = p;
+  b.bytes = bytes;
+  return 0;
+}
+
+size_t field_size(const Ctx *c, int f) {
+  if (f < PF_NDBL || f == PF_ID) return 8;
+  if (f == PF_NNGB) return 4;
+  return c->rsize;
+}
+
+int ensure_particles(Ctx *c, int64_t cap, bool keep) {
+  if (cap <= c->pcap) return 0;
+  for (int f = 0; f < PF_COUNT; ++f) {
+    const size_t b = (size_t)cap * field_size(c, f);
+    FSPH_TRY(ensure(c, c->f[f], b, keep));
+    FSPH_TRY(ensure(c, c->g[f], b, false));
+  }
+  c->pcap = cap;
+  return 0;
+}
+
+int read_flags(Ctx *c, int *flags) {
+  FSPH_HIP(c, hipMemcpyAsync(flags, c->flags.p, sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+namespace {
+
+struct PhaseTimer {
+  Ctx *c;
+  int slot;
+  PhaseTimer(Ctx *cc, int s) : c(cc), slot(s) {
+    if (c->timing) hipEventRecord(c->ev[2 * slot], c->stream);
+  }
+  ~PhaseTimer() {
+    if (c->timing) {
+      hipEventRecord(c->ev[2 * slot + 1], c->stream);
+      c->phase_used[slot] = true;
+    }
+  }
+};
+
+int do_update_iteration(Ctx *c) {
+  c->have_tree = false;
+  FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+  {
+    PhaseTimer t(c, FSPH_T_BOUNDARY);
+    FSPH_TRY(stage_boundary(c));
+  }
+  {
+    PhaseTimer t(c, FSPH_T_RANGE_KEYS);
+    FSPH_TRY(stage_range_keys(c));
+  }
+  {
+    PhaseTimer t(c, FSPH_T_SORT);
+    FSPH_TRY(stage_sort(c));
+  }
+  int fl = 0;
+  FSPH_TRY(read_flags(c, &fl));
+  if (fl & kFlagDupKey)
+    return fail(c, FSPH_E_DUPKEY,
+                "two particles share a Morton key (positions closer than "
+                "range/2^21)");
+  {
+    PhaseTimer t(c, FSPH_T_TREE);
+    FSPH_TRY(stage_tree(c));
+    FSPH_TRY(stage_groups(c));
+  }
+  {
+    PhaseTimer t(c, FSPH_T_NEIGHBORS);
+    FSPH_TRY(stage_neighbors(c));
+  }
+  c->have_tree = true;
+  return 0;
+}
+
+int need_tree(Ctx *c) {
+  if (!c->have_tree)
+    return fail(c, FSPH_E_ARG, "call fsph_update_iteration first");
+  return 0;
+}
+
+int do_timestep(Ctx *c) {
+  if (!c->p.adaptive_dt) return 0;
+  double dtmin = 0;
+  FSPH_TRY(stage_dt_min(c, &dtmin));
+  // set_adaptive_timestep, default_physics.h:762-823 (no output matching)
+  if (dtmin < c->dt) c->dt = (c->dt / 2.0 < dtmin) ? c->dt / 2.0 : dtmin;
+  if (dtmin > 2.0 * c->dt) c->dt = c->dt * 2.0;
+  c->totaltime_next = c->totaltime + c->dt;
+  volatile double tn = c->totaltime_next;
+  c->dt = tn - c->totaltime;
+  return 0;
+}
+
+int do_advance(Ctx *c) {
+  c->iteration++;
+  if (c->p.adaptive_dt)
+    c->totaltime = c->totaltime_next;
+  else
+    c->totaltime = c->totaltime + c->dt;
+  return 0;
+}
+
+int do_gravity(Ctx *c);
+
+int do_step(Ctx *c) {
+  if (c->timing)
+    for (int i = 0; i < FSPH_NTIMERS; ++i) c->phase_used[i] = false;
+  if (!c->initial_done) {
+    // hydro/main_driver.cc:102-173 (iteration 0: no integration)
+    FSPH_TRY(do_update_iteration(c));
+    {
+      PhaseTimer t(c, FSPH_T_DENSITY);
+      FSPH_TRY(stage_density(c));
+    }
+    FSPH_TRY(stage_integrate(c, FSPH_SAVE_VHALF));
+    {
+      PhaseTimer t(c, FSPH_T_ACCEL);
+      FSPH_TRY(stage_acceleration(c));
+    }
+    if (c->p.gravity) {
+      PhaseTimer t(c, FSPH_T_GRAVITY);
+      FSPH_TRY(do_gravity(c));
+    }
+    FSPH_TRY(do_timestep(c));
+    {
+      PhaseTimer t(c, FSPH_T_DUDT);
+      FSPH_TRY(stage_dudt(c));
+    }
+    c->initial_done = true;
+  } else {
+    // hydro/main_driver.cc:174-245
+    {
+      PhaseTimer t(c, FSPH_T_INTEGRATE);
+      FSPH_TRY(stage_integrate(
+          c, FSPH_KICK_U | FSPH_KICK_V | FSPH_SAVE_VHALF | FSPH_DRIFT));
+    }
+    FSPH_TRY(do_update_iteration(c));
+    {
+      PhaseTimer t(c, FSPH_T_DENSITY);
+      FSPH_TRY(stage_density(c));
+    }
+    {
+      PhaseTimer t(c, FSPH_T_ACCEL);
+      FSPH_TRY(stage_acceleration(c));
+    }
+    if (c->p.gravity) {
+      PhaseTimer t(c, FSPH_T_GRAVITY);
+      FSPH_TRY(do_gravity(c));
+    }
+    {
+      PhaseTimer t(c, FSPH_T_DUDT);
+      FSPH_TRY(stage_integrate(c, FSPH_KICK_V));
+      FSPH_TRY(stage_dudt(c));
+      FSPH_TRY(stage_integrate(c, FSPH_KICK_U));
+    }
+  }
+  {
+    PhaseTimer t(c, FSPH_T_H);
+    FSPH_TRY(stage_update_h(c));  // hydro/main_driver.cc:247-256
+  }
+  {
+    PhaseTimer t(c, FSPH_T_DT);
+    FSPH_TRY(do_timestep(c));  // hydro/main_driver.cc:267-273
+  }
+  return do_advance(c);
+}
+
+int do_gravity(Ctx *c) {
+  return fail(c, FSPH_E_ARG, "gravity is not available in this build");
+}
+
+}  // namespace
+}  // namespace fsph
+
+using fsph::Ctx;
+
+extern "C" {
+
+void fsph_default_params(fsph_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->kernel = FSPH_KERNEL_CUBIC_SPLINE;
+  p->eos = FSPH_EOS_IDEAL;
+  p->precision = FSPH_PREC_FP32;
+  p->adaptive_dt = 0;
+  p->gravity = FSPH_GRAV_NONE;
+  p->group_max = 128;
+  p->leaf_size = 32;
+  p->n_target = 100;
+  p->gamma = 1.4;
+  p->poly_K = 1.0;
+  p->eta = 1.0;
+  p->visc_alpha = 1.0;
+  p->visc_beta = 2.0;
+  p->visc_eps = 0.01;
+  p->cfl = 0.25;
+  p->initial_dt = 0.001;
+  p->G = 1.0;
+  p->theta = 0.0;
+  for (int d = 0; d < 3; ++d) p->box[d] = 1.0;
+}
+
+int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
+  if (!p || !out) return FSPH_E_ARG;
+  *out = nullptr;
+  if (p->group_max < 1 || p->group_max > 128) return FSPH_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return FSPH_E_NODEV;
+  Ctx *c = new (std::nothrow) Ctx();
+  if (!c) return FSPH_E_OOM;
+  c->p = *p;
+  c->device = device;
+  c->rsize = p->precision == FSPH_PREC_FP64 ? 8 : 4;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) !=
+          hipSuccess) {
+    delete c;
+    return FSPH_E_HIP;
+  }
+  if (fsph::ensure(c, c->flags, 64)) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return FSPH_E_OOM;
+  }
+  for (int i = 0; i < 2 * FSPH_NTIMERS; ++i) hipEventCreate(&c->ev[i]);
+  *out = reinterpret_cast<fsph_ctx *>(c);
+  return FSPH_OK;
+}
+
+void fsph_destroy(fsph_ctx *ctx) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  fsph::DBuf *all[] = {&c->key, &c->key2, &c->val, &c->val2, &c->perm,
+                       &c->hist, &c->scan_tmp, &c->red_tmp, &c->flags,
+                       &c->L, &c->pair_cnt, &c->pair_off, &c->nkey,
+                       &c->ndepth, &c->nstart, &c->ncount, &c->nparent,
+                       &c->nchild, &c->nbox, &c->ncom, &c->nmass, &c->nrad,
+                       &c->nlap, &c->nlevel, &c->leaf_parent, &c->gstart,
+                       &c->gcount, &c->gnode, &c->gflag, &c->gscan, &c->cand,
+                       &c->ccount, &c->moff, &c->mask};
+  for (auto *b : all)
+    if (b->p) hipFree(b->p);
+  for (int f = 0; f < fsph::PF_COUNT; ++f) {
+    if (c->f[f].p) hipFree(c->f[f].p);
+    if (c->g[f].p) hipFree(c->g[f].p);
+  }
+  for (int i = 0; i < 2 * FSPH_NTIMERS; ++i)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *fsph_last_error(const fsph_ctx *ctx) {
+  const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+  return c ? c->err.c_str() : "null context";
+}
+
+#define CTX_OR_FAIL                                   \
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);              \
+  if (!c) return FSPH_E_ARG;                          \
+  if (hipSetDevice(c->device) != hipSuccess) return FSPH_E_HIP
+
+int fsph_upload(fsph_ctx *ctx, const fsph_particles *in) {
+  CTX_OR_FAIL;
+  if (!in || in->n < 2 || !in->x || !in->y || !in->z || !in->vx || !in->vy ||
+      !in->vz || !in->u || !in->h || !in->m || !in->id)
+    return fsph::fail(c, FSPH_E_ARG, "upload needs n >= 2 and every field");
+  if (in->n >= (int64_t)1 << 31)
+    return fsph::fail(c, FSPH_E_ARG, "n must be below 2^31");
+  const int64_t n = in->n;
+  const bool per = c->p.periodic[0] || c->p.periodic[1] || c->p.periodic[2];
+  FSPH_TRY(fsph::ensure_particles(c, per ? (int64_t)(n * 2.2) + 4096 : n,
+                                  false));
+  const size_t b = (size_t)n * 8;
+  const double *src[] = {in->x,  in->y,  in->z, in->vx, in->vy, in->vz,
+                         in->vx, in->vy, in->vz, in->u, in->h, in->m};
+  for (int f = 0; f < fsph::PF_NDBL; ++f)
+    FSPH_HIP(c, hipMemcpyAsync(c->f[f].p, src[f], b, hipMemcpyHostToDevice,
+                               c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(c->f[fsph::PF_ID].p, in->id, b,
+                             hipMemcpyHostToDevice, c->stream));
+  // rho, P, c (optional) in the context's real type
+  const double *opt[3] = {in->rho, in->P, in->c};
+  const int of[3] = {fsph::PF_RHO, fsph::PF_P, fsph::PF_C};
+  std::vector<char> host((size_t)n * c->rsize);
+  for (int q = 0; q < 3; ++q) {
+    if (opt[q]) {
+      if (c->rsize == 4) {
+        float *h = reinterpret_cast<float *>(host.data());
+        for (int64_t i = 0; i < n; ++i) h[i] = (float)opt[q][i];
+      } else {
+        std::memcpy(host.data(), opt[q], b);
+      }
+    } else {
+      std::memset(host.data(), 0, host.size());
+    }
+    FSPH_HIP(c, hipMemcpyAsync(c->f[of[q]].p, host.data(), host.size(),
+                               hipMemcpyHostToDevice, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  const int zero_f[] = {fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY, fsph::PF_AZ,
+                        fsph::PF_DUDT, fsph::PF_GX, fsph::PF_GY, fsph::PF_GZ,
+                        fsph::PF_PHI, fsph::PF_COLD, fsph::PF_NNGB};
+  for (int f : zero_f)
+    FSPH_HIP(c, hipMemsetAsync(c->f[f].p, 0, (size_t)n * fsph::field_size(c, f),
+                               c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->n = c->n_real = n;
+  c->have_tree = false;
+  c->iteration = 0;
+  c->dt = c->p.initial_dt;
+  c->totaltime = 0;
+  c->totaltime_next = 0;
+  c->initial_done = false;
+  return FSPH_OK;
+}
+
+int fsph_count(const fsph_ctx *ctx, int64_t *n_total, int64_t *n_real) {
+  const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  if (n_total) *n_total = c->n;
+  if (n_real) *n_real = c->n_real;
+  return FSPH_OK;
+}
+
+int fsph_download(fsph_ctx *ctx, int field, void *dst) {
+  CTX_OR_FAIL;
+  if (!dst || field < 0 || field >= FSPH_NFIELDS)
+    return fsph::fail(c, FSPH_E_ARG, "bad field %d", field);
+  const int64_t n = c->n;
+  if (n == 0) return FSPH_OK;
+  static const int map[FSPH_NFIELDS] = {
+      fsph::PF_X,   fsph::PF_Y,   fsph::PF_Z,    fsph::PF_VX,   fsph::PF_VY,
+      fsph::PF_VZ,  fsph::PF_VHX, fsph::PF_VHY,  fsph::PF_VHZ,  fsph::PF_U,
+      fsph::PF_H,   fsph::PF_M,   fsph::PF_RHO,  fsph::PF_P,    fsph::PF_C,
+      fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY,   fsph::PF_AZ,   fsph::PF_DUDT,
+      fsph::PF_GX,  fsph::PF_GY,  fsph::PF_GZ,   fsph::PF_PHI,  fsph::PF_ID,
+      -1 /* key */, fsph::PF_NNGB};
+  if (field == FSPH_F_KEY) {
+    if (!c->key.p) return fsph::fail(c, FSPH_E_ARG, "no keys yet");
+    FSPH_HIP(c, hipMemcpyAsync(dst, c->key.p, (size_t)n * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+  } else if (field == FSPH_F_NNGB) {
+    std::vector<int32_t> t(n);
+    FSPH_HIP(c, hipMemcpyAsync(t.data(), c->f[fsph::PF_NNGB].p, (size_t)n * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    double *d = static_cast<double *>(dst);
+    for (int64_t i = 0; i < n; ++i) d[i] = t[i];
+    return FSPH_OK;
+  } else {
+    const int f = map[field];
+    const size_t es = fsph::field_size(c, f);
+    if (es == 8) {
+      FSPH_HIP(c, hipMemcpyAsync(dst, c->f[f].p, (size_t)n * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    } else {
+      // real field stored as float: widen on the device into scratch
+      FSPH_TRY(fsph::ensure(c, c->red_tmp, (size_t)n * 8));
+      FSPH_TRY(fsph::convert_to_double(c, c->f[f].p, es,
+                                       c->red_tmp.as<double>(), n));
+      FSPH_HIP(c, hipMemcpyAsync(dst, c->red_tmp.p, (size_t)n * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_update_iteration(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
+  return fsph::do_update_iteration(c);
+}
+
+int fsph_eos_init(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  return FSPH_OK;
+}
+
+int fsph_density_eos(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::stage_density(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_acceleration(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::stage_acceleration(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_gravity(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::do_gravity(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_dudt(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::stage_dudt(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_integrate(fsph_ctx *ctx, uint32_t what) {
+  CTX_OR_FAIL;
+  if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
+  FSPH_TRY(fsph::stage_integrate(c, what));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_update_h(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::stage_update_h(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_timestep(fsph_ctx *ctx, double *dt) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::do_timestep(c));
+  if (dt) *dt = c->dt;
+  return FSPH_OK;
+}
+
+int fsph_advance_time(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  return fsph::do_advance(c);
+}
+
+int fsph_step(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
+  FSPH_TRY(fsph::do_step(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->timing)
+    for (int i = 0; i < FSPH_NTIMERS; ++i) {
+      float ms = 0;
+      if (c->phase_used[i])
+        hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]);
+      c->phase_ms[i] = ms;
+    }
+  return FSPH_OK;
+}
+
+int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,
+              double *totaltime) {
+  const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  if (iteration) *iteration = c->iteration;
+  if (dt) *dt = c->dt;
+  if (totaltime) *totaltime = c->totaltime;
+  return FSPH_OK;
+}
+
+int fsph_set_dt(fsph_ctx *ctx, double dt) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !(dt > 0)) return FSPH_E_ARG;
+  c->dt = dt;
+  return FSPH_OK;
+}
+
+int fsph_reductions(fsph_ctx *ctx, double out[10]) {
+  CTX_OR_FAIL;
+  if (!out) return FSPH_E_ARG;
+  return fsph::stage_reductions(c, out);
+}
+
+int fsph_get_range(fsph_ctx *ctx, double lo[3], double hi[3]) {
+  CTX_OR_FAIL;
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = c->lo[d];
+    hi[d] = c->hi[d];
+  }
+  return FSPH_OK;
+}
+
+int fsph_get_tree(fsph_ctx *ctx, int64_t *n_nodes, uint64_t *node_key,
+                  int32_t *node_start, int32_t *node_count,
+                  int32_t *node_parent, int64_t *n_groups,
+                  int32_t *group_start, int32_t *group_count) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  if (n_nodes) *n_nodes = c->nn;
+  if (n_groups) *n_groups = c->ng;
+  const size_t nn = (size_t)c->nn, ng = (size_t)c->ng;
+  if (node_key)
+    FSPH_HIP(c, hipMemcpyAsync(node_key, c->nkey.p, nn * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+  if (node_start)
+    FSPH_HIP(c, hipMemcpyAsync(node_start, c->nstart.p, nn * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  if (node_count)
+    FSPH_HIP(c, hipMemcpyAsync(node_count, c->ncount.p, nn * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  if (node_parent)
+    FSPH_HIP(c, hipMemcpyAsync(node_parent, c->nparent.p, nn * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  if (group_start)
+    FSPH_HIP(c, hipMemcpyAsync(group_start, c->gstart.p, ng * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  if (group_count)
+    FSPH_HIP(c, hipMemcpyAsync(group_count, c->gcount.p, ng * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
+                        int64_t cap) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::need_tree(c));
+  if (!off) return fsph::fail(c, FSPH_E_ARG, "off is required");
+  const int64_t n = c->n, ng = c->ng;
+  std::vector<int32_t> gs(ng), gc(ng), cc(ng);
+  std::vector<int64_t> mo(ng + 1);
+  std::vector<uint32_t> mask((size_t)c->mask_words);
+  std::vector<int32_t> nngb(n);
+  FSPH_HIP(c, hipMemcpyAsync(gs.data(), c->gstart.p, ng * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(gc.data(), c->gcount.p, ng * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(cc.data(), c->ccount.p, ng * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(mo.data(), c->moff.p, (ng + 1) * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(nngb.data(), c->f[fsph::PF_NNGB].p, n * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (c->mask_words)
+    FSPH_HIP(c, hipMemcpyAsync(mask.data(), c->mask.p, mask.size() * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  off[0] = 0;
+  for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + nngb[i];
+  if (!ids) return FSPH_OK;
+  if (cap < off[n]) return fsph::fail(c, FSPH_E_CAP, "ids buffer too small");
+  std::vector<int32_t> cand;
+  for (int64_t g = 0; g < ng; ++g) {
+    cand.resize(cc[g]);
+    if (cc[g])
+      FSPH_HIP(c, hipMemcpy(cand.data(),
+                            c->cand.as<int32_t>() + g * c->ccap,
+                            (size_t)cc[g] * 4, hipMemcpyDeviceToHost));
+    const int nw = (cc[g] + 31) / 32;
+    for (int k = 0; k < gc[g]; ++k) {
+      const int64_t i = gs[g] + k;
+      std::vector<int64_t> lst;
+      for (int w = 0; w < nw; ++w) {
+        uint32_t bits = mask[mo[g] + (int64_t)w * gc[g] + k];
+        while (bits) {
+          const int j = __builtin_ctz(bits);
+          bits &= bits - 1;
+          lst.push_back(cand[w * 32 + j]);
+        }
+      }
+      std::sort(lst.begin(), lst.end());
+      if ((int64_t)lst.size() != off[i + 1] - off[i])
+        return fsph::fail(c, FSPH_E_ARG, "inconsistent neighbour count");
+      std::copy(lst.begin(), lst.end(), ids + off[i]);
+    }
+  }
+  return FSPH_OK;
+}
+
+int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !ms) return FSPH_E_ARG;
+  for (int i = 0; i < FSPH_NTIMERS; ++i) ms[i] = c->phase_ms[i];
+  return FSPH_OK;
+}
+
+int fsph_enable_timing(fsph_ctx *ctx, int on) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  c->timing = on != 0;
+  return FSPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// fsph_internal.h -- context layout and kernel-launch declarations shared by
+// the HIP translation units of libfsph.so.  Device state is structure-of-
+// arrays, resident between calls, in the particles' current (sorted) order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <utility>
+#include <string>
+#include <vector>
+
+#include "fsph.h"
+
+namespace fsph {
+
+constexpr int kMaxDepth = 21;     // filling_curve.h:44-47 (64-bit keys, 3D)
+constexpr int32_t kChildEmpty = -1;
+// child codes: >= 0 node index; <= -2 leaf particle p stored as -(p+2)
+__host__ __device__ inline int32_t leaf_code(int64_t p) {
+  return -static_cast<int32_t>(p) - 2;
+}
+__host__ __device__ inline int64_t leaf_of(int32_t c) { return -(int64_t)c - 2; }
+
+// Growable device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Error bits raised by kernels into ctx->flags (device int).
+enum : int {
+  kFlagDupKey = 1,
+  kFlagStack = 2,
+  kFlagCand = 4,
+  kFlagNaN = 8,
+  kFlagNeg = 16,
+};
+
+struct Ctx;
+
+// Persistent per-particle fields.  FP64 fields are double; derived fields
+// (density ... phi) are float or double by precision.
+enum PField : int {
+  PF_X = 0, PF_Y, PF_Z, PF_VX, PF_VY, PF_VZ, PF_VHX, PF_VHY, PF_VHZ, PF_U,
+  PF_H, PF_M, PF_NDBL,  // 12 double fields
+  PF_ID = PF_NDBL,      // int64
+  PF_RHO, PF_P, PF_C, PF_COLD, PF_VSIG, PF_AX, PF_AY, PF_AZ, PF_DUDT, PF_GX,
+  PF_GY, PF_GZ, PF_PHI, // real-typed fields
+  PF_NNGB,              // int32
+  PF_COUNT
+};
+
+struct Ctx {
+  fsph_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t rsize = 4;  // sizeof(real) for derived fields
+
+  int64_t n = 0;       // particles held (real + WALL copies)
+  int64_t n_real = 0;
+  int64_t pcap = 0;    // particle capacity of every per-particle buffer
+  DBuf f[PF_COUNT];    // current order
+  DBuf g[PF_COUNT];    // scratch of the same shapes (permutation target)
+
+  // keys / sort
+  DBuf key, key2, val, val2, perm;
+  DBuf hist;           // radix histograms
+  DBuf scan_tmp;
+  DBuf red_tmp;        // reduction partials
+  DBuf flags;          // int error flags
+  DBuf host_flags_dummy;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+
+  // tree
+  int64_t nn = 0, ncap = 0;
+  DBuf L;              // common-prefix depth per adjacent pair (int8 as int32)
+  DBuf pair_cnt, pair_off;   // nodes owned per pair, their offsets (int64)
+  DBuf nkey, ndepth, nstart, ncount, nparent, nchild, nbox, ncom, nmass,
+      nrad, nlap, nlevel;    // per-node arrays; nlevel = ids sorted by depth
+  DBuf leaf_parent;
+  int64_t level_off[kMaxDepth + 2] = {0};
+  int max_depth = 0;
+
+  // groups and neighbour tiles
+  int64_t ng = 0;
+  DBuf gstart, gcount, gnode, gflag, gscan;
+  int64_t ccap = 2048;       // candidate capacity per group
+  DBuf cand, ccount, moff;   // cand: ng*ccap int32; ccount int32; moff int64
+  DBuf mask;                 // uint32 words
+  int64_t mask_words = 0;
+  bool have_tree = false;
+
+  // time stepping
+  int64_t iteration = 0;
+  double dt = 0, totaltime = 0, totaltime_next = 0;
+  bool initial_done = false;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev[2 * FSPH_NTIMERS] = {};
+  float phase_ms[FSPH_NTIMERS] = {0};
+  bool phase_used[FSPH_NTIMERS] = {false};
+};
+
+// ---- helpers (fsph_api.hip) ---------------------------------------------
+int fail(Ctx *c, int code, const char *fmt, ...);
+int hip_check(Ctx *c, hipError_t e, const char *what);
+#define FSPH_HIP(c, call)                                 \
+  do {                                                    \
+    int _rc = ::fsph::hip_check((c), (call), #call);      \
+    if (_rc) return _rc;                                  \
+  } while (0)
+#define FSPH_TRY(call)      \
+  do {                      \
+    int _rc = (call);       \
+    if (_rc) return _rc;    \
+  } while (0)
+
+int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep = false);
+int ensure_particles(Ctx *c, int64_t cap, bool keep);
+size_t field_size(const Ctx *c, int f);
+int read_flags(Ctx *c, int *flags);
+
+// ---- primitives (k_prim.hip) --------------------------------------------
+// exclusive scan of n int64 values; out has n+1 entries (out[n] = total).
+int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n);
+// exclusive scan of n int32 values into int64 out (n+1 entries).
+int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n);
+
+// ---- pipeline stages ----------------------------------------------------
+int stage_boundary(Ctx *c);            // k_keys.hip
+int stage_range_keys(Ctx *c);          // k_keys.hip
+int stage_sort(Ctx *c);                // k_keys.hip
+int permute_particles(Ctx *c, const uint32_t *perm, int64_t n_out);
+int stage_tree(Ctx *c);                // k_tree.hip
+int stage_groups(Ctx *c);              // k_tree.hip
+int stage_neighbors(Ctx *c);           // k_neighbors.hip
+int stage_density(Ctx *c);             // k_sph.hip
+int stage_acceleration(Ctx *c);        // k_sph.hip
+int stage_dudt(Ctx *c);                // k_sph.hip
+int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
+int stage_update_h(Ctx *c);            // k_step.hip
+int stage_dt_min(Ctx *c, double *dtmin);     // k_step.hip
+int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
+int stage_max_h(Ctx *c, double *hmax);       // k_step.hip
+int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
+                      int64_t n);              // k_step.hip
+
+}  // namespace fsph

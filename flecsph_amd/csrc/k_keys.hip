@@ -1,0 +1,485 @@
+// k_keys.hip -- periodic WALL copies, range of x +- h, Morton keys, LSD
+// radix sort of (key, index), and the particle permutation.
+#include "fsph_internal.h"
+
+namespace fsph {
+
+namespace {
+
+constexpr int kT = 256;
+
+inline unsigned blocks_for(int64_t n, int t = kT) {
+  return (unsigned)((n + t - 1) / t);
+}
+
+// ---------------------------------------------------------------------------
+// Particle permutation: dst[f][i] = src[f][perm[i]] for every persistent field
+// ---------------------------------------------------------------------------
+constexpr int kMaxPermFields = 20;
+struct PermArgs {
+  const void *src[kMaxPermFields];
+  void *dst[kMaxPermFields];
+  int esize[kMaxPermFields];
+  int nf;
+};
+
+__global__ __launch_bounds__(kT) void k_permute(PermArgs a,
+                                                const uint32_t *__restrict__ perm,
+                                                int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int f = 0; f < a.nf; ++f) {
+    if (a.esize[f] == 8)
+      static_cast<uint64_t *>(a.dst[f])[i] =
+          static_cast<const uint64_t *>(a.src[f])[j];
+    else
+      static_cast<uint32_t *>(a.dst[f])[i] =
+          static_cast<const uint32_t *>(a.src[f])[j];
+  }
+}
+
+// Fields that carry over across a reordering: the state the reference keeps
+// in body_u between steps (body.h:222-251) that the next passes read before
+// recomputing it.
+const int kPersistent[] = {PF_X,  PF_Y,  PF_Z,  PF_VX,  PF_VY, PF_VZ,
+                           PF_VHX, PF_VHY, PF_VHZ, PF_U, PF_H, PF_M,
+                           PF_ID, PF_RHO, PF_P,  PF_C};
+constexpr int kNPersistent = sizeof(kPersistent) / sizeof(int);
+
+// ---------------------------------------------------------------------------
+// Periodic boundaries, include/physics/boundary.h
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_keep_real(const int64_t *__restrict__ id,
+                                                  int64_t n, int64_t n_real,
+                                                  int32_t *__restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) keep[i] = id[i] < n_real;
+}
+
+__global__ __launch_bounds__(kT) void k_compact_index(
+    const int32_t *__restrict__ keep, const int64_t *__restrict__ off,
+    int64_t n, uint32_t *__restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n && keep[i]) perm[off[i]] = (uint32_t)i;
+}
+
+struct Box {
+  double lo[3], hi[3];
+  int per[3];
+};
+
+// Step 1 of pboundary_generate (boundary.h:151-167): teleport into the box.
+__global__ __launch_bounds__(kT) void k_teleport(double *x, double *y,
+                                                 double *z, int64_t n, Box b) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  double *X[3] = {x, y, z};
+  for (int d = 0; d < 3; ++d) {
+    if (!b.per[d]) continue;
+    double v = X[d][i];
+    if (v > b.hi[d])
+      v -= b.hi[d] - b.lo[d];
+    else if (v <= b.lo[d])
+      v += b.hi[d] - b.lo[d];
+    X[d][i] = v;
+  }
+}
+
+__device__ inline int on_edge_mask(const double p[3], const Box &b,
+                                   double halo) {
+#pragma clang fp contract(off)
+  int on = 0;
+  for (int d = 0; d < 3; ++d)
+    if (b.per[d] && (p[d] + halo > b.hi[d] || p[d] - halo <= b.lo[d]))
+      on |= 1 << d;
+  return on;
+}
+
+__device__ inline int copies_of(int on) {
+  const int a = on & 1, bb = (on >> 1) & 1, cc = (on >> 2) & 1;
+  return a + bb + cc + (a & bb) + (a & cc) + (cc & bb) + (a & bb & cc);
+}
+
+__global__ __launch_bounds__(kT) void k_count_copies(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, int64_t n, Box b, double halo,
+    int32_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double p[3] = {x[i], y[i], z[i]};
+  cnt[i] = copies_of(on_edge_mask(p, b, halo));
+}
+
+// add_corner_XX/XXX (boundary.h:72-119): mirrored coordinate of a corner
+// copy.
+__device__ inline double corner_coord(double v, double lo, double hi,
+                                      double halo) {
+#pragma clang fp contract(off)
+  if (v + halo > hi) return lo - fabs(v + lo);
+  return hi + fabs(v + hi);
+}
+
+struct GenArgs {
+  void *f[kMaxPermFields];
+  int esize[kMaxPermFields];
+  int nf;
+};
+
+// Appends the WALL copies of particle i at n_real + off[i] + j in the order
+// of boundary.h:170-215: faces x, y, z, then corners xy, xz, zy, xyz.
+__global__ __launch_bounds__(kT) void k_gen_copies(
+    GenArgs a, double *x, double *y, double *z, int64_t *id,
+    const int64_t *__restrict__ off, int64_t n, int64_t n_real, Box b,
+    double halo) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double p[3] = {x[i], y[i], z[i]};
+  const int on = on_edge_mask(p, b, halo);
+  if (!on) return;
+  int64_t w = n_real + off[i];
+  double *X[3] = {x, y, z};
+  auto clone = [&](int64_t dst) {
+    for (int f = 0; f < a.nf; ++f) {
+      if (a.esize[f] == 8)
+        static_cast<uint64_t *>(a.f[f])[dst] = static_cast<uint64_t *>(a.f[f])[i];
+      else
+        static_cast<uint32_t *>(a.f[f])[dst] = static_cast<uint32_t *>(a.f[f])[i];
+    }
+    X[0][dst] = p[0]; X[1][dst] = p[1]; X[2][dst] = p[2];
+    id[dst] = dst;  // WALL ids: n_real + creation index (boundary.h:235-244)
+  };
+  for (int d = 0; d < 3; ++d) {
+    if (!(on >> d & 1)) continue;
+    clone(w);
+    X[d][w] = (p[d] + halo > b.hi[d]) ? p[d] - (b.hi[d] - b.lo[d])
+                                      : p[d] + (b.hi[d] - b.lo[d]);
+    ++w;
+  }
+  const int pairs[3][2] = {{0, 1}, {0, 2}, {2, 1}};
+  for (int q = 0; q < 3; ++q) {
+    const int d0 = pairs[q][0], d1 = pairs[q][1];
+    if ((on >> d0 & 1) && (on >> d1 & 1)) {
+      clone(w);
+      X[d0][w] = corner_coord(p[d0], b.lo[d0], b.hi[d0], halo);
+      X[d1][w] = corner_coord(p[d1], b.lo[d1], b.hi[d1], halo);
+      ++w;
+    }
+  }
+  if (on == 7) {
+    clone(w);
+    for (int d = 0; d < 3; ++d)
+      X[d][w] = corner_coord(p[d], b.lo[d], b.hi[d], halo);
+    ++w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Range (bodies_system.h:231-268) and keys (filling_curve.h:503-525)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_range_partial(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ h, int64_t n,
+    double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double s[6][kT];
+  double lo[3] = {INFINITY, INFINITY, INFINITY};
+  double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  const double *X[3] = {x, y, z};
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT) {
+    const double hh = h[i];
+    for (int d = 0; d < 3; ++d) {
+      const double v = X[d][i];
+      lo[d] = fmin(lo[d], v - hh);
+      hi[d] = fmax(hi[d], v + hh);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    s[d][threadIdx.x] = lo[d];
+    s[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int st = kT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int d = 0; d < 3; ++d) {
+        s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + st]);
+        s[3 + d][threadIdx.x] =
+            fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kT) void k_range_final(double *__restrict__ part,
+                                                    int nb) {
+  __shared__ double s[6][kT];
+  for (int d = 0; d < 6; ++d) {
+    double v = d < 3 ? INFINITY : -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += kT) {
+      const double q = part[b * 6 + d];
+      v = d < 3 ? fmin(v, q) : fmax(v, q);
+    }
+    s[d][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int st = kT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int d = 0; d < 6; ++d)
+        s[d][threadIdx.x] = d < 3 ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + st])
+                                  : fmax(s[d][threadIdx.x], s[d][threadIdx.x + st]);
+    __syncthreads();
+  }
+  // result at the end of the partial array: lo[3], hi[3]
+  if (threadIdx.x < 6) part[nb * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kT) void k_keys(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, int64_t n, const double *__restrict__ range,
+    uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t max_val = (1ull << 21) - 1;
+  const double p[3] = {x[i], y[i], z[i]};
+  uint64_t k = 1ull << 63;
+  for (int d = 0; d < 3; ++d) {
+    const double mn = range[d];
+    const double scale = range[3 + d] - mn;
+    const double q = (p[d] - mn) / scale * 2097152.0;
+    uint64_t cd = (uint64_t)q;
+    cd = cd < max_val ? cd : max_val;
+    // spread the 21 bits of cd to positions 3*b + d
+    uint64_t s = cd & 0x1fffffull;
+    s = (s | s << 32) & 0x1f00000000ffffull;
+    s = (s | s << 16) & 0x1f0000ff0000ffull;
+    s = (s | s << 8) & 0x100f00f00f00f00full;
+    s = (s | s << 4) & 0x10c30c30c30c30c3ull;
+    s = (s | s << 2) & 0x1249249249249249ull;
+    k |= s << d;
+  }
+  key[i] = k;
+  val[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8 bits per pass, 4096 keys per block (256 threads x 16).
+// ---------------------------------------------------------------------------
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kT * kRsItems;
+
+__global__ __launch_bounds__(kT) void k_radix_hist(
+    const uint64_t *__restrict__ key, int64_t n, int shift,
+    int32_t *__restrict__ hist, int64_t nblocks) {
+  __shared__ int32_t hs[256];
+  hs[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsItems; ++r) {
+    const int64_t i = base + r * kT + threadIdx.x;
+    if (i < n) atomicAdd(&hs[(key[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = hs[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kT) void k_radix_scatter(
+    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
+    int shift, const int64_t *__restrict__ hoff, int64_t nblocks) {
+  __shared__ int32_t wcnt[kT / 64][256];
+  __shared__ int32_t wbase[kT / 64][256];
+  __shared__ int32_t run[256];
+  __shared__ int64_t gofs[256];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  gofs[t] = hoff[(int64_t)t * nblocks + blockIdx.x];
+  run[t] = 0;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsItems; ++r) {
+    const int64_t i = base + r * kT + t;
+    const bool valid = i < n;
+    const uint64_t k = valid ? kin[i] : 0ull;
+    const uint32_t v = valid ? vin[i] : 0u;
+    const uint32_t d = valid ? (uint32_t)((k >> shift) & 255u) : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int bit = (d >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt);
+    const int cnt = __popcll(peers);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wcnt[w][lane * 4 + j] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wcnt[w][d] = cnt;
+    __syncthreads();
+    {
+      int acc = run[t];
+#pragma unroll
+      for (int ww = 0; ww < kT / 64; ++ww) {
+        wbase[ww][t] = acc;
+        acc += wcnt[ww][t];
+      }
+      run[t] = acc;
+    }
+    __syncthreads();
+    if (valid) {
+      const int64_t pos = gofs[d] + wbase[w][d] + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_check_dup(const uint64_t *__restrict__ key,
+                                                  int64_t n, int *flags) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x + 1;
+  if (i < n && key[i] == key[i - 1]) atomicOr(flags, kFlagDupKey);
+}
+
+}  // namespace
+
+int permute_particles(Ctx *c, const uint32_t *perm, int64_t n_out) {
+  if (n_out <= 0) return 0;
+  PermArgs a{};
+  a.nf = kNPersistent;
+  for (int q = 0; q < kNPersistent; ++q) {
+    const int f = kPersistent[q];
+    a.src[q] = c->f[f].p;
+    a.dst[q] = c->g[f].p;
+    a.esize[q] = (int)field_size(c, f);
+  }
+  k_permute<<<blocks_for(n_out), kT, 0, c->stream>>>(a, perm, n_out);
+  FSPH_HIP(c, hipGetLastError());
+  for (int q = 0; q < kNPersistent; ++q) std::swap(c->f[kPersistent[q]], c->g[kPersistent[q]]);
+  return 0;
+}
+
+int stage_boundary(Ctx *c) {
+  const int *per = c->p.periodic;
+  if (!(per[0] || per[1] || per[2])) return 0;
+  // pboundary_clean (boundary.h:46-70): stable removal of WALL copies.
+  if (c->n > c->n_real) {
+    FSPH_TRY(ensure(c, c->perm, (size_t)c->pcap * sizeof(uint32_t)));
+    int32_t *keep = c->f[PF_NNGB].as<int32_t>();  // scratch
+    FSPH_TRY(ensure(c, c->pair_off, (size_t)(c->n + 1) * sizeof(int64_t)));
+    int64_t *off = c->pair_off.as<int64_t>();
+    k_keep_real<<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_ID].as<int64_t>(), c->n, c->n_real, keep);
+    FSPH_TRY(scan_i32_to_i64(c, keep, off, c->n));
+    k_compact_index<<<blocks_for(c->n), kT, 0, c->stream>>>(
+        keep, off, c->n, c->perm.as<uint32_t>());
+    FSPH_HIP(c, hipGetLastError());
+    FSPH_TRY(permute_particles(c, c->perm.as<uint32_t>(), c->n_real));
+    c->n = c->n_real;
+  }
+  // getSmoothinglength (bodies_system.h:112-124): halo = 2.5 h_max.
+  double hmax = 0;
+  FSPH_TRY(stage_max_h(c, &hmax));
+  const double halo = 2.5 * hmax;
+  Box b{};
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = -0.5 * c->p.box[d];
+    b.hi[d] = 0.5 * c->p.box[d];
+    b.per[d] = per[d];
+  }
+  const int64_t n = c->n;
+  k_teleport<<<blocks_for(n), kT, 0, c->stream>>>(
+      c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), n, b);
+  int32_t *cnt = c->f[PF_NNGB].as<int32_t>();
+  k_count_copies<<<blocks_for(n), kT, 0, c->stream>>>(
+      c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), n, b, halo, cnt);
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(ensure(c, c->pair_off, (size_t)(n + 1) * sizeof(int64_t)));
+  int64_t *off = c->pair_off.as<int64_t>();
+  FSPH_TRY(scan_i32_to_i64(c, cnt, off, n));
+  int64_t extra = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&extra, off + n, sizeof(int64_t),
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  if (n + extra > c->pcap) {
+    // keep the offsets: they live in pair_off, which ensure_particles
+    // does not touch.
+    FSPH_TRY(ensure_particles(c, (int64_t)((n + extra) * 1.25) + 1024, true));
+  }
+  GenArgs a{};
+  a.nf = kNPersistent;
+  for (int q = 0; q < kNPersistent; ++q) {
+    a.f[q] = c->f[kPersistent[q]].p;
+    a.esize[q] = (int)field_size(c, kPersistent[q]);
+  }
+  k_gen_copies<<<blocks_for(n), kT, 0, c->stream>>>(
+      a, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), c->f[PF_ID].as<int64_t>(), off, n, c->n_real,
+      b, halo);
+  FSPH_HIP(c, hipGetLastError());
+  c->n = n + extra;
+  return 0;
+}
+
+int stage_range_keys(Ctx *c) {
+  const int64_t n = c->n;
+  const int nb = (int)std::min<int64_t>(1024, (n + kT - 1) / kT);
+  FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 6 * sizeof(double)));
+  double *part = c->red_tmp.as<double>();
+  k_range_partial<<<nb, kT, 0, c->stream>>>(
+      c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(), n, part);
+  k_range_final<<<1, kT, 0, c->stream>>>(part, nb);
+  FSPH_TRY(ensure(c, c->key, (size_t)c->pcap * 8));
+  FSPH_TRY(ensure(c, c->val, (size_t)c->pcap * 4));
+  k_keys<<<blocks_for(n), kT, 0, c->stream>>>(
+      c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), n, part + nb * 6, c->key.as<uint64_t>(),
+      c->val.as<uint32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  double r[6];
+  FSPH_HIP(c, hipMemcpyAsync(r, part + nb * 6, sizeof(r), hipMemcpyDeviceToHost,
+                             c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; ++d) {
+    c->lo[d] = r[d];
+    c->hi[d] = r[3 + d];
+  }
+  return 0;
+}
+
+int stage_sort(Ctx *c) {
+  const int64_t n = c->n;
+  const int64_t nb = (n + kRsTile - 1) / kRsTile;
+  FSPH_TRY(ensure(c, c->key2, (size_t)c->pcap * 8));
+  FSPH_TRY(ensure(c, c->val2, (size_t)c->pcap * 4));
+  FSPH_TRY(ensure(c, c->hist, (size_t)256 * nb * sizeof(int32_t) +
+                                  (size_t)(256 * nb + 1) * sizeof(int64_t)));
+  int32_t *hist = c->hist.as<int32_t>();
+  int64_t *hoff = reinterpret_cast<int64_t *>(
+      c->hist.as<char>() + (((size_t)256 * nb * sizeof(int32_t) + 7) & ~(size_t)7));
+  uint64_t *k0 = c->key.as<uint64_t>(), *k1 = c->key2.as<uint64_t>();
+  uint32_t *v0 = c->val.as<uint32_t>(), *v1 = c->val2.as<uint32_t>();
+  for (int pass = 0; pass < 8; ++pass) {
+    const int shift = pass * 8;
+    k_radix_hist<<<(unsigned)nb, kT, 0, c->stream>>>(k0, n, shift, hist, nb);
+    FSPH_TRY(scan_i32_to_i64(c, hist, hoff, 256 * nb));
+    k_radix_scatter<<<(unsigned)nb, kT, 0, c->stream>>>(k0, v0, k1, v1, n,
+                                                        shift, hoff, nb);
+    FSPH_HIP(c, hipGetLastError());
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  // 8 passes: the sorted data is back in key/val.
+  k_check_dup<<<blocks_for(n), kT, 0, c->stream>>>(k0, n, c->flags.as<int>());
+  FSPH_HIP(c, hipGetLastError());
+  return permute_particles(c, v0, n);
+}
+
+}  // namespace fsph

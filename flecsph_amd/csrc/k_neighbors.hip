@@ -1,0 +1,306 @@
+// k_neighbors.hip -- neighbour tiles of every SPH group.
+//
+// tree_topology::traversal_sph (tree_topology.h:213-442) visits, for each
+// group G, every node whose ancestors all pass two tests against G: the
+// node box intersects G's box (intersects_box_box), and the node box is
+// within h_k of some particle k of G (intersects_sphere_box).  Leaves under
+// accepted nodes are then kept if they pass a centre-of-mass filter and
+// d(k,b) <= max(h_k, h_b).  Here one wavefront per group walks the octree
+// with an LDS stack, 64 nodes per step (lanes over nodes, FP64 predicates
+// in the reference's operation order, no FMA contraction), and appends
+// reached leaves to the group's candidate tile.  A second kernel computes,
+// for every (group particle, candidate) pair, the reference's exact
+// neighbour predicate and stores it as a bitmask row per particle:
+// mask[moff[g] + w*gcount + k] bit j <=> candidate 32w+j neighbours k.
+// The SPH passes stage candidates in LDS and walk these bits.
+#include "fsph_internal.h"
+
+namespace fsph {
+
+namespace {
+
+constexpr int kW = 64;          // one wavefront per group
+constexpr int kStack = 4096;    // LDS traversal stack (node indices)
+constexpr int kMaxGroup = 128;  // tree_topology::sub_entities_
+
+__device__ inline bool box_box(const double *a, const double *b) {
+  // intersects_box_box(min1,max1,min2,max2), tree_geometry.h:306-314
+  return !((a[3] < b[0] || a[4] < b[1] || a[5] < b[2]) ||
+           (b[3] < a[0] || b[4] < a[1] || b[5] < a[2]));
+}
+
+__device__ inline bool sphere_box(const double *bx, double c0, double c1,
+                                  double c2, double r) {
+  // intersects_sphere_box, tree_geometry.h:329-341
+#pragma clang fp contract(off)
+  double x0 = c0 < bx[3] ? c0 : bx[3];
+  double x1 = c1 < bx[4] ? c1 : bx[4];
+  double x2 = c2 < bx[5] ? c2 : bx[5];
+  x0 = x0 < bx[0] ? bx[0] : x0;
+  x1 = x1 < bx[1] ? bx[1] : x1;
+  x2 = x2 < bx[2] ? bx[2] : x2;
+  const double dist =
+      (x0 - c0) * (x0 - c0) + (x1 - c1) * (x1 - c1) + (x2 - c2) * (x2 - c2);
+  return dist <= r * r;
+}
+
+__device__ inline bool within_distance2(double a0, double a1, double a2,
+                                        double b0, double b1, double b2,
+                                        double r) {
+  // within_distance2, tree_geometry.h:272-276
+#pragma clang fp contract(off)
+  const double d =
+      (a0 - b0) * (a0 - b0) + (a1 - b1) * (a1 - b1) + (a2 - b2) * (a2 - b2);
+  return d <= r * r;
+}
+
+__device__ inline int wave_excl_scan(int v, int *total) {
+  const int lane = threadIdx.x & 63;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  *total = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+struct TravArgs {
+  const double *x, *y, *z, *h;
+  const double *box;     // 6 per node
+  const int32_t *child;  // 8 per node
+  const int32_t *gstart, *gcount, *gnode;
+  int32_t *cand;
+  int32_t *ccount;
+  int64_t ccap;
+  int *flags;
+};
+
+__global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
+  __shared__ double sx[kMaxGroup], sy[kMaxGroup], sz[kMaxGroup],
+      sh[kMaxGroup];
+  __shared__ int32_t stk[kStack];
+  const int lane = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  const int64_t gs = a.gstart[g];
+  const int gc = a.gcount[g];
+  const int gn = a.gnode[g];
+  for (int k = lane; k < gc; k += kW) {
+    sx[k] = a.x[gs + k];
+    sy[k] = a.y[gs + k];
+    sz[k] = a.z[gs + k];
+    sh[k] = a.h[gs + k];
+  }
+  double gb[6];
+  if (gn >= 0)
+    for (int d = 0; d < 6; ++d) gb[d] = a.box[6 * (int64_t)gn + d];
+  if (lane == 0) stk[0] = 0;  // root
+  __syncthreads();
+  int sp = 1;
+  int64_t ncand = 0;
+  const int64_t cbase = g * a.ccap;
+  bool overflow = false;
+  while (sp > 0) {
+    const int nb = sp < kW ? sp : kW;
+    sp -= nb;
+    const int node = lane < nb ? stk[sp + lane] : -1;
+    __syncthreads();
+    bool acc = false;
+    int32_t ch[8];
+    if (node >= 0) {
+      double bx[6];
+      for (int d = 0; d < 6; ++d) bx[d] = a.box[6 * (int64_t)node + d];
+      if (gn < 0 || box_box(bx, gb)) {
+        for (int k = 0; k < gc; ++k)
+          if (sphere_box(bx, sx[k], sy[k], sz[k], sh[k])) {
+            acc = true;
+            break;
+          }
+      }
+      if (acc)
+        for (int j = 0; j < 8; ++j) ch[j] = a.child[8 * (int64_t)node + j];
+    }
+    int nnode = 0, nleaf = 0;
+    if (acc)
+      for (int j = 0; j < 8; ++j) {
+        nnode += ch[j] >= 0;
+        nleaf += ch[j] <= -2;
+      }
+    int tn, tl;
+    const int pn = wave_excl_scan(nnode, &tn);
+    const int pl = wave_excl_scan(nleaf, &tl);
+    if (sp + tn > kStack) {
+      overflow = true;
+      if (lane == 0) atomicOr(a.flags, kFlagStack);
+      break;
+    }
+    if (ncand + tl > a.ccap) {
+      overflow = true;
+      if (lane == 0) atomicOr(a.flags, kFlagCand);
+      break;
+    }
+    if (acc) {
+      int qn = sp + pn;
+      int64_t ql = cbase + ncand + pl;
+      for (int j = 0; j < 8; ++j) {
+        if (ch[j] >= 0)
+          stk[qn++] = ch[j];
+        else if (ch[j] <= -2)
+          a.cand[ql++] = (int32_t)leaf_of(ch[j]);
+      }
+    }
+    sp += tn;
+    ncand += tl;
+    __syncthreads();
+  }
+  if (lane == 0) a.ccount[g] = overflow ? 0 : (int32_t)ncand;
+}
+
+__global__ __launch_bounds__(256) void k_mask_words(
+    const int32_t *__restrict__ ccount, const int32_t *__restrict__ gcount,
+    int64_t ng, int64_t *__restrict__ words) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g < ng) words[g] = (int64_t)gcount[g] * ((ccount[g] + 31) / 32);
+}
+
+struct MaskArgs {
+  const double *x, *y, *z, *h;
+  const double *com, *rad, *lap;
+  const int32_t *gstart, *gcount, *gnode;
+  const int32_t *cand, *ccount;
+  const int64_t *moff;
+  int64_t ccap;
+  uint32_t *mask;
+  int32_t *nngb;
+};
+
+__global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
+  __shared__ double cx[32], cy[32], cz[32], chh[32];
+  __shared__ int cok[32];
+  const int lane = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  const int64_t gs = a.gstart[g];
+  const int gc = a.gcount[g];
+  const int gn = a.gnode[g];
+  const int cc = a.ccount[g];
+  const int64_t mo = a.moff[g];
+  const int64_t cbase = g * a.ccap;
+  double px[2], py[2], pz[2], ph[2];
+  for (int s = 0; s < 2; ++s) {
+    const int k = lane + kW * s;
+    if (k < gc) {
+      px[s] = a.x[gs + k];
+      py[s] = a.y[gs + k];
+      pz[s] = a.z[gs + k];
+      ph[s] = a.h[gs + k];
+    }
+  }
+  double gcom[3] = {0, 0, 0}, gext_lap = 0, grad = 0;
+  if (gn >= 0) {
+    gcom[0] = a.com[3 * (int64_t)gn];
+    gcom[1] = a.com[3 * (int64_t)gn + 1];
+    gcom[2] = a.com[3 * (int64_t)gn + 2];
+    gext_lap = a.lap[gn];
+    grad = a.rad[gn];
+  }
+  int cnt[2] = {0, 0};
+  const int nw = (cc + 31) / 32;
+  for (int w = 0; w < nw; ++w) {
+    if (lane < 32) {
+      const int j = w * 32 + lane;
+      int ok = 0;
+      if (j < cc) {
+        const int64_t b = a.cand[cbase + j];
+        const double bx = a.x[b], by = a.y[b], bz = a.z[b], bh = a.h[b];
+        cx[lane] = bx;
+        cy[lane] = by;
+        cz[lane] = bz;
+        chh[lane] = bh;
+        ok = 1;
+        if (gn >= 0) {
+          // COM filter, tree_topology.h:365-371
+          const double ext = ((bh < gext_lap) ? gext_lap : bh) + grad;
+          ok = within_distance2(bx, by, bz, gcom[0], gcom[1], gcom[2], ext);
+        }
+      }
+      cok[lane] = ok;
+    }
+    __syncthreads();
+    for (int s = 0; s < 2; ++s) {
+      const int k = lane + kW * s;
+      if (k >= gc) break;
+      uint32_t word = 0;
+      for (int j = 0; j < 32; ++j) {
+        if (!cok[j]) continue;
+        // tree_topology.h:372-379: d^2 <= max(h_k, h_b)^2
+        const double r = (ph[s] < chh[j]) ? chh[j] : ph[s];
+        if (within_distance2(px[s], py[s], pz[s], cx[j], cy[j], cz[j], r))
+          word |= 1u << j;
+      }
+      a.mask[mo + (int64_t)w * gc + k] = word;
+      cnt[s] += __popc(word);
+    }
+    __syncthreads();
+  }
+  for (int s = 0; s < 2; ++s) {
+    const int k = lane + kW * s;
+    if (k < gc) a.nngb[gs + k] = cnt[s];
+  }
+}
+
+}  // namespace
+
+int stage_neighbors(Ctx *c) {
+  const int64_t ng = c->ng;
+  FSPH_TRY(ensure(c, c->ccount, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, c->moff, (size_t)(ng + 1) * 8));
+  for (int attempt = 0;; ++attempt) {
+    FSPH_TRY(ensure(c, c->cand, (size_t)ng * c->ccap * 4));
+    FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+    TravArgs t{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+               c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
+               c->nbox.as<double>(),    c->nchild.as<int32_t>(),
+               c->gstart.as<int32_t>(), c->gcount.as<int32_t>(),
+               c->gnode.as<int32_t>(),  c->cand.as<int32_t>(),
+               c->ccount.as<int32_t>(), c->ccap,
+               c->flags.as<int>()};
+    k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
+    FSPH_HIP(c, hipGetLastError());
+    int fl = 0;
+    FSPH_TRY(read_flags(c, &fl));
+    if (fl & kFlagStack)
+      return fail(c, FSPH_E_CAP, "neighbour traversal stack overflow");
+    if (fl & kFlagCand) {
+      if (attempt >= 6)
+        return fail(c, FSPH_E_CAP, "candidate tile overflow (cap %lld)",
+                    (long long)c->ccap);
+      c->ccap *= 2;
+      continue;
+    }
+    break;
+  }
+  k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
+      c->ccount.as<int32_t>(), c->gcount.as<int32_t>(), ng,
+      c->moff.as<int64_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i64(c, c->moff.as<int64_t>(), c->moff.as<int64_t>(), ng));
+  int64_t words = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&words, c->moff.as<int64_t>() + ng, 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->mask_words = words;
+  FSPH_TRY(ensure(c, c->mask, (size_t)(words > 0 ? words : 1) * 4));
+  MaskArgs m{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+             c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
+             c->ncom.as<double>(),    c->nrad.as<double>(),
+             c->nlap.as<double>(),    c->gstart.as<int32_t>(),
+             c->gcount.as<int32_t>(), c->gnode.as<int32_t>(),
+             c->cand.as<int32_t>(),   c->ccount.as<int32_t>(),
+             c->moff.as<int64_t>(),   c->ccap,
+             c->mask.as<uint32_t>(),  c->f[PF_NNGB].as<int32_t>()};
+  k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
+  return hip_check(c, hipGetLastError(), "k_masks");
+}
+
+}  // namespace fsph

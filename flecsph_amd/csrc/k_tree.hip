@@ -1,0 +1,407 @@
+// k_tree.hip -- device octree of the sorted keys and its SPH groups.
+//
+// The reference inserts particles one by one into a hash table
+// (tree_topology.h:879-997); the resulting tree has one node per key prefix
+// shared by at least two particles, and each particle sits as a leaf at its
+// shortest prefix shared with neither sorted neighbour.  Here the same tree
+// is built in parallel from the common-prefix depth L[i] of adjacent sorted
+// keys: adjacent pair i "owns" the nodes of depths L[i-1]+1 .. L[i] that
+// start at particle i, which numbers every node exactly once, parents
+// before children.  Node aggregates (physics::compute_cofm,
+// default_physics.h:60-119) are then computed level by level, deepest
+// first, in FP64 with the reference's operation order.
+#include <cfloat>
+
+#include "fsph_internal.h"
+
+namespace fsph {
+
+namespace {
+
+constexpr int kT = 256;
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+__device__ inline int digit_at(uint64_t key, int depth) {
+  return (int)((key >> (3 * (kMaxDepth - depth))) & 7u);
+}
+
+__device__ inline int32_t Lm1(const int32_t *L, int64_t i) {
+  return i > 0 ? L[i - 1] : -1;
+}
+
+// First sorted index whose key is >= v.
+__device__ inline int64_t lower_bound(const uint64_t *__restrict__ k, int64_t n,
+                                      uint64_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (k[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Node index of the depth-D node whose key range starts at sorted index s.
+__device__ inline int64_t node_id(const int64_t *__restrict__ off,
+                                  const int32_t *__restrict__ L, int64_t s,
+                                  int D) {
+  return off[s] + D - (Lm1(L, s) + 1);
+}
+
+// Node index of the depth-D node containing sorted particle i.
+__device__ inline int64_t containing_node(const uint64_t *__restrict__ k,
+                                          int64_t n,
+                                          const int64_t *__restrict__ off,
+                                          const int32_t *__restrict__ L,
+                                          int64_t i, int D) {
+  const int sh = 3 * (kMaxDepth - D);
+  const uint64_t first = (D == 0) ? (1ull << 63) : ((k[i] >> sh) << sh);
+  return node_id(off, L, lower_bound(k, n, first), D);
+}
+
+__global__ __launch_bounds__(kT) void k_prefix(const uint64_t *__restrict__ k,
+                                               int64_t n, int32_t *__restrict__ L) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  if (i == n - 1) {
+    L[i] = -1;
+    return;
+  }
+  const uint64_t x = k[i] ^ k[i + 1];
+  const int lz = x ? __clzll(x) : 64;
+  int d = (lz - 1) / 3;  // common digits below the leading 1
+  L[i] = d > kMaxDepth - 1 ? kMaxDepth - 1 : d;
+}
+
+__global__ __launch_bounds__(kT) void k_pair_count(const int32_t *__restrict__ L,
+                                                   int64_t n,
+                                                   int32_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int v = L[i] - Lm1(L, i);
+  cnt[i] = v > 0 ? v : 0;
+}
+
+struct NodeOut {
+  uint64_t *key;
+  int32_t *depth, *start, *count, *parent, *child;
+};
+
+__global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
+                                              int64_t n,
+                                              const int32_t *__restrict__ L,
+                                              const int64_t *__restrict__ off,
+                                              NodeOut o) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int d0 = Lm1(L, i) + 1, d1 = L[i];
+  for (int d = d0; d <= d1; ++d) {
+    const int64_t id = off[i] + (d - d0);
+    const int sh = 3 * (kMaxDepth - d);
+    const uint64_t prefix = (d == 0) ? 1ull : (k[i] >> sh);
+    o.key[id] = prefix;
+    o.depth[id] = d;
+    o.start[id] = (int32_t)i;
+    // last index with the same prefix
+    const uint64_t last_key =
+        (d == 0) ? ~0ull : ((prefix << sh) | ((1ull << sh) - 1ull));
+    int64_t lo = i, hi = n;  // first index with key > last_key
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (k[mid] <= last_key)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    o.count[id] = (int32_t)(lo - i);
+    int64_t par = -1;
+    if (d > 0) {
+      par = (d - 1 >= d0) ? id - 1 : containing_node(k, n, off, L, i, d - 1);
+      o.child[par * 8 + digit_at(k[i], d)] = (int32_t)id;
+    }
+    o.parent[id] = (int32_t)par;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_leaves(const uint64_t *__restrict__ k,
+                                               int64_t n,
+                                               const int32_t *__restrict__ L,
+                                               const int64_t *__restrict__ off,
+                                               int32_t *__restrict__ child,
+                                               int32_t *__restrict__ leaf_parent) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int a = Lm1(L, i), b = L[i];
+  const int D = a > b ? a : b;
+  const int64_t par = (b > a) ? off[i] + b - (a + 1)
+                              : containing_node(k, n, off, L, i, D);
+  child[par * 8 + digit_at(k[i], D + 1)] = leaf_code(i);
+  leaf_parent[i] = (int32_t)par;
+}
+
+__global__ __launch_bounds__(kT) void k_level_hist(const int32_t *__restrict__ depth,
+                                                   int64_t nn,
+                                                   int64_t *__restrict__ lvl) {
+  __shared__ int32_t h[kMaxDepth + 1];
+  if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nn) atomicAdd(&h[depth[i]], 1);
+  __syncthreads();
+  if (threadIdx.x <= kMaxDepth && h[threadIdx.x])
+    atomicAdd((unsigned long long *)&lvl[threadIdx.x],
+              (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kT) void k_level_scatter(
+    const int32_t *__restrict__ depth, int64_t nn,
+    unsigned long long *__restrict__ cursor, int32_t *__restrict__ order) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nn) return;
+  const unsigned long long pos = atomicAdd(&cursor[depth[i]], 1ull);
+  order[pos] = (int32_t)i;
+}
+
+struct CofmArgs {
+  const double *x, *y, *z, *h, *m;
+  const int32_t *child;
+  double *box;  // 6 per node: bmin xyz, bmax xyz
+  double *com;  // 3 per node
+  double *mass, *rad, *lap;
+};
+
+// physics::compute_cofm (default_physics.h:60-119): leaf children first,
+// then node children, each in digit order (tree_topology.h:1949-1966).
+__global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
+                                             const int32_t *__restrict__ ids,
+                                             int64_t cnt) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= cnt) return;
+  const int64_t nd = ids[t];
+  int32_t ch[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ch[j] = a.child[nd * 8 + j];
+  double cx = 0, cy = 0, cz = 0, mass = 0;
+  double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int j = 0; j < 8; ++j) {
+    if (ch[j] > -2) continue;
+    const int64_t p = leaf_of(ch[j]);
+    const double m = a.m[p], X[3] = {a.x[p], a.y[p], a.z[p]}, hh = a.h[p];
+    cx += m * X[0];
+    cy += m * X[1];
+    cz += m * X[2];
+    mass += m;
+    for (int d = 0; d < 3; ++d) {
+      const double lo = X[d] - hh / 2., hi = X[d] + hh / 2.;
+      bmin[d] = (lo < bmin[d]) ? lo : bmin[d];
+      bmax[d] = (bmax[d] < hi) ? hi : bmax[d];
+    }
+  }
+  for (int j = 0; j < 8; ++j) {
+    const int32_t c = ch[j];
+    if (c < 0) continue;
+    const double m = a.mass[c];
+    cx += m * a.com[3 * c];
+    cy += m * a.com[3 * c + 1];
+    cz += m * a.com[3 * c + 2];
+    mass += m;
+    for (int d = 0; d < 3; ++d) {
+      const double lo = a.box[6 * c + d], hi = a.box[6 * c + 3 + d];
+      bmin[d] = (lo < bmin[d]) ? lo : bmin[d];
+      bmax[d] = (bmax[d] < hi) ? hi : bmax[d];
+    }
+  }
+  cx /= mass;
+  cy /= mass;
+  cz /= mass;
+  double radius = 0, lap = 0;
+  for (int j = 0; j < 8; ++j) {
+    if (ch[j] > -2) continue;
+    const int64_t p = leaf_of(ch[j]);
+    const double dx = cx - a.x[p], dy = cy - a.y[p], dz = cz - a.z[p];
+    double s = 0;
+    s += dx * dx;
+    s += dy * dy;
+    s += dz * dz;
+    const double dist = sqrt(s);
+    radius = (radius < dist) ? dist : radius;
+    const double l = dist + a.h[p];
+    lap = (lap < l) ? l : lap;
+  }
+  for (int j = 0; j < 8; ++j) {
+    const int32_t c = ch[j];
+    if (c < 0) continue;
+    const double dx = cx - a.com[3 * c], dy = cy - a.com[3 * c + 1],
+                 dz = cz - a.com[3 * c + 2];
+    double s = 0;
+    s += dx * dx;
+    s += dy * dy;
+    s += dz * dz;
+    const double dist = sqrt(s);
+    const double r = dist + a.rad[c];
+    radius = (radius < r) ? r : radius;
+    const double l = dist + a.rad[c] + a.lap[c];
+    lap = (lap < l) ? l : lap;
+  }
+  a.com[3 * nd] = cx;
+  a.com[3 * nd + 1] = cy;
+  a.com[3 * nd + 2] = cz;
+  a.mass[nd] = mass;
+  a.rad[nd] = radius;
+  a.lap[nd] = lap;
+  for (int d = 0; d < 3; ++d) {
+    a.box[6 * nd + d] = bmin[d];
+    a.box[6 * nd + 3 + d] = bmax[d];
+  }
+}
+
+// Groups (tree_topology.h:222-235): nodes with at most gmax particles whose
+// parent has more (or the root), and single leaves whose parent has more.
+__global__ __launch_bounds__(kT) void k_group_flags(
+    const int32_t *__restrict__ count, const int32_t *__restrict__ parent,
+    int64_t nn, const int32_t *__restrict__ leaf_parent, int64_t n, int gmax,
+    int32_t *__restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nn) {
+    const int32_t p = parent[i];
+    flag[i] = count[i] <= gmax && (p < 0 || count[p] > gmax);
+  } else if (i < nn + n) {
+    flag[i] = count[leaf_parent[i - nn]] > gmax;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_group_write(
+    const int32_t *__restrict__ flag, const int64_t *__restrict__ pos,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+    int64_t nn, int64_t n, int32_t *__restrict__ gstart,
+    int32_t *__restrict__ gcount, int32_t *__restrict__ gnode) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nn + n || !flag[i]) return;
+  const int64_t g = pos[i];
+  if (i < nn) {
+    gstart[g] = start[i];
+    gcount[g] = count[i];
+    gnode[g] = (int32_t)i;
+  } else {
+    gstart[g] = (int32_t)(i - nn);
+    gcount[g] = 1;
+    gnode[g] = -1;
+  }
+}
+
+}  // namespace
+
+int stage_tree(Ctx *c) {
+  const int64_t n = c->n;
+  const uint64_t *k = c->key.as<uint64_t>();
+  FSPH_TRY(ensure(c, c->L, (size_t)c->pcap * 4));
+  FSPH_TRY(ensure(c, c->pair_cnt, (size_t)c->pcap * 4));
+  FSPH_TRY(ensure(c, c->pair_off, (size_t)(c->pcap + 1) * 8));
+  FSPH_TRY(ensure(c, c->leaf_parent, (size_t)c->pcap * 4));
+  int32_t *L = c->L.as<int32_t>();
+  k_prefix<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L);
+  k_pair_count<<<blocks_for(n), kT, 0, c->stream>>>(L, n,
+                                                    c->pair_cnt.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  int64_t *off = c->pair_off.as<int64_t>();
+  FSPH_TRY(scan_i32_to_i64(c, c->pair_cnt.as<int32_t>(), off, n));
+  int64_t nn = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&nn, off + n, 8, hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->nn = nn;
+  if (nn > c->ncap) {
+    const int64_t cap = (int64_t)(nn * 1.25) + 1024;
+    FSPH_TRY(ensure(c, c->nkey, cap * 8));
+    FSPH_TRY(ensure(c, c->ndepth, cap * 4));
+    FSPH_TRY(ensure(c, c->nstart, cap * 4));
+    FSPH_TRY(ensure(c, c->ncount, cap * 4));
+    FSPH_TRY(ensure(c, c->nparent, cap * 4));
+    FSPH_TRY(ensure(c, c->nchild, cap * 32));
+    FSPH_TRY(ensure(c, c->nbox, cap * 48));
+    FSPH_TRY(ensure(c, c->ncom, cap * 24));
+    FSPH_TRY(ensure(c, c->nmass, cap * 8));
+    FSPH_TRY(ensure(c, c->nrad, cap * 8));
+    FSPH_TRY(ensure(c, c->nlap, cap * 8));
+    FSPH_TRY(ensure(c, c->nlevel, cap * 4));
+    c->ncap = cap;
+  }
+  FSPH_HIP(c, hipMemsetAsync(c->nchild.p, 0xff, (size_t)nn * 32, c->stream));
+  NodeOut o{c->nkey.as<uint64_t>(), c->ndepth.as<int32_t>(),
+            c->nstart.as<int32_t>(), c->ncount.as<int32_t>(),
+            c->nparent.as<int32_t>(), c->nchild.as<int32_t>()};
+  k_nodes<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L, off, o);
+  k_leaves<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L, off,
+                                                c->nchild.as<int32_t>(),
+                                                c->leaf_parent.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  // level order
+  FSPH_TRY(ensure(c, c->red_tmp, 2 * (kMaxDepth + 2) * 8));
+  int64_t *lvl = c->red_tmp.as<int64_t>();
+  FSPH_HIP(c, hipMemsetAsync(lvl, 0, (kMaxDepth + 2) * 8, c->stream));
+  k_level_hist<<<blocks_for(nn), kT, 0, c->stream>>>(c->ndepth.as<int32_t>(),
+                                                     nn, lvl);
+  int64_t cnt[kMaxDepth + 1];
+  FSPH_HIP(c, hipMemcpyAsync(cnt, lvl, sizeof(cnt), hipMemcpyDeviceToHost,
+                             c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  int64_t acc = 0;
+  c->max_depth = 0;
+  for (int d = 0; d <= kMaxDepth; ++d) {
+    c->level_off[d] = acc;
+    acc += cnt[d];
+    if (cnt[d]) c->max_depth = d;
+  }
+  c->level_off[kMaxDepth + 1] = acc;
+  unsigned long long *cursor =
+      reinterpret_cast<unsigned long long *>(lvl + (kMaxDepth + 2));
+  FSPH_HIP(c, hipMemcpyAsync(cursor, c->level_off, (kMaxDepth + 1) * 8,
+                             hipMemcpyHostToDevice, c->stream));
+  k_level_scatter<<<blocks_for(nn), kT, 0, c->stream>>>(
+      c->ndepth.as<int32_t>(), nn, cursor, c->nlevel.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  CofmArgs a{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+             c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
+             c->f[PF_M].as<double>(), c->nchild.as<int32_t>(),
+             c->nbox.as<double>(),    c->ncom.as<double>(),
+             c->nmass.as<double>(),   c->nrad.as<double>(),
+             c->nlap.as<double>()};
+  for (int d = c->max_depth; d >= 0; --d) {
+    const int64_t cntd = c->level_off[d + 1] - c->level_off[d];
+    if (!cntd) continue;
+    k_cofm<<<blocks_for(cntd), kT, 0, c->stream>>>(
+        a, c->nlevel.as<int32_t>() + c->level_off[d], cntd);
+  }
+  return hip_check(c, hipGetLastError(), "k_cofm");
+}
+
+int stage_groups(Ctx *c) {
+  const int64_t n = c->n, nn = c->nn;
+  const int64_t tot = nn + n;
+  FSPH_TRY(ensure(c, c->gflag, (size_t)tot * 4));
+  FSPH_TRY(ensure(c, c->gscan, (size_t)(tot + 1) * 8));
+  k_group_flags<<<blocks_for(tot), kT, 0, c->stream>>>(
+      c->ncount.as<int32_t>(), c->nparent.as<int32_t>(), nn,
+      c->leaf_parent.as<int32_t>(), n, c->p.group_max, c->gflag.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i32_to_i64(c, c->gflag.as<int32_t>(), c->gscan.as<int64_t>(),
+                           tot));
+  int64_t ng = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&ng, c->gscan.as<int64_t>() + tot, 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->ng = ng;
+  FSPH_TRY(ensure(c, c->gstart, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, c->gcount, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, c->gnode, (size_t)ng * 4));
+  k_group_write<<<blocks_for(tot), kT, 0, c->stream>>>(
+      c->gflag.as<int32_t>(), c->gscan.as<int64_t>(), c->nstart.as<int32_t>(),
+      c->ncount.as<int32_t>(), nn, n, c->gstart.as<int32_t>(),
+      c->gcount.as<int32_t>(), c->gnode.as<int32_t>());
+  return hip_check(c, hipGetLastError(), "k_group_write");
+}
+
+}  // namespace fsph

@@ -1,0 +1,252 @@
+/*
+ * fsph.h -- C ABI of the MI355X (gfx950) per-step SPH particle pipeline.
+ *
+ * One fsph_ctx holds one MPI rank's share of particles (the state that
+ * flecsi::topology::tree_topology::entities_ holds in the reference) as
+ * device-resident structure-of-arrays, plus the octree, the SPH groups and
+ * the neighbour tiles built from it.  Every entry point below replaces one
+ * call of the reference's body_system / physics:: surface; the citation in
+ * each comment names the reference function (file:line) whose effect the
+ * call reproduces.
+ *
+ * Conventions
+ *   - Every function returns FSPH_OK (0) or a negative FSPH_E_* code; the
+ *     message of the last failure is fsph_last_error(ctx).  No C++ exception
+ *     crosses this boundary.
+ *   - The caller owns every host buffer; the context owns all device memory.
+ *   - All work of a context is ordered on the context's own HIP stream and
+ *     every call returns after that work has completed.
+ *   - A context is not thread-safe: one host thread per context, as the
+ *     reference runs one thread per MPI rank.
+ *   - Particle arrays are in the context's current order.  After
+ *     fsph_update_iteration that order is ascending (Morton key, id), the
+ *     order psort leaves bodies in (mpisph/psort.h:247-253).  Periodic WALL
+ *     copies (include/physics/boundary.h:121-250) follow the real particles
+ *     in that sorted order and are told apart by id >= n_real.
+ */
+#ifndef FSPH_H
+#define FSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define FSPH_OK 0
+#define FSPH_E_ARG (-1)     /* bad argument or call order                   */
+#define FSPH_E_HIP (-2)     /* HIP runtime error                            */
+#define FSPH_E_OOM (-3)     /* device allocation failed                     */
+#define FSPH_E_DUPKEY (-4)  /* two particles share a Morton key; the
+                               reference's build_tree has undefined
+                               behaviour here (tree_topology.h:957-971)     */
+#define FSPH_E_NAN (-5)     /* NaN in state (default_physics.h:919-945)     */
+#define FSPH_E_NEG (-6)     /* non-positive density/energy/h
+                               (default_physics.h:950-979)                  */
+#define FSPH_E_CAP (-7)     /* an internal capacity was exceeded            */
+#define FSPH_E_NODEV (-8)   /* no HIP device / library built without one    */
+
+/* ---- parameter keywords (include/params.h:128-152) --------------------- */
+#define FSPH_KERNEL_CUBIC_SPLINE 0 /* kernels.h:106-153                    */
+#define FSPH_KERNEL_WENDLAND_C6 1  /* kernels.h:502-566                    */
+
+#define FSPH_EOS_IDEAL 0           /* eos.h:148-209                        */
+#define FSPH_EOS_POLYTROPIC 1      /* eos.h:57-145                         */
+
+#define FSPH_PREC_FP32 0 /* pair sums in FP32 with compensated summation    */
+#define FSPH_PREC_FP64 1 /* pair sums in FP64 (the reference's arithmetic)  */
+
+#define FSPH_GRAV_NONE 0
+#define FSPH_GRAV_FMM 1  /* body_system::gravitation_fmm, bodies_system.h:282 */
+
+/* Parameters of one context.  Field names follow the reference's param::
+   names (include/params.h) where one exists. */
+typedef struct fsph_params {
+  int32_t kernel;      /* sph_kernel                                        */
+  int32_t eos;         /* eos_type                                          */
+  int32_t precision;   /* FSPH_PREC_*                                       */
+  int32_t adaptive_dt; /* adaptive_timestep                                 */
+  int32_t gravity;     /* enable_fmm                                        */
+  int32_t group_max;   /* tree_topology::sub_entities_ (128)                */
+  int32_t leaf_size;   /* FMM leaf bucket size                              */
+  int32_t periodic[3]; /* periodic_boundary_x/y/z                           */
+  int32_t n_target;    /* target neighbour count (smoothing-length solvers) */
+  int32_t pad0;
+  double gamma;        /* poly_gamma                                        */
+  double poly_K;       /* adiabatic constant of the polytropic EOS          */
+  double eta;          /* sph_eta                                           */
+  double visc_alpha;   /* sph_viscosity_alpha                               */
+  double visc_beta;    /* sph_viscosity_beta                                */
+  double visc_eps;     /* sph_viscosity_epsilon                             */
+  double cfl;          /* timestep_cfl_factor                               */
+  double initial_dt;   /* initial_dt                                        */
+  double G;            /* gravitational_constant                            */
+  double theta;        /* fmm_macangle                                      */
+  double box[3];       /* box_length, box_width, box_height (periodic box
+                          centred on the origin, boundary.h:134-144)        */
+} fsph_params;
+
+/* Fills p with the reference's defaults (params.h DECLARE_PARAM values,
+   kernel = cubic spline, eos = ideal, gamma 1.4, group_max 128). */
+void fsph_default_params(fsph_params *p);
+
+/* Host view of particle input, one array per field (SoA).  h is the
+   smoothing length in the reference's sense: the kernel support radius
+   (body_u::radius, body.h).  rho/P/c may be NULL (then zero). */
+typedef struct fsph_particles {
+  int64_t n;
+  const double *x, *y, *z;
+  const double *vx, *vy, *vz;
+  const double *u;
+  const double *h;
+  const double *m;
+  const int64_t *id;
+  const double *rho, *P, *c;
+} fsph_particles;
+
+/* Field selectors for fsph_download (each field is n doubles, except
+   FSPH_F_ID (int64) and FSPH_F_KEY (uint64)). */
+enum fsph_field {
+  FSPH_F_X = 0, FSPH_F_Y, FSPH_F_Z,
+  FSPH_F_VX, FSPH_F_VY, FSPH_F_VZ,
+  FSPH_F_VHX, FSPH_F_VHY, FSPH_F_VHZ,
+  FSPH_F_U, FSPH_F_H, FSPH_F_M,
+  FSPH_F_RHO, FSPH_F_P, FSPH_F_C, FSPH_F_VSIG,
+  FSPH_F_AX, FSPH_F_AY, FSPH_F_AZ,
+  FSPH_F_DUDT,
+  FSPH_F_GX, FSPH_F_GY, FSPH_F_GZ, FSPH_F_PHI,
+  FSPH_F_ID, FSPH_F_KEY, FSPH_F_NNGB,
+  FSPH_NFIELDS
+};
+
+typedef struct fsph_ctx fsph_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Creates a context on HIP device `device` (replaces constructing
+   body_system<double,3>, mpisph/bodies_system.h:38-90). */
+int fsph_create(const fsph_params *p, int device, fsph_ctx **out);
+void fsph_destroy(fsph_ctx *ctx);
+const char *fsph_last_error(const fsph_ctx *ctx);
+
+/* Copies host particles to the device (replaces body_system::read_bodies,
+   bodies_system.h:92-98, minus the HDF5 reader).  Resets time and dt. */
+int fsph_upload(fsph_ctx *ctx, const fsph_particles *parts);
+
+/* Number of particles currently held: *n_total includes periodic WALL
+   copies, *n_real does not (body_system::getNLocalBodies). */
+int fsph_count(const fsph_ctx *ctx, int64_t *n_total, int64_t *n_real);
+
+/* Copies one field of all n_total particles to host memory dst. */
+int fsph_download(fsph_ctx *ctx, int field, void *dst);
+
+/* ---- the per-step pipeline (app/drivers/hydro/main_driver.cc:98-279) -- */
+
+/* body_system::update_iteration (bodies_system.h:152-229): removes and
+   regenerates periodic WALL copies (boundary.h:46-250), computes the range
+   of x+-h (bodies_system.h:231-268), the Morton keys (tree_topology.h:
+   828-832, filling_curve.h:503-525), sorts by (key,id) (psort.h:247),
+   builds the octree with its centre-of-mass data (tree_topology.h:879-997,
+   default_physics.h:60-119), selects the <=group_max groups
+   (tree_topology.h:222-235) and builds the neighbour tiles that
+   traversal_sph would visit (tree_topology.h:213-442).  Positions and h
+   do not change between this call and the end of the step's SPH passes,
+   so the tiles serve all of them (the reference rebuilds the tree in
+   reset_ghosts, bodies_system.h:273-275, with identical result). */
+int fsph_update_iteration(fsph_ctx *ctx);
+
+/* apply_in_smoothinglength(physics::compute_density_pressure_soundspeed)
+   (default_physics.h:310-320): density, EOS pressure and sound speed,
+   signal speed. */
+int fsph_density_eos(fsph_ctx *ctx);
+
+/* eos::init over all particles (eos.h:155-159): for the ideal EOS nothing
+   that the pipeline reads; kept for call-order fidelity. */
+int fsph_eos_init(fsph_ctx *ctx);
+
+/* apply_in_smoothinglength(physics::compute_acceleration)
+   (default_physics.h:349-408): hydro acceleration; zeroes g and phi. */
+int fsph_acceleration(fsph_ctx *ctx);
+
+/* apply_in_smoothinglength(physics::compute_dudt) (default_physics.h:
+   422-487) followed by apply_all(recompute_pressure_soundspeed)
+   (default_physics.h:154-161) with the context's current dt. */
+int fsph_dudt(fsph_ctx *ctx);
+
+/* Leapfrog pieces (include/physics/integration.h:53-110), with the
+   context's current dt.  `what` is a bitmask of FSPH_KICK_*. */
+#define FSPH_KICK_V 1u      /* leapfrog_kick_v   integration.h:66-70     */
+#define FSPH_KICK_U 2u      /* leapfrog_kick_u   integration.h:81-84     */
+#define FSPH_SAVE_VHALF 4u  /* save_velocityhalf integration.h:53-55     */
+#define FSPH_DRIFT 8u       /* leapfrog_drift    integration.h:107-110   */
+int fsph_integrate(fsph_ctx *ctx, uint32_t what);
+
+/* get_all(physics::compute_smoothinglength) (default_physics.h:826-848). */
+int fsph_update_h(fsph_ctx *ctx);
+
+/* apply_all(physics::compute_dt) + get_all(set_adaptive_timestep)
+   (default_physics.h:603-695, 762-823): sets the context's dt and
+   returns it in *dt (may be NULL). */
+int fsph_timestep(fsph_ctx *ctx, double *dt);
+
+/* physics::advance_time (default_physics.h:855-862). */
+int fsph_advance_time(fsph_ctx *ctx);
+
+/* One whole iteration of the hydro driver loop body
+   (app/drivers/hydro/main_driver.cc:98-277; with gravity, the newtonian
+   driver app/drivers/newtonian/main_driver.cc:103-296), including
+   advance_time.  The first call after fsph_upload runs the initial
+   iteration (no integration). */
+int fsph_step(fsph_ctx *ctx);
+
+/* Current time-step state: iteration, dt, totaltime
+   (physics::iteration/dt/totaltime, default_physics.h:31-40). */
+int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,
+              double *totaltime);
+/* Overrides dt (e.g. a fixed-step run). */
+int fsph_set_dt(fsph_ctx *ctx, double dt);
+
+/* Global FP64 reductions over real particles (analysis.h:57-219):
+   out[0] mass, out[1..3] linear momentum, out[4] kinetic energy,
+   out[5] internal energy, out[6] gravitational energy (1/2 sum m phi),
+   out[7..9] angular momentum. */
+int fsph_reductions(fsph_ctx *ctx, double out[10]);
+
+/* ---- parity/introspection ---------------------------------------------- */
+
+/* Range of the last update_iteration: lo[3], hi[3]. */
+int fsph_get_range(fsph_ctx *ctx, double lo[3], double hi[3]);
+
+/* Octree of the last update_iteration.  *n_nodes and *n_groups are
+   always set; the arrays may be NULL.  node_key: the node's key with its
+   leading 1 (filling_curve.h:65-67 convention, depth = (63-clz)/3);
+   node_start/node_count: its particle range in sorted order;
+   node_parent: parent node index (-1 for the root). */
+int fsph_get_tree(fsph_ctx *ctx, int64_t *n_nodes, uint64_t *node_key,
+                  int32_t *node_start, int32_t *node_count,
+                  int32_t *node_parent, int64_t *n_groups,
+                  int32_t *group_start, int32_t *group_count);
+
+/* Neighbour lists that traversal_sph hands to the physics functors
+   (tree_topology.h:372-379, 406), in sorted particle order: the ids of the
+   neighbours of particle i are ids[off[i]..off[i+1]) in ascending sorted
+   index.  off has n_total+1 entries; ids may be NULL to query the total
+   (returned in off[n_total]).  cap is the length of ids. */
+int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
+                        int64_t cap);
+
+/* Milliseconds spent in each phase of the last fsph_step, measured with
+   HIP events on the context's stream; see FSPH_T_* for the slots. */
+enum fsph_timer {
+  FSPH_T_INTEGRATE = 0, FSPH_T_BOUNDARY, FSPH_T_RANGE_KEYS, FSPH_T_SORT,
+  FSPH_T_TREE, FSPH_T_NEIGHBORS, FSPH_T_DENSITY, FSPH_T_ACCEL,
+  FSPH_T_GRAVITY, FSPH_T_DUDT, FSPH_T_H, FSPH_T_DT, FSPH_NTIMERS
+};
+int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]);
+/* Turns the per-phase HIP-event timing on or off (default off). */
+int fsph_enable_timing(fsph_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSPH_H */

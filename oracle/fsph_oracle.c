@@ -1,0 +1,1222 @@
+/*
+ * fsph_oracle.c -- FP64 CPU restatement of FleCSPH's per-step particle
+ * pipeline.  TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline);
+ * the product path never links it.
+ *
+ * Every function states the reference code it restates (paths relative to
+ * the laristra/flecsph tree).  Arithmetic follows the reference expression
+ * by expression in IEEE FP64 (build: -O2 -fno-fast-math -ffp-contract=off),
+ * so that results agree with the reference to the last bit wherever the
+ * summation order is the same.
+ *
+ * Parity status of each part (how it is pinned) is listed in the test
+ * files; the one documented deviation is the signal speed: the reference
+ * reads neighbours' sound speed in the same pass that writes it
+ * (default_physics.h:237-265), and this restatement reproduces exactly
+ * that order (particles are processed in descending sorted index, see
+ * fo_density_eos).
+ */
+#include "fsph_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* std::max / std::min exactly as libstdc++ defines them (NaN handling of
+   compute_signalspeed depends on it, default_physics.h:254-262). */
+#define SMAX(a, b) (((a) < (b)) ? (b) : (a))
+#define SMIN(a, b) (((b) < (a)) ? (b) : (a))
+
+static const double TINY = 1e10 * DBL_MIN; /* kernels.h:64 */
+#define MAX_DEPTH 21                       /* filling_curve.h:44-47 */
+#define CHILD_EMPTY (-1)
+/* child codes: >= 0 node index; <= -2 leaf particle p encoded -(p+2) */
+#define LEAF_CODE(p) (-((int32_t)(p)) - 2)
+#define LEAF_OF(c) (-(c)-2)
+
+struct fo_sys {
+  fsph_params p;
+  int64_t n, n_real, cap;
+  double *x, *y, *z, *vx, *vy, *vz, *vhx, *vhy, *vhz, *u, *h, *m;
+  double *rho, *P, *c, *vsig, *ax, *ay, *az, *dudt, *gx, *gy, *gz, *phi;
+  int64_t *id;
+  uint64_t *key;
+  int32_t *nngb;
+  /* tree */
+  int64_t nn, ncap;
+  uint64_t *nkey;
+  int32_t *ndepth, *nstart, *ncount, *nparent, *nchild; /* nchild: 8/node */
+  double *ncom, *nmass, *nrad, *nlap, *nbmin, *nbmax;
+  /* groups */
+  int64_t ng;
+  int32_t *gnode, *gstart, *gcount;
+  /* neighbour lists (CSR over sorted index) */
+  int64_t *noff;
+  int32_t *nidx;
+  int have_tree;
+  double lo[3], hi[3];
+  int64_t iteration;
+  double dt, totaltime, totaltime_next;
+  int initial_done;
+};
+
+static int g_threads = 1;
+void fo_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+/* ------------------------------------------------------------------ */
+/* parameters                                                          */
+/* ------------------------------------------------------------------ */
+
+static double kernel_width(const fsph_params *p) {
+  (void)p;
+  return 2.0; /* kernels.h:753-756 for cubic spline and Wendland C6 */
+}
+
+/* ------------------------------------------------------------------ */
+/* allocation                                                          */
+/* ------------------------------------------------------------------ */
+#define NDBL 24
+static double **dbl_fields(fo_sys *s, double **out[NDBL]) {
+  double **f[NDBL] = {&s->x,   &s->y,    &s->z,   &s->vx,  &s->vy,  &s->vz,
+                      &s->vhx, &s->vhy,  &s->vhz, &s->u,   &s->h,   &s->m,
+                      &s->rho, &s->P,    &s->c,   &s->vsig, &s->ax, &s->ay,
+                      &s->az,  &s->dudt, &s->gx,  &s->gy,  &s->gz,  &s->phi};
+  memcpy(out, f, sizeof(f));
+  return NULL;
+}
+
+static int reserve(fo_sys *s, int64_t cap) {
+  if (cap <= s->cap) return 0;
+  double **f[NDBL];
+  dbl_fields(s, f);
+  for (int i = 0; i < NDBL; ++i) {
+    double *q = (double *)realloc(*f[i], (size_t)cap * sizeof(double));
+    if (!q) return FSPH_E_OOM;
+    *f[i] = q;
+  }
+  s->id = (int64_t *)realloc(s->id, (size_t)cap * sizeof(int64_t));
+  s->key = (uint64_t *)realloc(s->key, (size_t)cap * sizeof(uint64_t));
+  s->nngb = (int32_t *)realloc(s->nngb, (size_t)cap * sizeof(int32_t));
+  if (!s->id || !s->key || !s->nngb) return FSPH_E_OOM;
+  s->cap = cap;
+  return 0;
+}
+
+fo_sys *fo_create(const fsph_params *p) {
+  fo_sys *s = (fo_sys *)calloc(1, sizeof(fo_sys));
+  if (!s) return NULL;
+  s->p = *p;
+  return s;
+}
+
+static void free_tree(fo_sys *s) {
+  free(s->nkey); free(s->ndepth); free(s->nstart); free(s->ncount);
+  free(s->nparent); free(s->nchild); free(s->ncom); free(s->nmass);
+  free(s->nrad); free(s->nlap); free(s->nbmin); free(s->nbmax);
+  s->nkey = NULL; s->ndepth = s->nstart = s->ncount = s->nparent = NULL;
+  s->nchild = NULL;
+  s->ncom = s->nmass = s->nrad = s->nlap = s->nbmin = s->nbmax = NULL;
+  s->nn = s->ncap = 0;
+  free(s->gnode); free(s->gstart); free(s->gcount);
+  s->gnode = s->gstart = s->gcount = NULL;
+  s->ng = 0;
+  free(s->noff); free(s->nidx);
+  s->noff = NULL; s->nidx = NULL;
+  s->have_tree = 0;
+}
+
+void fo_destroy(fo_sys *s) {
+  if (!s) return;
+  double **f[NDBL];
+  dbl_fields(s, f);
+  for (int i = 0; i < NDBL; ++i) free(*f[i]);
+  free(s->id); free(s->key); free(s->nngb);
+  free_tree(s);
+  free(s);
+}
+
+int fo_upload(fo_sys *s, const fsph_particles *in) {
+  if (in->n < 2) return FSPH_E_ARG;
+  int rc = reserve(s, in->n);
+  if (rc) return rc;
+  int64_t n = in->n;
+  s->n = s->n_real = n;
+  size_t b = (size_t)n * sizeof(double);
+  memcpy(s->x, in->x, b); memcpy(s->y, in->y, b); memcpy(s->z, in->z, b);
+  memcpy(s->vx, in->vx, b); memcpy(s->vy, in->vy, b); memcpy(s->vz, in->vz, b);
+  memcpy(s->vhx, in->vx, b); memcpy(s->vhy, in->vy, b); memcpy(s->vhz, in->vz, b);
+  memcpy(s->u, in->u, b); memcpy(s->h, in->h, b); memcpy(s->m, in->m, b);
+  memcpy(s->id, in->id, (size_t)n * sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i) {
+    s->rho[i] = in->rho ? in->rho[i] : 0.0;
+    s->P[i] = in->P ? in->P[i] : 0.0;
+    s->c[i] = in->c ? in->c[i] : 0.0;
+    s->vsig[i] = 0; s->ax[i] = s->ay[i] = s->az[i] = 0; s->dudt[i] = 0;
+    s->gx[i] = s->gy[i] = s->gz[i] = s->phi[i] = 0;
+    s->key[i] = 0; s->nngb[i] = 0;
+  }
+  free_tree(s);
+  s->iteration = 0;
+  s->dt = s->p.initial_dt;
+  s->totaltime = 0.0;
+  s->totaltime_next = 0.0;
+  s->initial_done = 0;
+  return 0;
+}
+
+int fo_count(const fo_sys *s, int64_t *n_total, int64_t *n_real) {
+  if (n_total) *n_total = s->n;
+  if (n_real) *n_real = s->n_real;
+  return 0;
+}
+
+int fo_download(fo_sys *s, int field, void *dst) {
+  size_t b = (size_t)s->n * sizeof(double);
+  double *src = NULL;
+  switch (field) {
+    case FSPH_F_X: src = s->x; break;
+    case FSPH_F_Y: src = s->y; break;
+    case FSPH_F_Z: src = s->z; break;
+    case FSPH_F_VX: src = s->vx; break;
+    case FSPH_F_VY: src = s->vy; break;
+    case FSPH_F_VZ: src = s->vz; break;
+    case FSPH_F_VHX: src = s->vhx; break;
+    case FSPH_F_VHY: src = s->vhy; break;
+    case FSPH_F_VHZ: src = s->vhz; break;
+    case FSPH_F_U: src = s->u; break;
+    case FSPH_F_H: src = s->h; break;
+    case FSPH_F_M: src = s->m; break;
+    case FSPH_F_RHO: src = s->rho; break;
+    case FSPH_F_P: src = s->P; break;
+    case FSPH_F_C: src = s->c; break;
+    case FSPH_F_VSIG: src = s->vsig; break;
+    case FSPH_F_AX: src = s->ax; break;
+    case FSPH_F_AY: src = s->ay; break;
+    case FSPH_F_AZ: src = s->az; break;
+    case FSPH_F_DUDT: src = s->dudt; break;
+    case FSPH_F_GX: src = s->gx; break;
+    case FSPH_F_GY: src = s->gy; break;
+    case FSPH_F_GZ: src = s->gz; break;
+    case FSPH_F_PHI: src = s->phi; break;
+    case FSPH_F_ID: memcpy(dst, s->id, (size_t)s->n * 8); return 0;
+    case FSPH_F_KEY: memcpy(dst, s->key, (size_t)s->n * 8); return 0;
+    case FSPH_F_NNGB: {
+      double *d = (double *)dst;
+      for (int64_t i = 0; i < s->n; ++i) d[i] = s->nngb[i];
+      return 0;
+    }
+    default: return FSPH_E_ARG;
+  }
+  memcpy(dst, src, b);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A17: periodic WALL copies, include/physics/boundary.h:46-250        */
+/* ------------------------------------------------------------------ */
+static int any_periodic(const fo_sys *s) {
+  return s->p.periodic[0] || s->p.periodic[1] || s->p.periodic[2];
+}
+
+static void copy_particle(fo_sys *s, int64_t dst, int64_t src) {
+  double **f[NDBL];
+  dbl_fields(s, f);
+  for (int i = 0; i < NDBL; ++i) (*f[i])[dst] = (*f[i])[src];
+  s->id[dst] = s->id[src];
+  s->key[dst] = s->key[src];
+  s->nngb[dst] = s->nngb[src];
+}
+
+/* pboundary_clean (boundary.h:46-70): stable removal of WALL particles
+   (ids >= n_real). */
+static void pboundary_clean(fo_sys *s) {
+  int64_t w = 0;
+  for (int64_t i = 0; i < s->n; ++i) {
+    if (s->id[i] < s->n_real) {
+      if (w != i) copy_particle(s, w, i);
+      ++w;
+    }
+  }
+  s->n = w;
+}
+
+/* add_corner_XX / add_corner_XXX (boundary.h:72-119): a corner copy takes
+   the mirrored formula box_lo - |x + box_lo| or box_hi + |x + box_hi|. */
+static double corner_coord(double x, double lo, double hi, double halo) {
+  if (x + halo > hi) return lo - fabs(x + lo);
+  return hi + fabs(x + hi);
+}
+
+/* pboundary_generate (boundary.h:121-250) */
+static int pboundary_generate(fo_sys *s, double halo) {
+  const int *per = s->p.periodic;
+  double bmin[3], bmax[3];
+  for (int d = 0; d < 3; ++d) {
+    bmin[d] = -0.5 * s->p.box[d];
+    bmax[d] = 0.5 * s->p.box[d];
+  }
+  int64_t n = s->n;
+  double *X[3] = {s->x, s->y, s->z};
+  /* Step 1: teleport into the box (boundary.h:151-167). */
+  for (int64_t i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      if (!per[d]) continue;
+      if (X[d][i] > bmax[d])
+        X[d][i] -= bmax[d] - bmin[d];
+      else if (X[d][i] <= bmin[d])
+        X[d][i] += bmax[d] - bmin[d];
+    }
+  }
+  /* Count copies first. */
+  int64_t extra = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int on[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+      if (per[d] && (X[d][i] + halo > bmax[d] || X[d][i] - halo <= bmin[d])) {
+        on[d] = 1;
+        ++extra;
+      }
+    extra += (on[0] && on[1]) + (on[0] && on[2]) + (on[2] && on[1]) +
+             (on[0] && on[1] && on[2]);
+  }
+  int rc = reserve(s, n + extra);
+  if (rc) return rc;
+  X[0] = s->x; X[1] = s->y; X[2] = s->z;
+  int64_t w = n;
+  for (int64_t i = 0; i < n; ++i) {
+    int on[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      if (!per[d]) continue;
+      if (X[d][i] + halo > bmax[d] || X[d][i] - halo <= bmin[d]) {
+        on[d] = 1;
+        copy_particle(s, w, i);
+        if (X[d][i] + halo > bmax[d])
+          X[d][w] -= (bmax[d] - bmin[d]);
+        else
+          X[d][w] += (bmax[d] - bmin[d]);
+        ++w;
+      }
+    }
+    static const int pairs[3][2] = {{0, 1}, {0, 2}, {2, 1}};
+    for (int q = 0; q < 3; ++q) {
+      int a = pairs[q][0], b = pairs[q][1];
+      if (on[a] && on[b]) {
+        copy_particle(s, w, i);
+        X[a][w] = corner_coord(X[a][i], bmin[a], bmax[a], halo);
+        X[b][w] = corner_coord(X[b][i], bmin[b], bmax[b], halo);
+        ++w;
+      }
+    }
+    if (on[0] && on[1] && on[2]) {
+      copy_particle(s, w, i);
+      for (int d = 0; d < 3; ++d)
+        X[d][w] = corner_coord(X[d][i], bmin[d], bmax[d], halo);
+      ++w;
+    }
+  }
+  /* Re-index WALL particles from n_real in order of creation
+     (boundary.h:235-244). */
+  for (int64_t i = n; i < w; ++i) s->id[i] = s->n_real + (i - n);
+  s->n = w;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A2: range, bodies_system.h:231-268                                  */
+/* ------------------------------------------------------------------ */
+static void compute_range(fo_sys *s) {
+  double *X[3] = {s->x, s->y, s->z};
+  int64_t last = s->n - 1;
+  for (int d = 0; d < 3; ++d) {
+    s->lo[d] = X[d][last];
+    s->hi[d] = X[d][last];
+  }
+  for (int64_t i = 0; i < s->n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      if (X[d][i] + s->h[i] > s->hi[d]) s->hi[d] = X[d][i] + s->h[i];
+      if (X[d][i] - s->h[i] < s->lo[d]) s->lo[d] = X[d][i] - s->h[i];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* A3: Morton keys, filling_curve.h:503-525                            */
+/* ------------------------------------------------------------------ */
+uint64_t fo_morton_key(const double lo[3], const double hi[3], double x,
+                       double y, double z) {
+  const uint64_t max_val = (UINT64_C(1) << 21) - 1;
+  double p[3] = {x, y, z};
+  uint64_t coords[3];
+  for (int i = 0; i < 3; ++i) {
+    double mn = lo[i];
+    double scale = hi[i] - mn;
+    uint64_t c = (uint64_t)((p[i] - mn) / scale * (double)(UINT64_C(1) << 21));
+    coords[i] = c < max_val ? c : max_val;
+  }
+  uint64_t key = UINT64_C(1) << 63;
+  for (int i = 0; i < 21; ++i)
+    for (int j = 0; j < 3; ++j) {
+      uint64_t bit = (coords[j] >> i) & 1u;
+      key |= bit << (i * 3 + j);
+    }
+  return key;
+}
+
+/* ------------------------------------------------------------------ */
+/* A4: (key, id) sort, bodies_system.h:195-206 + psort.h:247           */
+/* ------------------------------------------------------------------ */
+static const fo_sys *g_sort_sys;
+static int cmp_key_id(const void *pa, const void *pb) {
+  int64_t a = *(const int64_t *)pa, b = *(const int64_t *)pb;
+  uint64_t ka = g_sort_sys->key[a], kb = g_sort_sys->key[b];
+  if (ka < kb) return -1;
+  if (ka > kb) return 1;
+  int64_t ia = g_sort_sys->id[a], ib = g_sort_sys->id[b];
+  return (ia < ib) ? -1 : (ia > ib);
+}
+
+static int sort_particles(fo_sys *s) {
+  int64_t n = s->n;
+  int64_t *perm = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+  if (!perm) return FSPH_E_OOM;
+  for (int64_t i = 0; i < n; ++i) perm[i] = i;
+  g_sort_sys = s;
+  qsort(perm, (size_t)n, sizeof(int64_t), cmp_key_id);
+  double **f[NDBL];
+  dbl_fields(s, f);
+  void *tmp = malloc((size_t)n * 8);
+  if (!tmp) { free(perm); return FSPH_E_OOM; }
+  for (int k = 0; k < NDBL + 3; ++k) {
+    if (k < NDBL) {
+      double *a = *f[k], *t = (double *)tmp;
+      for (int64_t i = 0; i < n; ++i) t[i] = a[perm[i]];
+      memcpy(a, t, (size_t)n * 8);
+    } else if (k == NDBL) {
+      int64_t *a = s->id, *t = (int64_t *)tmp;
+      for (int64_t i = 0; i < n; ++i) t[i] = a[perm[i]];
+      memcpy(a, t, (size_t)n * 8);
+    } else if (k == NDBL + 1) {
+      uint64_t *a = s->key, *t = (uint64_t *)tmp;
+      for (int64_t i = 0; i < n; ++i) t[i] = a[perm[i]];
+      memcpy(a, t, (size_t)n * 8);
+    } else {
+      int32_t *a = s->nngb, *t = (int32_t *)tmp;
+      for (int64_t i = 0; i < n; ++i) t[i] = a[perm[i]];
+      memcpy(a, t, (size_t)n * 4);
+    }
+  }
+  free(tmp);
+  free(perm);
+  for (int64_t i = 1; i < n; ++i)
+    if (s->key[i] == s->key[i - 1]) return FSPH_E_DUPKEY;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A5: octree of key prefixes shared by >= 2 particles                 */
+/*     (tree_topology.h:879-997), aggregates (default_physics.h:60-119) */
+/* ------------------------------------------------------------------ */
+static int digit_at(uint64_t key, int depth) {
+  return (int)((key >> (3 * (MAX_DEPTH - depth))) & 7u);
+}
+
+static int64_t new_node(fo_sys *s) {
+  if (s->nn == s->ncap) {
+    int64_t nc = s->ncap ? 2 * s->ncap : 1024;
+    s->nkey = (uint64_t *)realloc(s->nkey, (size_t)nc * 8);
+    s->ndepth = (int32_t *)realloc(s->ndepth, (size_t)nc * 4);
+    s->nstart = (int32_t *)realloc(s->nstart, (size_t)nc * 4);
+    s->ncount = (int32_t *)realloc(s->ncount, (size_t)nc * 4);
+    s->nparent = (int32_t *)realloc(s->nparent, (size_t)nc * 4);
+    s->nchild = (int32_t *)realloc(s->nchild, (size_t)nc * 32);
+    if (!s->nkey || !s->ndepth || !s->nstart || !s->ncount || !s->nparent ||
+        !s->nchild)
+      return -1;
+    s->ncap = nc;
+  }
+  return s->nn++;
+}
+
+/* Builds the node for sorted range [b, e) at `depth`; returns its index or
+   a negative error. */
+static int64_t build_node(fo_sys *s, int64_t b, int64_t e, int depth,
+                          int32_t parent) {
+  if (depth >= MAX_DEPTH) return FSPH_E_DUPKEY;
+  int64_t me = new_node(s);
+  if (me < 0) return FSPH_E_OOM;
+  s->nkey[me] = s->key[b] >> (3 * (MAX_DEPTH - depth));
+  s->ndepth[me] = depth;
+  s->nstart[me] = (int32_t)b;
+  s->ncount[me] = (int32_t)(e - b);
+  s->nparent[me] = parent;
+  for (int k = 0; k < 8; ++k) s->nchild[me * 8 + k] = CHILD_EMPTY;
+  int64_t i = b;
+  while (i < e) {
+    int d = digit_at(s->key[i], depth + 1);
+    int64_t j = i + 1;
+    while (j < e && digit_at(s->key[j], depth + 1) == d) ++j;
+    int32_t code;
+    if (j - i == 1) {
+      code = LEAF_CODE(i);
+    } else {
+      int64_t c = build_node(s, i, j, depth + 1, (int32_t)me);
+      if (c < 0) return c;
+      code = (int32_t)c;
+    }
+    s->nchild[me * 8 + d] = code;
+    i = j;
+  }
+  return me;
+}
+
+/* physics::compute_cofm (default_physics.h:60-119): leaves first, then
+   nodes, each in digit order (tree_topology.h:1949-1966). */
+static void compute_cofm(fo_sys *s, int64_t nd) {
+  double cx = 0, cy = 0, cz = 0, mass = 0, radius = 0, lap = 0;
+  double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  const int32_t *ch = s->nchild + nd * 8;
+  for (int k = 0; k < 8; ++k) {
+    int32_t c = ch[k];
+    if (c > CHILD_EMPTY || c == CHILD_EMPTY) continue;
+    int64_t p = LEAF_OF(c);
+    double m = s->m[p], X[3] = {s->x[p], s->y[p], s->z[p]};
+    cx += m * X[0]; cy += m * X[1]; cz += m * X[2];
+    mass += m;
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = SMIN(bmin[d], X[d] - s->h[p] / 2.);
+      bmax[d] = SMAX(bmax[d], X[d] + s->h[p] / 2.);
+    }
+  }
+  for (int k = 0; k < 8; ++k) {
+    int32_t c = ch[k];
+    if (c < 0) continue;
+    double m = s->nmass[c];
+    cx += m * s->ncom[3 * c]; cy += m * s->ncom[3 * c + 1];
+    cz += m * s->ncom[3 * c + 2];
+    mass += m;
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = SMIN(bmin[d], s->nbmin[3 * c + d]);
+      bmax[d] = SMAX(bmax[d], s->nbmax[3 * c + d]);
+    }
+  }
+  cx /= mass; cy /= mass; cz /= mass;
+  for (int k = 0; k < 8; ++k) {
+    int32_t c = ch[k];
+    if (c > CHILD_EMPTY || c == CHILD_EMPTY) continue;
+    int64_t p = LEAF_OF(c);
+    double dx = cx - s->x[p], dy = cy - s->y[p], dz = cz - s->z[p];
+    double sum = 0;
+    sum += dx * dx; sum += dy * dy; sum += dz * dz;
+    double dist = sqrt(sum);
+    radius = SMAX(radius, dist);
+    lap = SMAX(lap, dist + s->h[p]);
+  }
+  for (int k = 0; k < 8; ++k) {
+    int32_t c = ch[k];
+    if (c < 0) continue;
+    double dx = cx - s->ncom[3 * c], dy = cy - s->ncom[3 * c + 1],
+           dz = cz - s->ncom[3 * c + 2];
+    double sum = 0;
+    sum += dx * dx; sum += dy * dy; sum += dz * dz;
+    double dist = sqrt(sum);
+    radius = SMAX(radius, dist + s->nrad[c]);
+    lap = SMAX(lap, dist + s->nrad[c] + s->nlap[c]);
+  }
+  s->ncom[3 * nd] = cx; s->ncom[3 * nd + 1] = cy; s->ncom[3 * nd + 2] = cz;
+  s->nmass[nd] = mass;
+  s->nrad[nd] = radius;
+  s->nlap[nd] = lap;
+  for (int d = 0; d < 3; ++d) {
+    s->nbmin[3 * nd + d] = bmin[d];
+    s->nbmax[3 * nd + d] = bmax[d];
+  }
+}
+
+static int build_tree(fo_sys *s) {
+  s->nn = 0;
+  int64_t r = build_node(s, 0, s->n, 0, -1);
+  if (r < 0) return (int)r;
+  int64_t nn = s->nn;
+  s->ncom = (double *)realloc(s->ncom, (size_t)nn * 24);
+  s->nbmin = (double *)realloc(s->nbmin, (size_t)nn * 24);
+  s->nbmax = (double *)realloc(s->nbmax, (size_t)nn * 24);
+  s->nmass = (double *)realloc(s->nmass, (size_t)nn * 8);
+  s->nrad = (double *)realloc(s->nrad, (size_t)nn * 8);
+  s->nlap = (double *)realloc(s->nlap, (size_t)nn * 8);
+  if (!s->ncom || !s->nbmin || !s->nbmax || !s->nmass || !s->nrad || !s->nlap)
+    return FSPH_E_OOM;
+  /* children have larger indices than parents (pre-order allocation) */
+  for (int64_t i = nn - 1; i >= 0; --i) compute_cofm(s, i);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A6: groups, tree_topology.h:222-235 (DFS with a stack)              */
+/* ------------------------------------------------------------------ */
+static int build_groups(fo_sys *s) {
+  int64_t cap = 1024, top = 0;
+  int32_t *stk = (int32_t *)malloc((size_t)cap * 4);
+  int64_t gcap = 1024;
+  s->gnode = (int32_t *)realloc(s->gnode, (size_t)gcap * 4);
+  s->gstart = (int32_t *)realloc(s->gstart, (size_t)gcap * 4);
+  s->gcount = (int32_t *)realloc(s->gcount, (size_t)gcap * 4);
+  s->ng = 0;
+  stk[top++] = 0; /* root node code */
+  while (top) {
+    int32_t c = stk[--top];
+    if (c >= 0 && s->ncount[c] > s->p.group_max) {
+      for (int k = 0; k < 8; ++k) {
+        int32_t ch = s->nchild[c * 8 + k];
+        if (ch == CHILD_EMPTY) continue;
+        if (top == cap) {
+          cap *= 2;
+          stk = (int32_t *)realloc(stk, (size_t)cap * 4);
+        }
+        stk[top++] = ch;
+      }
+      continue;
+    }
+    if (s->ng == gcap) {
+      gcap *= 2;
+      s->gnode = (int32_t *)realloc(s->gnode, (size_t)gcap * 4);
+      s->gstart = (int32_t *)realloc(s->gstart, (size_t)gcap * 4);
+      s->gcount = (int32_t *)realloc(s->gcount, (size_t)gcap * 4);
+    }
+    if (c >= 0) {
+      s->gnode[s->ng] = c;
+      s->gstart[s->ng] = s->nstart[c];
+      s->gcount[s->ng] = s->ncount[c];
+    } else {
+      s->gnode[s->ng] = -1;
+      s->gstart[s->ng] = (int32_t)LEAF_OF(c);
+      s->gcount[s->ng] = 1;
+    }
+    s->ng++;
+  }
+  free(stk);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A7: neighbour search, tree_topology.h:213-442 with the predicates of */
+/*     tree_geometry.h:265-341                                          */
+/* ------------------------------------------------------------------ */
+static int box_box(const double *min1, const double *max1, const double *min2,
+                   const double *max2) {
+  return !((max1[0] < min2[0] || max1[1] < min2[1] || max1[2] < min2[2]) ||
+           (max2[0] < min1[0] || max2[1] < min1[1] || max2[2] < min1[2]));
+}
+
+static int sphere_box(const double *mn, const double *mx, double c0, double c1,
+                      double c2, double r) {
+  double x0 = c0 < mx[0] ? c0 : mx[0];
+  double x1 = c1 < mx[1] ? c1 : mx[1];
+  double x2 = c2 < mx[2] ? c2 : mx[2];
+  x0 = x0 < mn[0] ? mn[0] : x0;
+  x1 = x1 < mn[1] ? mn[1] : x1;
+  x2 = x2 < mn[2] ? mn[2] : x2;
+  double dist = (x0 - c0) * (x0 - c0) + (x1 - c1) * (x1 - c1) +
+                (x2 - c2) * (x2 - c2);
+  return dist <= r * r;
+}
+
+static int within_distance2(double a0, double a1, double a2, double b0,
+                            double b1, double b2, double r) {
+  double d = (a0 - b0) * (a0 - b0) + (a1 - b1) * (a1 - b1) +
+             (a2 - b2) * (a2 - b2);
+  return d <= r * r;
+}
+
+typedef struct {
+  int32_t *v;
+  int64_t n, cap;
+} ivec;
+static void ivec_push(ivec *q, int32_t x) {
+  if (q->n == q->cap) {
+    q->cap = q->cap ? 2 * q->cap : 256;
+    q->v = (int32_t *)realloc(q->v, (size_t)q->cap * 4);
+  }
+  q->v[q->n++] = x;
+}
+
+/* One group: the BFS of tree_topology.h:314-400; appends to lists[k]. */
+static void group_neighbors(fo_sys *s, int64_t g, ivec *lists, ivec *qa,
+                            ivec *qb) {
+  int32_t gn = s->gnode[g];
+  int64_t gs = s->gstart[g], gc = s->gcount[g];
+  const double *gbmin = gn >= 0 ? s->nbmin + 3 * gn : NULL;
+  const double *gbmax = gn >= 0 ? s->nbmax + 3 * gn : NULL;
+  ivec *q = qa, *nq = qb;
+  q->n = 0;
+  ivec_push(q, 0);
+  while (q->n) {
+    nq->n = 0;
+    for (int64_t j = 0; j < q->n; ++j) {
+      int32_t c = q->v[j];
+      if (c >= 0) {
+        const double *mn = s->nbmin + 3 * c, *mx = s->nbmax + 3 * c;
+        if (gn >= 0 && !box_box(mn, mx, gbmin, gbmax)) continue;
+        for (int64_t k = 0; k < gc; ++k) {
+          int64_t pk = gs + k;
+          if (sphere_box(mn, mx, s->x[pk], s->y[pk], s->z[pk], s->h[pk])) {
+            for (int d = 0; d < 8; ++d)
+              if (s->nchild[c * 8 + d] != CHILD_EMPTY)
+                ivec_push(nq, s->nchild[c * 8 + d]);
+            break;
+          }
+        }
+      } else {
+        int64_t e = LEAF_OF(c);
+        if (gn >= 0) {
+          double ext = SMAX(s->h[e], s->nlap[gn]) + s->nrad[gn];
+          if (!within_distance2(s->x[e], s->y[e], s->z[e], s->ncom[3 * gn],
+                                s->ncom[3 * gn + 1], s->ncom[3 * gn + 2], ext))
+            continue;
+        }
+        for (int64_t k = 0; k < gc; ++k) {
+          int64_t pk = gs + k;
+          double ext = SMAX(s->h[pk], s->h[e]);
+          if (within_distance2(s->x[pk], s->y[pk], s->z[pk], s->x[e], s->y[e],
+                               s->z[e], ext))
+            ivec_push(&lists[k], (int32_t)e);
+        }
+      }
+    }
+    ivec *t = q; q = nq; nq = t;
+  }
+}
+
+static int build_lists(fo_sys *s) {
+  int64_t n = s->n;
+  ivec *all = (ivec *)calloc((size_t)n, sizeof(ivec));
+  if (!all) return FSPH_E_OOM;
+#pragma omp parallel num_threads(g_threads)
+  {
+    ivec qa = {0}, qb = {0};
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t g = 0; g < s->ng; ++g)
+      group_neighbors(s, g, all + s->gstart[g], &qa, &qb);
+    free(qa.v);
+    free(qb.v);
+  }
+  s->noff = (int64_t *)realloc(s->noff, (size_t)(n + 1) * 8);
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    s->noff[i] = tot;
+    tot += all[i].n;
+    s->nngb[i] = (int32_t)all[i].n;
+  }
+  s->noff[n] = tot;
+  s->nidx = (int32_t *)realloc(s->nidx, (size_t)(tot ? tot : 1) * 4);
+  for (int64_t i = 0; i < n; ++i) {
+    if (all[i].n)
+      memcpy(s->nidx + s->noff[i], all[i].v, (size_t)all[i].n * 4);
+    free(all[i].v);
+  }
+  free(all);
+  return 0;
+}
+
+/* body_system::update_iteration, bodies_system.h:152-229 */
+int fo_update_iteration(fo_sys *s) {
+  int rc;
+  free_tree(s);
+  if (any_periodic(s)) {
+    pboundary_clean(s);
+    /* getSmoothinglength (bodies_system.h:112-124) */
+    double hmax = 0;
+    for (int64_t i = 0; i < s->n; ++i)
+      if (hmax < s->h[i]) hmax = s->h[i];
+    rc = pboundary_generate(s, 2.5 * hmax);
+    if (rc) return rc;
+  }
+  compute_range(s);
+  for (int64_t i = 0; i < s->n; ++i)
+    s->key[i] = fo_morton_key(s->lo, s->hi, s->x[i], s->y[i], s->z[i]);
+  rc = sort_particles(s);
+  if (rc) return rc;
+  rc = build_tree(s);
+  if (rc) return rc;
+  rc = build_groups(s);
+  if (rc) return rc;
+  rc = build_lists(s);
+  if (rc) return rc;
+  s->have_tree = 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A11: kernels, include/physics/kernels.h                             */
+/* ------------------------------------------------------------------ */
+static const double cubic_sigma3 = 8. / M_PI;            /* kernels.h:40 */
+static const double wendland_c6_sigma3 = 1365. / (64. * M_PI); /* :45-46 */
+
+double fo_kernel(int kernel, double r, double h) {
+  if (kernel == FSPH_KERNEL_CUBIC_SPLINE) { /* kernels.h:106-121 */
+    double rh = 2. * r / h;
+    double result = 0.;
+    if (rh < 2.0) {
+      if (rh < 1.0)
+        result = 1.0 - 1.5 * rh * rh + .75 * rh * rh * rh;
+      else
+        result = 0.25 * (2 - rh) * (2 - rh) * (2 - rh);
+      result *= cubic_sigma3 / pow(h, 3);
+    }
+    return result;
+  }
+  /* Wendland C6, 3D (kernels.h:502-512) */
+  double rh = r / h;
+  double hd = h * h * h;
+  double rh2 = (1 - rh) * (1 - rh);
+  double rh4 = rh2 * rh2;
+  double sigma = wendland_c6_sigma3 / hd;
+  double result = sigma * rh4 * rh4 * (1 + rh * (8 + rh * (25 + rh * 32)));
+  return result * (rh < 1.0);
+}
+
+void fo_kernel_gradient(int kernel, const double v[3], double h,
+                        double out[3]) {
+  double sum = 0;
+  sum += v[0] * v[0]; sum += v[1] * v[1]; sum += v[2] * v[2];
+  double r = sqrt(sum);
+  if (kernel == FSPH_KERNEL_CUBIC_SPLINE) { /* kernels.h:134-153 */
+    double rh = 2. * r / h;
+    out[0] = out[1] = out[2] = 0.0;
+    if (rh < 2.0) {
+      double dWdr;
+      double sigma = 2. * cubic_sigma3 / pow(h, 3 + 1);
+      if (rh < 1.0)
+        dWdr = -3.0 * rh + 9. / 4. * rh * rh;
+      else
+        dWdr = -.75 * (2 - rh) * (2 - rh);
+      for (int d = 0; d < 3; ++d) out[d] = v[d] * sigma * dWdr / (r + TINY);
+    }
+    return;
+  }
+  /* Wendland C6, 3D (kernels.h:552-566) */
+  double rh = r / h;
+  double hd1 = h * h * h * h;
+  double rh2 = (1 - rh) * (1 - rh);
+  double rh3 = rh2 * (1 - rh);
+  double rh4 = rh2 * rh2;
+  double sigma = wendland_c6_sigma3 / hd1;
+  double dWdr = -22. * rh * rh4 * rh3 * (1 + rh * (7 + rh * 16));
+  double f = (rh < 1.0) * (sigma * dWdr / (r + TINY));
+  for (int d = 0; d < 3; ++d) out[d] = v[d] * f;
+}
+
+/* ------------------------------------------------------------------ */
+/* A8: density + EOS + signal speed, default_physics.h:194-320         */
+/* ------------------------------------------------------------------ */
+static void eos_pressure_soundspeed(const fsph_params *p, double rho, double u,
+                                    double *P, double *c) {
+  if (p->eos == FSPH_EOS_POLYTROPIC) { /* eos.h:102-119 */
+    *P = p->poly_K * pow(rho, p->gamma);
+    *c = sqrt(p->poly_K * p->gamma * pow(rho, p->gamma - 1.));
+  } else { /* eos.h:166-183 */
+    *P = (p->gamma - 1.) * rho * u;
+    *c = sqrt(p->gamma * (p->gamma - 1.) * u);
+  }
+}
+
+int fo_eos_init(fo_sys *s) {
+  (void)s; /* polytropic K is a parameter here; ideal EOS reads none */
+  return 0;
+}
+
+/* Particles are handed to the functor in descending sorted index (groups
+   and their entities are emitted by a stack DFS that pops digit 7 first,
+   tree_topology.h:176-191, 222-235, 292-310), so when particle a computes
+   its signal speed, neighbours b with a larger sorted index already hold
+   this pass's sound speed and those with a smaller index the previous one
+   (default_physics.h:237-265 reading getSoundspeed of neighbours). */
+int fo_density_eos(fo_sys *s) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  const fsph_params *p = &s->p;
+  int64_t n = s->n;
+  double *c_old = (double *)malloc((size_t)n * 8);
+  memcpy(c_old, s->c, (size_t)n * 8);
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    double h_a = s->h[a];
+    double rho_a = 0.0;
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double dx = s->x[a] - s->x[b], dy = s->y[a] - s->y[b],
+             dz = s->z[a] - s->z[b];
+      double sum = 0;
+      sum += dx * dx; sum += dy * dy; sum += dz * dz;
+      double r = sqrt(sum);
+      double Wab = fo_kernel(p->kernel, r, .5 * (h_a + s->h[b]));
+      rho_a += s->m[b] * Wab;
+    }
+    s->rho[a] = rho_a;
+    eos_pressure_soundspeed(p, rho_a, s->u[a], &s->P[a], &s->c[a]);
+  }
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    double c_a = s->c[a];
+    double vsig = 0.0;
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double c_b = (b >= a) ? s->c[b] : c_old[b];
+      double dx = s->x[a] - s->x[b], dy = s->y[a] - s->y[b],
+             dz = s->z[a] - s->z[b];
+      double sum = 0;
+      sum += dx * dx; sum += dy * dy; sum += dz * dz;
+      double dist = sqrt(sum);
+      double n0 = dx / dist, n1 = dy / dist, n2 = dz / dist;
+      double v0 = s->vx[a] - s->vx[b], v1 = s->vy[a] - s->vy[b],
+             v2 = s->vz[a] - s->vz[b];
+      double cab = SMAX(c_a, c_b);
+      double dot = 0;
+      dot += v0 * n0; dot += v1 * n1; dot += v2 * n2;
+      vsig = SMAX(vsig, cab - SMIN(dot, 0.0));
+    }
+    s->vsig[a] = vsig;
+  }
+  free(c_old);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A9: acceleration, default_physics.h:349-408; viscosity.h:81-137     */
+/* ------------------------------------------------------------------ */
+static double visc_mu(double h_ab, const double v[3], const double r[3],
+                      double eps) {
+  double dot = 0, dist2 = 0;
+  dot += v[0] * r[0]; dot += v[1] * r[1]; dot += v[2] * r[2];
+  dist2 += r[0] * r[0]; dist2 += r[1] * r[1]; dist2 += r[2] * r[2];
+  double result = h_ab * dot / (dist2 + eps * h_ab * h_ab + TINY);
+  return result * (dot < 0.0);
+}
+
+static double visc_pi(const fsph_params *p, double rho_ab, double c_ab,
+                      double mu_ab) {
+  return (-p->visc_alpha * c_ab + p->visc_beta * mu_ab) * mu_ab / rho_ab;
+}
+
+int fo_acceleration(fo_sys *s) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  const fsph_params *p = &s->p;
+  int64_t n = s->n;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    double h_a = s->h[a], rho_a = s->rho[a], P_a = s->P[a], c_a = s->c[a];
+    double Prho2_a = P_a / (rho_a * rho_a);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double pos_ab[3] = {s->x[a] - s->x[b], s->y[a] - s->y[b],
+                          s->z[a] - s->z[b]};
+      int differ = (s->x[b] != s->x[a]) || (s->y[b] != s->y[a]) ||
+                   (s->z[b] != s->z[a]);
+      double m_b = s->m[b] * differ;
+      double v12_ab[3] = {s->vhx[a] - s->vhx[b], s->vhy[a] - s->vhy[b],
+                          s->vhz[a] - s->vhz[b]};
+      double h_ab = .5 * (h_a + s->h[b]);
+      double mu_ab = visc_mu(h_ab, v12_ab, pos_ab, p->visc_eps);
+      double rho_ab = .5 * (rho_a + s->rho[b]);
+      double c_ab = .5 * (c_a + s->c[b]);
+      double Pi = visc_pi(p, rho_ab, c_ab, mu_ab);
+      double DiW[3];
+      fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
+      double Prho2_b = s->P[b] / (s->rho[b] * s->rho[b]);
+      double f = -m_b * (Prho2_a + Prho2_b + Pi);
+      acc[0] += DiW[0] * f; acc[1] += DiW[1] * f; acc[2] += DiW[2] * f;
+    }
+    s->ax[a] = acc[0]; s->ay[a] = acc[1]; s->az[a] = acc[2];
+    s->gx[a] = s->gy[a] = s->gz[a] = 0.0;
+    s->phi[a] = 0.0;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A10: du/dt, default_physics.h:422-487, then                          */
+/*      recompute_pressure_soundspeed, default_physics.h:154-161       */
+/* ------------------------------------------------------------------ */
+int fo_dudt(fo_sys *s) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  const fsph_params *p = &s->p;
+  int64_t n = s->n;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    double h_a = s->h[a], rho_a = s->rho[a], P_a = s->P[a], c_a = s->c[a];
+    double dp = 0.0, dv = 0.0;
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double pos_ab[3] = {s->x[a] - s->x[b], s->y[a] - s->y[b],
+                          s->z[a] - s->z[b]};
+      int differ = (s->x[b] != s->x[a]) || (s->y[b] != s->y[a]) ||
+                   (s->z[b] != s->z[a]);
+      double m_b = s->m[b] * differ;
+      double v12_ab[3] = {s->vhx[a] - s->vhx[b], s->vhy[a] - s->vhy[b],
+                          s->vhz[a] - s->vhz[b]};
+      double vel_ab[3] = {s->vx[a] - s->vx[b], s->vy[a] - s->vy[b],
+                          s->vz[a] - s->vz[b]};
+      double h_ab = .5 * (h_a + s->h[b]);
+      double mu_ab = visc_mu(h_ab, v12_ab, pos_ab, p->visc_eps);
+      double rho_ab = .5 * (rho_a + s->rho[b]);
+      double c_ab = .5 * (c_a + s->c[b]);
+      double Pi = visc_pi(p, rho_ab, c_ab, mu_ab);
+      double DiW[3];
+      fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
+      double vdw = 0;
+      vdw += vel_ab[0] * DiW[0]; vdw += vel_ab[1] * DiW[1];
+      vdw += vel_ab[2] * DiW[2];
+      dp += m_b * vdw;
+      dv += m_b * vdw * Pi;
+    }
+    s->dudt[a] = P_a / (rho_a * rho_a) * dp + .5 * dv;
+  }
+  /* recompute_pressure_soundspeed: P, c at u + dt/2 du/dt */
+  for (int64_t a = 0; a < n; ++a) {
+    double uh = s->u[a] + 0.5 * s->dt * s->dudt[a];
+    eos_pressure_soundspeed(p, s->rho[a], uh, &s->P[a], &s->c[a]);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A13: leapfrog, integration.h:53-110                                  */
+/* ------------------------------------------------------------------ */
+int fo_integrate(fo_sys *s, uint32_t what) {
+  int64_t n = s->n;
+  double dt = s->dt;
+  if (what & FSPH_KICK_U)
+    for (int64_t i = 0; i < n; ++i) s->u[i] = s->u[i] + 0.5 * dt * s->dudt[i];
+  if (what & FSPH_KICK_V)
+    for (int64_t i = 0; i < n; ++i) {
+      double f = 0.5 * dt;
+      s->vx[i] = s->vx[i] + (s->ax[i] + s->gx[i]) * f;
+      s->vy[i] = s->vy[i] + (s->ay[i] + s->gy[i]) * f;
+      s->vz[i] = s->vz[i] + (s->az[i] + s->gz[i]) * f;
+    }
+  if (what & FSPH_SAVE_VHALF)
+    for (int64_t i = 0; i < n; ++i) {
+      s->vhx[i] = s->vx[i]; s->vhy[i] = s->vy[i]; s->vhz[i] = s->vz[i];
+    }
+  if (what & FSPH_DRIFT)
+    for (int64_t i = 0; i < n; ++i) {
+      s->x[i] = s->x[i] + s->vx[i] * dt;
+      s->y[i] = s->y[i] + s->vy[i] * dt;
+      s->z[i] = s->z[i] + s->vz[i] * dt;
+    }
+  return 0;
+}
+
+/* A15: compute_smoothinglength, default_physics.h:826-848 (3D) */
+int fo_update_h(fo_sys *s) {
+  double kw = kernel_width(&s->p);
+  for (int64_t i = 0; i < s->n; ++i)
+    s->h[i] = cbrt(s->m[i] / s->rho[i]) * s->p.eta * kw;
+  return 0;
+}
+
+/* A14: compute_dt (default_physics.h:603-695) over every body, WALL copies
+   included, then set_adaptive_timestep (default_physics.h:762-823) with
+   no output-time matching. */
+int fo_timestep(fo_sys *s, double *dt_out) {
+  const fsph_params *p = &s->p;
+  if (p->adaptive_dt) {
+    const double tiny = 1e-24, mc = 0.6;
+    double kw = kernel_width(p);
+    double dtmin = 1e24;
+    for (int64_t i = 0; i < s->n; ++i) {
+      double dx = s->h[i] / (p->eta * kw);
+      double sum = 0;
+      sum += s->vx[i] * s->vx[i]; sum += s->vy[i] * s->vy[i];
+      sum += s->vz[i] * s->vz[i];
+      double vn = sqrt(sum);
+      double dt_v = dx / (vn + tiny);
+      double a0 = s->ax[i] + s->gx[i], a1 = s->ay[i] + s->gy[i],
+             a2 = s->az[i] + s->gz[i];
+      double sa = 0;
+      sa += a0 * a0; sa += a1 * a1; sa += a2 * a2;
+      double acc = sqrt(sa);
+      double dt_a = sqrt(dx / (acc + tiny));
+      double dt_c = dx / (tiny + s->vsig[i] * (1 + mc * p->visc_alpha));
+      double dti = p->cfl * SMIN(SMIN(dt_v, dt_a), dt_c);
+      dtmin = SMIN(dtmin, dti);
+    }
+    if (dtmin < s->dt) s->dt = SMIN(dtmin, s->dt / 2.0);
+    if (dtmin > 2.0 * s->dt) s->dt = s->dt * 2.0;
+    s->totaltime_next = s->totaltime + s->dt;
+    s->dt = s->totaltime_next - s->totaltime;
+  }
+  if (dt_out) *dt_out = s->dt;
+  return 0;
+}
+
+/* physics::advance_time, default_physics.h:855-862 */
+int fo_advance_time(fo_sys *s) {
+  s->iteration++;
+  if (s->p.adaptive_dt)
+    s->totaltime = s->totaltime_next;
+  else
+    s->totaltime = s->totaltime + s->dt;
+  return 0;
+}
+
+int fo_time(const fo_sys *s, int64_t *it, double *dt, double *tt) {
+  if (it) *it = s->iteration;
+  if (dt) *dt = s->dt;
+  if (tt) *tt = s->totaltime;
+  return 0;
+}
+
+int fo_set_dt(fo_sys *s, double dt) {
+  s->dt = dt;
+  return 0;
+}
+
+/* One loop iteration of app/drivers/hydro/main_driver.cc:98-277 (and of
+   app/drivers/newtonian/main_driver.cc when gravity is on), with
+   sph_variable_h = yes, thermokinetic_formulation = no, constant
+   viscosity, no relaxation, pressure_updates_number = 1. */
+int fo_step(fo_sys *s) {
+  int rc;
+  if (!s->initial_done) {
+    if ((rc = fo_update_iteration(s))) return rc;
+    fo_eos_init(s);
+    if ((rc = fo_density_eos(s))) return rc;
+    fo_integrate(s, FSPH_SAVE_VHALF);
+    if ((rc = fo_acceleration(s))) return rc;
+    if (s->p.gravity && (rc = fo_gravity(s))) return rc;
+    fo_timestep(s, NULL);
+    if ((rc = fo_dudt(s))) return rc;
+    s->initial_done = 1;
+  } else {
+    fo_integrate(s, FSPH_KICK_U);
+    fo_integrate(s, FSPH_KICK_V);
+    fo_integrate(s, FSPH_SAVE_VHALF);
+    fo_integrate(s, FSPH_DRIFT);
+    if ((rc = fo_update_iteration(s))) return rc;
+    if ((rc = fo_density_eos(s))) return rc;
+    if ((rc = fo_acceleration(s))) return rc;
+    if (s->p.gravity && (rc = fo_gravity(s))) return rc;
+    fo_integrate(s, FSPH_KICK_V);
+    if ((rc = fo_dudt(s))) return rc;
+    fo_integrate(s, FSPH_KICK_U);
+  }
+  fo_update_h(s);
+  fo_timestep(s, NULL);
+  fo_advance_time(s);
+  return 0;
+}
+
+/* A18: analysis.h:57-219 over NORMAL (non-WALL) particles */
+int fo_reductions(fo_sys *s, double out[10]) {
+  for (int k = 0; k < 10; ++k) out[k] = 0.0;
+  for (int64_t i = 0; i < s->n; ++i) {
+    if (s->id[i] >= s->n_real) continue;
+    double m = s->m[i];
+    out[0] += m;
+    out[1] += m * s->vx[i]; out[2] += m * s->vy[i]; out[3] += m * s->vz[i];
+    double v2 = 0;
+    v2 += s->vx[i] * s->vx[i]; v2 += s->vy[i] * s->vy[i];
+    v2 += s->vz[i] * s->vz[i];
+    out[4] += .5 * m * v2;
+    out[5] += m * s->u[i];
+    out[6] += s->phi[i] * m;
+    out[7] += m * (s->y[i] * s->vz[i] - s->z[i] * s->vy[i]);
+    out[8] += m * (s->z[i] * s->vx[i] - s->x[i] * s->vz[i]);
+    out[9] += m * (s->x[i] * s->vy[i] - s->y[i] * s->vx[i]);
+  }
+  out[6] *= 0.5;
+  return 0;
+}
+
+int fo_get_range(fo_sys *s, double lo[3], double hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = s->lo[d];
+    hi[d] = s->hi[d];
+  }
+  return 0;
+}
+
+int fo_get_tree(fo_sys *s, int64_t *n_nodes, uint64_t *node_key,
+                int32_t *node_start, int32_t *node_count, int32_t *node_parent,
+                int64_t *n_groups, int32_t *group_start, int32_t *group_count) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  if (n_nodes) *n_nodes = s->nn;
+  if (n_groups) *n_groups = s->ng;
+  if (node_key) memcpy(node_key, s->nkey, (size_t)s->nn * 8);
+  if (node_start) memcpy(node_start, s->nstart, (size_t)s->nn * 4);
+  if (node_count) memcpy(node_count, s->ncount, (size_t)s->nn * 4);
+  if (node_parent) memcpy(node_parent, s->nparent, (size_t)s->nn * 4);
+  if (group_start) memcpy(group_start, s->gstart, (size_t)s->ng * 4);
+  if (group_count) memcpy(group_count, s->gcount, (size_t)s->ng * 4);
+  return 0;
+}
+
+int fo_get_node_data(fo_sys *s, double *com, double *mass, double *radius,
+                     double *lap, double *bmin, double *bmax) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  size_t nn = (size_t)s->nn;
+  if (com) memcpy(com, s->ncom, nn * 24);
+  if (mass) memcpy(mass, s->nmass, nn * 8);
+  if (radius) memcpy(radius, s->nrad, nn * 8);
+  if (lap) memcpy(lap, s->nlap, nn * 8);
+  if (bmin) memcpy(bmin, s->nbmin, nn * 24);
+  if (bmax) memcpy(bmax, s->nbmax, nn * 24);
+  return 0;
+}
+
+int fo_neighbor_lists(fo_sys *s, int64_t *off, int64_t *ids, int64_t cap) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  int64_t n = s->n;
+  for (int64_t i = 0; i <= n; ++i) off[i] = s->noff[i];
+  if (!ids) return 0;
+  if (cap < s->noff[n]) return FSPH_E_CAP;
+  for (int64_t j = 0; j < s->noff[n]; ++j) ids[j] = s->nidx[j];
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* A12: gravity                                                         */
+/* ------------------------------------------------------------------ */
+void fo_direct_gravity(int64_t n, const double *x, const double *y,
+                       const double *z, const double *m, double G, double *gx,
+                       double *gy, double *gz, double *phi) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double a0 = 0, a1 = 0, a2 = 0, pot = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double d0 = x[i] - x[j], d1 = y[i] - y[j], d2 = z[i] - z[j];
+      double sum = 0;
+      sum += d0 * d0; sum += d1 * d1; sum += d2 * d2;
+      double dist = sqrt(sum);
+      pot += -G * m[j] / dist;
+      double f = -G * m[j] / (dist * dist * dist);
+      a0 += f * d0; a1 += f * d1; a2 += f * d2;
+    }
+    gx[i] = a0; gy[i] = a1; gz[i] = a2; phi[i] = pot;
+  }
+}
+
+int fo_gravity(fo_sys *s) {
+  fo_direct_gravity(s->n, s->x, s->y, s->z, s->m, s->p.G, s->gx, s->gy, s->gz,
+                    s->phi);
+  return 0;
+}
+
+/* Exported for the pin tests against the reference's tree_geometry.h. */
+int fo_sphere_box(const double *mn, const double *mx, const double *c,
+                  double r) {
+  return sphere_box(mn, mx, c[0], c[1], c[2], r);
+}
+int fo_box_box(const double *mn1, const double *mx1, const double *mn2,
+               const double *mx2) {
+  return box_box(mn1, mx1, mn2, mx2);
+}
+int fo_within_distance2(const double *p1, const double *p2, double r) {
+  return within_distance2(p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], r);
+}
