@@ -42,8 +42,8 @@ FIELDS = ["x", "y", "z", "vx", "vy", "vz", "vhx", "vhy", "vhz", "u", "h", "m",
           "phi", "id", "key", "nngb"]
 FIELD_INDEX = {name: i for i, name in enumerate(FIELDS)}
 
-TIMERS = ["integrate", "boundary", "range_keys", "sort", "tree", "neighbors",
-          "density", "accel", "gravity", "dudt", "h", "dt"]
+TIMERS = ["integrate", "boundary", "range_keys", "sort", "tree", "traverse",
+          "masks", "density", "vsig", "accel", "gravity", "dudt", "h", "dt"]
 
 
 class FsphParams(C.Structure):

@@ -72,20 +72,6 @@ int read_flags(Ctx *c, int *flags) {
 
 namespace {
 
-struct PhaseTimer {
-  Ctx *c;
-  int slot;
-  PhaseTimer(Ctx *cc, int s) : c(cc), slot(s) {
-    if (c->timing) hipEventRecord(c->ev[2 * slot], c->stream);
-  }
-  ~PhaseTimer() {
-    if (c->timing) {
-      hipEventRecord(c->ev[2 * slot + 1], c->stream);
-      c->phase_used[slot] = true;
-    }
-  }
-};
-
 int do_update_iteration(Ctx *c) {
   c->have_tree = false;
   FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
@@ -112,10 +98,7 @@ int do_update_iteration(Ctx *c) {
     FSPH_TRY(stage_tree(c));
     FSPH_TRY(stage_groups(c));
   }
-  {
-    PhaseTimer t(c, FSPH_T_NEIGHBORS);
-    FSPH_TRY(stage_neighbors(c));
-  }
+  FSPH_TRY(stage_neighbors(c));
   c->have_tree = true;
   return 0;
 }
@@ -151,29 +134,25 @@ int do_advance(Ctx *c) {
 int do_gravity(Ctx *c);
 
 int do_step(Ctx *c) {
-  if (c->timing)
-    for (int i = 0; i < FSPH_NTIMERS; ++i) c->phase_used[i] = false;
+  c->ev_used = 0;
   if (!c->initial_done) {
     // hydro/main_driver.cc:102-173 (iteration 0: no integration)
     FSPH_TRY(do_update_iteration(c));
+    FSPH_TRY(stage_density(c));
     {
-      PhaseTimer t(c, FSPH_T_DENSITY);
-      FSPH_TRY(stage_density(c));
+      PhaseTimer t(c, FSPH_T_INTEGRATE);
+      FSPH_TRY(stage_integrate(c, FSPH_SAVE_VHALF));
     }
-    FSPH_TRY(stage_integrate(c, FSPH_SAVE_VHALF));
-    {
-      PhaseTimer t(c, FSPH_T_ACCEL);
-      FSPH_TRY(stage_acceleration(c));
-    }
+    FSPH_TRY(stage_acceleration(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
       FSPH_TRY(do_gravity(c));
     }
-    FSPH_TRY(do_timestep(c));
     {
-      PhaseTimer t(c, FSPH_T_DUDT);
-      FSPH_TRY(stage_dudt(c));
+      PhaseTimer t(c, FSPH_T_DT);
+      FSPH_TRY(do_timestep(c));
     }
+    FSPH_TRY(stage_dudt(c));
     c->initial_done = true;
   } else {
     // hydro/main_driver.cc:174-245
@@ -183,22 +162,19 @@ int do_step(Ctx *c) {
           c, FSPH_KICK_U | FSPH_KICK_V | FSPH_SAVE_VHALF | FSPH_DRIFT));
     }
     FSPH_TRY(do_update_iteration(c));
-    {
-      PhaseTimer t(c, FSPH_T_DENSITY);
-      FSPH_TRY(stage_density(c));
-    }
-    {
-      PhaseTimer t(c, FSPH_T_ACCEL);
-      FSPH_TRY(stage_acceleration(c));
-    }
+    FSPH_TRY(stage_density(c));
+    FSPH_TRY(stage_acceleration(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
       FSPH_TRY(do_gravity(c));
     }
     {
-      PhaseTimer t(c, FSPH_T_DUDT);
+      PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(c, FSPH_KICK_V));
-      FSPH_TRY(stage_dudt(c));
+    }
+    FSPH_TRY(stage_dudt(c));
+    {
+      PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(c, FSPH_KICK_U));
     }
   }
@@ -270,7 +246,7 @@ int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
     delete c;
     return FSPH_E_OOM;
   }
-  for (int i = 0; i < 2 * FSPH_NTIMERS; ++i) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < 2 * Ctx::kEvPool; ++i) hipEventCreate(&c->ev[i]);
   *out = reinterpret_cast<fsph_ctx *>(c);
   return FSPH_OK;
 }
@@ -294,7 +270,7 @@ void fsph_destroy(fsph_ctx *ctx) {
     if (c->f[f].p) hipFree(c->f[f].p);
     if (c->g[f].p) hipFree(c->g[f].p);
   }
-  for (int i = 0; i < 2 * FSPH_NTIMERS; ++i)
+  for (int i = 0; i < 2 * Ctx::kEvPool; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   hipStreamDestroy(c->stream);
   delete c;
@@ -492,13 +468,13 @@ int fsph_step(fsph_ctx *ctx) {
   if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
   FSPH_TRY(fsph::do_step(c));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  if (c->timing)
-    for (int i = 0; i < FSPH_NTIMERS; ++i) {
-      float ms = 0;
-      if (c->phase_used[i])
-        hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]);
-      c->phase_ms[i] = ms;
-    }
+  for (int i = 0; i < FSPH_NTIMERS; ++i) c->phase_ms[i] = 0;
+  for (int k = 0; k < c->ev_used; ++k) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]) == hipSuccess)
+      c->phase_ms[c->ev_slot[k]] += ms;
+  }
+  c->ev_used = 0;
   return FSPH_OK;
 }
 

@@ -100,11 +100,30 @@ struct Ctx {
   double dt = 0, totaltime = 0, totaltime_next = 0;
   bool initial_done = false;
 
-  // timing
+  // timing: event pairs handed out per timed region of a step
+  static constexpr int kEvPool = 96;
   bool timing = false;
-  hipEvent_t ev[2 * FSPH_NTIMERS] = {};
+  hipEvent_t ev[2 * kEvPool] = {};
+  int ev_slot[kEvPool] = {0};
+  int ev_used = 0;
   float phase_ms[FSPH_NTIMERS] = {0};
-  bool phase_used[FSPH_NTIMERS] = {false};
+};
+
+// Records HIP events around a region on the context's stream when timing is
+// on; fsph_step sums the regions per slot.
+struct PhaseTimer {
+  Ctx *c;
+  int k = -1;
+  PhaseTimer(Ctx *cc, int slot) : c(cc) {
+    if (c->timing && c->ev_used < Ctx::kEvPool) {
+      k = c->ev_used++;
+      c->ev_slot[k] = slot;
+      hipEventRecord(c->ev[2 * k], c->stream);
+    }
+  }
+  ~PhaseTimer() {
+    if (k >= 0) hipEventRecord(c->ev[2 * k + 1], c->stream);
+  }
 };
 
 // ---- helpers (fsph_api.hip) ---------------------------------------------

@@ -265,7 +265,10 @@ int stage_neighbors(Ctx *c) {
                c->gnode.as<int32_t>(),  c->cand.as<int32_t>(),
                c->ccount.as<int32_t>(), c->ccap,
                c->flags.as<int>()};
-    k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
+    {
+      PhaseTimer tm(c, FSPH_T_TRAVERSE);
+      k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
+    }
     FSPH_HIP(c, hipGetLastError());
     int fl = 0;
     FSPH_TRY(read_flags(c, &fl));
@@ -299,7 +302,10 @@ int stage_neighbors(Ctx *c) {
              c->cand.as<int32_t>(),   c->ccount.as<int32_t>(),
              c->moff.as<int64_t>(),   c->ccap,
              c->mask.as<uint32_t>(),  c->f[PF_NNGB].as<int32_t>()};
-  k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
+  {
+    PhaseTimer tm(c, FSPH_T_MASKS);
+    k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
+  }
   return hip_check(c, hipGetLastError(), "k_masks");
 }
 

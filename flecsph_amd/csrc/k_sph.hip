@@ -408,7 +408,10 @@ template <class T> int density_impl(Ctx *c) {
   a.rho = c->f[PF_RHO].as<T>(); a.P = c->f[PF_P].as<T>();
   a.c = c->f[PF_C].as<T>();
   a.ph = phys_of(c);
-  k_density<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+  {
+    PhaseTimer tm(c, FSPH_T_DENSITY);
+    k_density<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+  }
   FSPH_HIP(c, hipGetLastError());
   VsigArgs<T> v;
   v.t = a.t;
@@ -417,7 +420,10 @@ template <class T> int density_impl(Ctx *c) {
   v.vz = c->f[PF_VZ].as<double>();
   v.c = c->f[PF_C].as<T>(); v.c_old = c->f[PF_COLD].as<T>();
   v.vsig = c->f[PF_VSIG].as<T>();
-  k_vsig<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(v);
+  {
+    PhaseTimer tm(c, FSPH_T_VSIG);
+    k_vsig<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(v);
+  }
   return hip_check(c, hipGetLastError(), "k_vsig");
 }
 
@@ -445,7 +451,10 @@ template <class T, bool DUDT> int force_impl(Ctx *c) {
   a.c_out = c->g[PF_C].as<T>();
   a.dt = c->dt;
   a.ph = phys_of(c);
-  k_force<T, DUDT><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+  {
+    PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
+    k_force<T, DUDT><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+  }
   FSPH_HIP(c, hipGetLastError());
   if (DUDT) {
     std::swap(c->f[PF_P], c->g[PF_P]);

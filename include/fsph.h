@@ -236,11 +236,25 @@ int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
                         int64_t cap);
 
 /* Milliseconds spent in each phase of the last fsph_step, measured with
-   HIP events on the context's stream; see FSPH_T_* for the slots. */
+   HIP events on the context's stream around the phase's kernels (summed
+   when a phase runs more than once in a step); see FSPH_T_* for the slots.
+   Slots marked (1 kernel) time exactly one kernel launch. */
 enum fsph_timer {
-  FSPH_T_INTEGRATE = 0, FSPH_T_BOUNDARY, FSPH_T_RANGE_KEYS, FSPH_T_SORT,
-  FSPH_T_TREE, FSPH_T_NEIGHBORS, FSPH_T_DENSITY, FSPH_T_ACCEL,
-  FSPH_T_GRAVITY, FSPH_T_DUDT, FSPH_T_H, FSPH_T_DT, FSPH_NTIMERS
+  FSPH_T_INTEGRATE = 0, /* leapfrog kernels                              */
+  FSPH_T_BOUNDARY,      /* periodic WALL copies                          */
+  FSPH_T_RANGE_KEYS,    /* range reduction + Morton keys                 */
+  FSPH_T_SORT,          /* radix sort + permutation                      */
+  FSPH_T_TREE,          /* octree, aggregates, groups                    */
+  FSPH_T_TRAVERSE,      /* neighbour-tile traversal (1 kernel)           */
+  FSPH_T_MASKS,         /* neighbour bitmasks (1 kernel)                 */
+  FSPH_T_DENSITY,       /* density + EOS (1 kernel)                      */
+  FSPH_T_VSIG,          /* signal speed (1 kernel)                       */
+  FSPH_T_ACCEL,         /* hydro acceleration (1 kernel)                 */
+  FSPH_T_GRAVITY,       /* self-gravity                                  */
+  FSPH_T_DUDT,          /* du/dt + P, c at the half step (1 kernel)      */
+  FSPH_T_H,             /* smoothing-length update                       */
+  FSPH_T_DT,            /* time-step reduction                           */
+  FSPH_NTIMERS
 };
 int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]);
 /* Turns the per-phase HIP-event timing on or off (default off). */
