@@ -1,10 +1,15 @@
 // k_sph.hip -- the SPH pair passes over the neighbour tiles.
 //
-// One wavefront per group: lane l owns group particles l and l+64.  The
-// group's candidates are staged in LDS in chunks of kChunk (SoA, coalesced
-// gathers), and every lane walks its own mask bits.  Pair arithmetic runs in
-// T (float by default, double for FSPH_PREC_FP64) with compensated
-// (Neumaier) sums; position differences are taken in FP64 first.
+// One 128-thread workgroup per group.  A group of gc particles gets
+// LP = 128/gc (rounded down to a power of two, at most 64) lanes per
+// particle: lane r of particle k walks mask words r, r+LP, ... of the
+// particle's row, and the LP partial results are combined with xor
+// shuffles.  Candidates and the chunk's mask words are staged in LDS in
+// chunks of kChunk candidates (SoA, coalesced loads); each lane walks its set
+// bits in one flattened loop, so the loop count is the lane's own neighbour
+// count rather than a per-word maximum.  Pair arithmetic runs in T (float by
+// default, double for FSPH_PREC_FP64) with compensated (Kahan) sums;
+// position differences are taken in FP64 first.
 #include <cfloat>
 
 #include "fsph_internal.h"
@@ -13,21 +18,48 @@ namespace fsph {
 
 namespace {
 
-constexpr int kW = 64;
-constexpr int kChunk = 256;
+constexpr int kB = 128;      // threads per workgroup
+constexpr int kChunk = 128;  // candidates staged per round
+constexpr int kChunkW = kChunk / 32;
 
-template <class T> struct KSum {  // Neumaier compensated sum
+template <class T> struct KSum {  // Kahan compensated sum
   T s = 0, c = 0;
   __device__ inline void add(T v) {
-    const T t = s + v;
-    if (fabs(s) >= fabs(v))
-      c += (s - t) + v;
-    else
-      c += (v - t) + s;
+    const T y = v - c;
+    const T t = s + y;
+    c = (t - s) - y;
     s = t;
   }
-  __device__ inline T get() const { return s + c; }
 };
+
+template <class T> __device__ inline T xor_sum(T v, int LP) {
+  for (int o = LP >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T> __device__ inline T xor_max(T v, int LP) {
+  for (int o = LP >> 1; o > 0; o >>= 1) {
+    const T u = __shfl_xor(v, o, 64);
+    v = v < u ? u : v;
+  }
+  return v;
+}
+// Combines the lanes' Kahan partials of one particle.
+template <class T> __device__ inline T ksum_total(const KSum<T> &k, int LP) {
+  return xor_sum(k.s, LP) - xor_sum(k.c, LP);
+}
+
+__device__ inline int lanes_per_particle(int gc) {
+  int lp = 1;
+  while (lp < 64 && gc * lp * 2 <= kB) lp <<= 1;
+  return lp;
+}
+
+// Fast reciprocal / reciprocal square root: hardware approximations in
+// FP32 (about 1 ulp), IEEE operations in FP64.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp(double x) { return 1.0 / x; }
+__device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
 
 struct Phys {
   int kernel;
@@ -36,9 +68,8 @@ struct Phys {
 };
 
 // kernels.h: cubic spline (106-153) and Wendland C6 3D (502-566); h is the
-// support radius.  W(r,h) and F(r,h) with grad W = r_vec * F.
-template <class T> __device__ inline T kernel_W(int kind, T r, T h) {
-  const T ih = T(1) / h;
+// support radius.  W(r, 1/h), and F with grad W = r_vec * F.
+template <class T> __device__ inline T kernel_W(int kind, T r, T ih) {
   const T ih3 = ih * ih * ih;
   if (kind == FSPH_KERNEL_CUBIC_SPLINE) {
     const T q = T(2) * r * ih;
@@ -56,10 +87,9 @@ template <class T> __device__ inline T kernel_W(int kind, T r, T h) {
          (T(1) + q * (T(8) + q * (T(25) + q * T(32))));
 }
 
-template <class T> __device__ inline T kernel_F(int kind, T r, T h) {
-  // r > 0 is guaranteed by the callers (coincident pairs carry no force).
-  const T ih = T(1) / h;
-  const T ih4 = ih * ih * ih * ih;
+// F = (dW/dr) / r for r > 0 (callers skip coincident pairs).
+template <class T> __device__ inline T kernel_F(int kind, T r, T ir, T ih) {
+  const T ih4 = (ih * ih) * (ih * ih);
   if (kind == FSPH_KERNEL_CUBIC_SPLINE) {
     const T q = T(2) * r * ih;
     if (q >= T(2)) return T(0);
@@ -71,14 +101,14 @@ template <class T> __device__ inline T kernel_F(int kind, T r, T h) {
       const T t = T(2) - q;
       dW = T(-0.75) * t * t;
     }
-    return sig * dW / r;
+    return sig * dW * ir;
   }
   const T q = r * ih;
   if (q >= T(1)) return T(0);
   const T om = T(1) - q;
   const T om2 = om * om, om3 = om2 * om, om4 = om2 * om2;
   const T dW = T(-22) * q * om4 * om3 * (T(1) + q * (T(7) + q * T(16)));
-  return T(1365.0 / (64.0 * M_PI)) * ih4 * dW / r;
+  return T(1365.0 / (64.0 * M_PI)) * ih4 * dW * ir;
 }
 
 __device__ inline void eos_pc(const Phys &ph, double rho, double u, double *P,
@@ -99,18 +129,52 @@ struct Tiles {
   int64_t ccap;
 };
 
+// Per-workgroup view of one group's tile.
+struct TileCtx {
+  int64_t g, gs, mo, cb;
+  int gc, cc, LP, k, r;
+  bool active;
+};
+
+__device__ inline TileCtx tile_ctx(const Tiles &t) {
+  TileCtx c;
+  c.g = blockIdx.x;
+  c.gs = t.gstart[c.g];
+  c.gc = t.gcount[c.g];
+  c.cc = t.ccount[c.g];
+  c.mo = t.moff[c.g];
+  c.cb = c.g * t.ccap;
+  c.LP = lanes_per_particle(c.gc);
+  c.k = threadIdx.x / c.LP;
+  c.r = threadIdx.x & (c.LP - 1);
+  c.active = c.k < c.gc;
+  return c;
+}
+
+// Stages the chunk's mask words ([word][particle] layout, contiguous) in LDS.
+__device__ inline void stage_masks(const Tiles &t, const TileCtx &c, int c0,
+                                   int nw, uint32_t *smask) {
+  const uint32_t *src = t.mask + c.mo + (int64_t)(c0 >> 5) * c.gc;
+  for (int q = threadIdx.x; q < nw * c.gc; q += kB) smask[q] = src[q];
+}
+
+// Calls f(j) for every candidate j of the chunk that neighbours this lane's
+// particle, over the lane's share of the mask words.
 template <class F>
-__device__ inline void walk_bits(const Tiles &t, int64_t mo, int gc, int k,
-                                 int c0, int nch, F &&f) {
-  const int nw = (nch + 31) >> 5;
-  const int w0 = c0 >> 5;
-  for (int w = 0; w < nw; ++w) {
-    uint32_t bits = t.mask[mo + (int64_t)(w0 + w) * gc + k];
-    while (bits) {
-      const int j = __ffs(bits) - 1;
-      bits &= bits - 1;
-      f(w * 32 + j);
+__device__ inline void walk(const TileCtx &c, const uint32_t *smask, int nw,
+                            F &&f) {
+  if (!c.active) return;
+  int w = c.r;
+  uint32_t bits = w < nw ? smask[w * c.gc + c.k] : 0u;
+  for (;;) {
+    while (bits == 0u) {
+      w += c.LP;
+      if (w >= nw) return;
+      bits = smask[w * c.gc + c.k];
     }
+    const int j = __ffs(bits) - 1;
+    bits &= bits - 1u;
+    f(w * 32 + j);
   }
 }
 
@@ -125,55 +189,46 @@ template <class T> struct DensArgs {
 };
 
 template <class T>
-__global__ __launch_bounds__(kW) void k_density(DensArgs<T> a) {
+__global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
   __shared__ T sh[kChunk], sm[kChunk];
-  const int lane = threadIdx.x;
-  const int64_t g = blockIdx.x;
-  const int64_t gs = a.t.gstart[g];
-  const int gc = a.t.gcount[g], cc = a.t.ccount[g];
-  const int64_t mo = a.t.moff[g], cb = g * a.t.ccap;
-  double px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
-  T ph[2] = {1, 1};
-  KSum<T> rho[2];
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k < gc) {
-      px[s] = a.x[gs + k]; py[s] = a.y[gs + k]; pz[s] = a.z[gs + k];
-      ph[s] = (T)a.h[gs + k];
-    }
+  __shared__ uint32_t smask[kChunkW * kB];
+  const TileCtx c = tile_ctx(a.t);
+  double px = 0, py = 0, pz = 0;
+  T ph = 1;
+  if (c.active) {
+    const int64_t i = c.gs + c.k;
+    px = a.x[i]; py = a.y[i]; pz = a.z[i];
+    ph = (T)a.h[i];
   }
-  for (int c0 = 0; c0 < cc; c0 += kChunk) {
-    const int nch = min(kChunk, cc - c0);
-    for (int j = lane; j < nch; j += kW) {
-      const int64_t b = a.t.cand[cb + c0 + j];
+  KSum<T> rho;
+  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
+    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+    for (int j = threadIdx.x; j < nch; j += kB) {
+      const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
       sh[j] = (T)a.h[b]; sm[j] = (T)a.m[b];
     }
+    stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-      const int k = lane + kW * s;
-      if (k >= gc) break;
-      walk_bits(a.t, mo, gc, k, c0, nch, [&](int j) {
-        const T dx = (T)(px[s] - sx[j]), dy = (T)(py[s] - sy[j]),
-                dz = (T)(pz[s] - sz[j]);
-        const T r = sqrt(dx * dx + dy * dy + dz * dz);
-        const T hab = T(0.5) * (ph[s] + sh[j]);
-        rho[s].add(sm[j] * kernel_W<T>(a.ph.kernel, r, hab));
-      });
-    }
+    walk(c, smask, nw, [&](int j) {
+      const T dx = (T)(px - sx[j]), dy = (T)(py - sy[j]), dz = (T)(pz - sz[j]);
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      const T r = r2 > T(0) ? r2 * rsq(r2) : T(0);
+      const T ih = rcp(T(0.5) * (ph + sh[j]));
+      rho.add(sm[j] * kernel_W<T>(a.ph.kernel, r, ih));
+    });
     __syncthreads();
   }
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k >= gc) break;
-    const int64_t i = gs + k;
-    const double r = (double)rho[s].get();
-    double P, c;
-    eos_pc(a.ph, r, a.u[i], &P, &c);
-    a.rho[i] = (T)r;
+  const T tot = ksum_total(rho, c.LP);
+  if (c.active && c.r == 0) {
+    const int64_t i = c.gs + c.k;
+    const double rr = (double)tot;
+    double P, cs;
+    eos_pc(a.ph, rr, a.u[i], &P, &cs);
+    a.rho[i] = (T)rr;
     a.P[i] = (T)P;
-    a.c[i] = (T)c;
+    a.c[i] = (T)cs;
   }
 }
 
@@ -191,63 +246,49 @@ template <class T> struct VsigArgs {
 };
 
 template <class T>
-__global__ __launch_bounds__(kW) void k_vsig(VsigArgs<T> a) {
+__global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
   __shared__ T svx[kChunk], svy[kChunk], svz[kChunk], scn[kChunk],
       sco[kChunk];
   __shared__ int32_t sid[kChunk];
-  const int lane = threadIdx.x;
-  const int64_t g = blockIdx.x;
-  const int64_t gs = a.t.gstart[g];
-  const int gc = a.t.gcount[g], cc = a.t.ccount[g];
-  const int64_t mo = a.t.moff[g], cb = g * a.t.ccap;
-  double px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
-  T pvx[2] = {0, 0}, pvy[2] = {0, 0}, pvz[2] = {0, 0}, pc[2] = {0, 0},
-    vs[2] = {0, 0};
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k < gc) {
-      const int64_t i = gs + k;
-      px[s] = a.x[i]; py[s] = a.y[i]; pz[s] = a.z[i];
-      pvx[s] = (T)a.vx[i]; pvy[s] = (T)a.vy[i]; pvz[s] = (T)a.vz[i];
-      pc[s] = a.c[i];
-    }
+  __shared__ uint32_t smask[kChunkW * kB];
+  const TileCtx c = tile_ctx(a.t);
+  double px = 0, py = 0, pz = 0;
+  T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0;
+  const int32_t ia = (int32_t)(c.gs + c.k);
+  if (c.active) {
+    px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
+    pvx = (T)a.vx[ia]; pvy = (T)a.vy[ia]; pvz = (T)a.vz[ia];
+    pc = a.c[ia];
   }
-  for (int c0 = 0; c0 < cc; c0 += kChunk) {
-    const int nch = min(kChunk, cc - c0);
-    for (int j = lane; j < nch; j += kW) {
-      const int32_t b = a.t.cand[cb + c0 + j];
+  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
+    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+    for (int j = threadIdx.x; j < nch; j += kB) {
+      const int32_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
       svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
       scn[j] = a.c[b]; sco[j] = a.c_old[b];
       sid[j] = b;
     }
+    stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-      const int k = lane + kW * s;
-      if (k >= gc) break;
-      const int32_t ia = (int32_t)(gs + k);
-      walk_bits(a.t, mo, gc, k, c0, nch, [&](int j) {
-        const double ddx = px[s] - sx[j], ddy = py[s] - sy[j],
-                     ddz = pz[s] - sz[j];
-        if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
-        const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-        const T dist = sqrt(dx * dx + dy * dy + dz * dz);
-        const T vdn = ((pvx[s] - svx[j]) * dx + (pvy[s] - svy[j]) * dy +
-                       (pvz[s] - svz[j]) * dz) / dist;
-        const T cb2 = sid[j] >= ia ? scn[j] : sco[j];
-        const T cab = pc[s] < cb2 ? cb2 : pc[s];
-        const T v = cab - (vdn < T(0) ? vdn : T(0));
-        vs[s] = vs[s] < v ? v : vs[s];
-      });
-    }
+    walk(c, smask, nw, [&](int j) {
+      const double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+      if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
+      const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
+      const T ir = rsq(dx * dx + dy * dy + dz * dz);
+      const T vdn =
+          ((pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz) *
+          ir;
+      const T cb = sid[j] >= ia ? scn[j] : sco[j];
+      const T cab = pc < cb ? cb : pc;
+      const T v = cab - (vdn < T(0) ? vdn : T(0));
+      vs = vs < v ? v : vs;
+    });
     __syncthreads();
   }
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k >= gc) break;
-    a.vsig[gs + k] = vs[s];
-  }
+  vs = xor_max(vs, c.LP);
+  if (c.active && c.r == 0) a.vsig[ia] = vs;
 }
 
 // ---------------------------------------------------------------------------
@@ -266,43 +307,36 @@ template <class T, bool DUDT> struct ForceArgs {
 };
 
 template <class T, bool DUDT>
-__global__ __launch_bounds__(kW) void k_force(ForceArgs<T, DUDT> a) {
+__global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
   __shared__ T svhx[kChunk], svhy[kChunk], svhz[kChunk];
   __shared__ T svx[DUDT ? kChunk : 1], svy[DUDT ? kChunk : 1],
       svz[DUDT ? kChunk : 1];
   __shared__ T sh[kChunk], sm[kChunk], srho[kChunk], sprho2[kChunk],
       sc[kChunk];
-  const int lane = threadIdx.x;
-  const int64_t g = blockIdx.x;
-  const int64_t gs = a.t.gstart[g];
-  const int gc = a.t.gcount[g], cc = a.t.ccount[g];
-  const int64_t mo = a.t.moff[g], cb = g * a.t.ccap;
-  double px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
-  T pvhx[2] = {0, 0}, pvhy[2] = {0, 0}, pvhz[2] = {0, 0};
-  T pvx[2] = {0, 0}, pvy[2] = {0, 0}, pvz[2] = {0, 0};
-  T ph[2] = {1, 1}, prho[2] = {1, 1}, pprho2[2] = {0, 0}, pc[2] = {0, 0};
-  KSum<T> s0[2], s1[2], s2[2];
+  __shared__ uint32_t smask[kChunkW * kB];
+  const TileCtx c = tile_ctx(a.t);
+  double px = 0, py = 0, pz = 0;
+  T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
+  T ph = 1, prho = 1, pprho2 = 0, pc = 0;
   const T alpha = (T)a.ph.alpha, beta = (T)a.ph.beta, eps = (T)a.ph.eps;
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k < gc) {
-      const int64_t i = gs + k;
-      px[s] = a.x[i]; py[s] = a.y[i]; pz[s] = a.z[i];
-      pvhx[s] = (T)a.vhx[i]; pvhy[s] = (T)a.vhy[i]; pvhz[s] = (T)a.vhz[i];
-      if (DUDT) {
-        pvx[s] = (T)a.vx[i]; pvy[s] = (T)a.vy[i]; pvz[s] = (T)a.vz[i];
-      }
-      ph[s] = (T)a.h[i];
-      prho[s] = a.rho[i];
-      pprho2[s] = a.P[i] / (a.rho[i] * a.rho[i]);
-      pc[s] = a.c[i];
+  if (c.active) {
+    const int64_t i = c.gs + c.k;
+    px = a.x[i]; py = a.y[i]; pz = a.z[i];
+    pvhx = (T)a.vhx[i]; pvhy = (T)a.vhy[i]; pvhz = (T)a.vhz[i];
+    if (DUDT) {
+      pvx = (T)a.vx[i]; pvy = (T)a.vy[i]; pvz = (T)a.vz[i];
     }
+    ph = (T)a.h[i];
+    prho = a.rho[i];
+    pprho2 = a.P[i] / (prho * prho);
+    pc = a.c[i];
   }
-  for (int c0 = 0; c0 < cc; c0 += kChunk) {
-    const int nch = min(kChunk, cc - c0);
-    for (int j = lane; j < nch; j += kW) {
-      const int64_t b = a.t.cand[cb + c0 + j];
+  KSum<T> s0, s1, s2;
+  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
+    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+    for (int j = threadIdx.x; j < nch; j += kB) {
+      const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
       svhx[j] = (T)a.vhx[b]; svhy[j] = (T)a.vhy[b]; svhz[j] = (T)a.vhz[b];
       if (DUDT) {
@@ -315,62 +349,59 @@ __global__ __launch_bounds__(kW) void k_force(ForceArgs<T, DUDT> a) {
       sprho2[j] = a.P[b] / (rb * rb);
       sc[j] = a.c[b];
     }
+    stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-      const int k = lane + kW * s;
-      if (k >= gc) break;
-      walk_bits(a.t, mo, gc, k, c0, nch, [&](int j) {
-        const double ddx = px[s] - sx[j], ddy = py[s] - sy[j],
-                     ddz = pz[s] - sz[j];
-        if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
-        const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-        const T r2 = dx * dx + dy * dy + dz * dz;
-        const T r = sqrt(r2);
-        const T hab = T(0.5) * (ph[s] + sh[j]);
-        const T F = kernel_F<T>(a.ph.kernel, r, hab);
-        if (F == T(0)) return;
-        const T vhdr = (pvhx[s] - svhx[j]) * dx + (pvhy[s] - svhy[j]) * dy +
-                       (pvhz[s] - svhz[j]) * dz;
-        T mu = T(0);
-        if (vhdr < T(0)) mu = hab * vhdr / (r2 + eps * hab * hab);
-        const T rhoab = T(0.5) * (prho[s] + srho[j]);
-        const T cab = T(0.5) * (pc[s] + sc[j]);
-        const T Pi = (-alpha * cab + beta * mu) * mu / rhoab;
-        if (!DUDT) {
-          const T f = -sm[j] * (pprho2[s] + sprho2[j] + Pi) * F;
-          s0[s].add(f * dx);
-          s1[s].add(f * dy);
-          s2[s].add(f * dz);
-        } else {
-          const T vdw = ((pvx[s] - svx[j]) * dx + (pvy[s] - svy[j]) * dy +
-                         (pvz[s] - svz[j]) * dz) * F;
-          const T mv = sm[j] * vdw;
-          s0[s].add(mv);
-          s1[s].add(mv * Pi);
-        }
-      });
-    }
+    walk(c, smask, nw, [&](int j) {
+      const double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+      if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
+      const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      const T ir = rsq(r2);
+      const T r = r2 * ir;
+      const T hab = T(0.5) * (ph + sh[j]);
+      const T ih = rcp(hab);
+      const T F = kernel_F<T>(a.ph.kernel, r, ir, ih);
+      if (F == T(0)) return;
+      const T vhdr = (pvhx - svhx[j]) * dx + (pvhy - svhy[j]) * dy +
+                     (pvhz - svhz[j]) * dz;
+      T mu = T(0);
+      if (vhdr < T(0)) mu = hab * vhdr * rcp(r2 + eps * hab * hab);
+      const T rhoab = T(0.5) * (prho + srho[j]);
+      const T cab = T(0.5) * (pc + sc[j]);
+      const T Pi = (-alpha * cab + beta * mu) * mu * rcp(rhoab);
+      if (!DUDT) {
+        const T f = -sm[j] * (pprho2 + sprho2[j] + Pi) * F;
+        s0.add(f * dx);
+        s1.add(f * dy);
+        s2.add(f * dz);
+      } else {
+        const T vdw = ((pvx - svx[j]) * dx + (pvy - svy[j]) * dy +
+                       (pvz - svz[j]) * dz) * F;
+        const T mv = sm[j] * vdw;
+        s0.add(mv);
+        s1.add(mv * Pi);
+      }
+    });
     __syncthreads();
   }
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k >= gc) break;
-    const int64_t i = gs + k;
-    if (!DUDT) {
-      a.ax[i] = s0[s].get();
-      a.ay[i] = s1[s].get();
-      a.az[i] = s2[s].get();
-      a.gx[i] = T(0); a.gy[i] = T(0); a.gz[i] = T(0); a.phi[i] = T(0);
-    } else {
-      const T du = pprho2[s] * s0[s].get() + T(0.5) * s1[s].get();
-      a.dudt[i] = du;
-      // recompute_pressure_soundspeed (default_physics.h:154-161)
-      const double uh = a.u[i] + 0.5 * a.dt * (double)du;
-      double P, c;
-      eos_pc(a.ph, (double)a.rho[i], uh, &P, &c);
-      a.P_out[i] = (T)P;
-      a.c_out[i] = (T)c;
-    }
+  const T t0 = ksum_total(s0, c.LP), t1 = ksum_total(s1, c.LP);
+  const T t2 = DUDT ? T(0) : ksum_total(s2, c.LP);
+  if (!(c.active && c.r == 0)) return;
+  const int64_t i = c.gs + c.k;
+  if (!DUDT) {
+    a.ax[i] = t0;
+    a.ay[i] = t1;
+    a.az[i] = t2;
+    a.gx[i] = T(0); a.gy[i] = T(0); a.gz[i] = T(0); a.phi[i] = T(0);
+  } else {
+    const T du = pprho2 * t0 + T(0.5) * t1;
+    a.dudt[i] = du;
+    // recompute_pressure_soundspeed (default_physics.h:154-161)
+    const double uh = a.u[i] + 0.5 * a.dt * (double)du;
+    double P, cs;
+    eos_pc(a.ph, (double)prho, uh, &P, &cs);
+    a.P_out[i] = (T)P;
+    a.c_out[i] = (T)cs;
   }
 }
 
@@ -410,7 +441,7 @@ template <class T> int density_impl(Ctx *c) {
   a.ph = phys_of(c);
   {
     PhaseTimer tm(c, FSPH_T_DENSITY);
-    k_density<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+    k_density<T><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
   }
   FSPH_HIP(c, hipGetLastError());
   VsigArgs<T> v;
@@ -422,7 +453,7 @@ template <class T> int density_impl(Ctx *c) {
   v.vsig = c->f[PF_VSIG].as<T>();
   {
     PhaseTimer tm(c, FSPH_T_VSIG);
-    k_vsig<T><<<(unsigned)c->ng, kW, 0, c->stream>>>(v);
+    k_vsig<T><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
   }
   return hip_check(c, hipGetLastError(), "k_vsig");
 }
@@ -453,7 +484,7 @@ template <class T, bool DUDT> int force_impl(Ctx *c) {
   a.ph = phys_of(c);
   {
     PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
-    k_force<T, DUDT><<<(unsigned)c->ng, kW, 0, c->stream>>>(a);
+    k_force<T, DUDT><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
   }
   FSPH_HIP(c, hipGetLastError());
   if (DUDT) {

@@ -155,13 +155,25 @@ __global__ __launch_bounds__(kT) void k_level_hist(const int32_t *__restrict__ d
               (unsigned long long)h[threadIdx.x]);
 }
 
+// Counting-sort scatter of node ids by depth: positions are reserved per
+// block and level (one global atomic per non-empty level per block), then
+// handed out inside the block from LDS.
 __global__ __launch_bounds__(kT) void k_level_scatter(
     const int32_t *__restrict__ depth, int64_t nn,
     unsigned long long *__restrict__ cursor, int32_t *__restrict__ order) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= nn) return;
-  const unsigned long long pos = atomicAdd(&cursor[depth[i]], 1ull);
-  order[pos] = (int32_t)i;
+  __shared__ int32_t cnt[kMaxDepth + 1];
+  __shared__ unsigned long long base[kMaxDepth + 1];
+  const int t = threadIdx.x;
+  if (t <= kMaxDepth) cnt[t] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kT + t;
+  const int d = i < nn ? depth[i] : -1;
+  const int local = d >= 0 ? atomicAdd(&cnt[d], 1) : 0;
+  __syncthreads();
+  if (t <= kMaxDepth && cnt[t])
+    base[t] = atomicAdd(&cursor[t], (unsigned long long)cnt[t]);
+  __syncthreads();
+  if (d >= 0) order[base[d] + local] = (int32_t)i;
 }
 
 struct CofmArgs {
