@@ -52,7 +52,7 @@ class FsphParams(C.Structure):
         ("adaptive_dt", C.c_int32), ("gravity", C.c_int32),
         ("group_max", C.c_int32), ("leaf_size", C.c_int32),
         ("periodic", C.c_int32 * 3), ("n_target", C.c_int32),
-        ("pad0", C.c_int32),
+        ("fmm_order", C.c_int32),
         ("gamma", C.c_double), ("poly_K", C.c_double), ("eta", C.c_double),
         ("visc_alpha", C.c_double), ("visc_beta", C.c_double),
         ("visc_eps", C.c_double), ("cfl", C.c_double),
@@ -80,6 +80,7 @@ def default_params(**kw):
     p.gravity = GRAV_NONE
     p.group_max = 128
     p.leaf_size = 32
+    p.fmm_order = 2
     p.n_target = 100
     p.gamma = 1.4
     p.poly_K = 1.0

@@ -189,9 +189,7 @@ int do_step(Ctx *c) {
   return do_advance(c);
 }
 
-int do_gravity(Ctx *c) {
-  return fail(c, FSPH_E_ARG, "gravity is not available in this build");
-}
+int do_gravity(Ctx *c) { return stage_gravity(c); }
 
 }  // namespace
 }  // namespace fsph
@@ -209,6 +207,7 @@ void fsph_default_params(fsph_params *p) {
   p->gravity = FSPH_GRAV_NONE;
   p->group_max = 128;
   p->leaf_size = 32;
+  p->fmm_order = 2;
   p->n_target = 100;
   p->gamma = 1.4;
   p->poly_K = 1.0;
@@ -266,6 +265,7 @@ void fsph_destroy(fsph_ctx *ctx) {
                        &c->ccount, &c->moff, &c->mask};
   for (auto *b : all)
     if (b->p) hipFree(b->p);
+  fsph::free_gravity(c);
   for (int f = 0; f < fsph::PF_COUNT; ++f) {
     if (c->f[f].p) hipFree(c->f[f].p);
     if (c->g[f].p) hipFree(c->g[f].p);

@@ -95,6 +95,17 @@ struct Ctx {
   int64_t mask_words = 0;
   bool have_tree = false;
 
+  // FMM gravity: cells (internal nodes and buckets), pair lists
+  struct Grav {
+    int64_t nc = 0, n_node_cells = 0, cap_cells = 0;
+    DBuf flag, pos, nodecell, partcell;
+    DBuf cnode, cstart, ccount, cbucket, cchild, ccom, cmass, crad, cQ, cL;
+    int64_t cap_q = 0, cap_m2l = 0, cap_p2p = 0;
+    int64_t n_m2l = 0, n_p2p = 0;
+    DBuf qa, qb, m2l, p2p, ctr;
+    DBuf m2l_off, m2l_src, p2p_off, p2p_src, cursor;
+  } grav;
+
   // time stepping
   int64_t iteration = 0;
   double dt = 0, totaltime = 0, totaltime_next = 0;
@@ -162,6 +173,8 @@ int stage_neighbors(Ctx *c);           // k_neighbors.hip
 int stage_density(Ctx *c);             // k_sph.hip
 int stage_acceleration(Ctx *c);        // k_sph.hip
 int stage_dudt(Ctx *c);                // k_sph.hip
+int stage_gravity(Ctx *c);             // k_grav.hip
+void free_gravity(Ctx *c);             // k_grav.hip
 int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
 int stage_update_h(Ctx *c);            // k_step.hip
 int stage_dt_min(Ctx *c, double *dtmin);     // k_step.hip

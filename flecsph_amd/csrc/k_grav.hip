@@ -1,0 +1,709 @@
+// k_grav.hip -- FMM self-gravity on the device octree.
+//
+// body_system::gravitation_fmm (bodies_system.h:282-288) runs the
+// dual-tree traversal of tree_topology.h:449-795: a breadth-first queue of
+// (sink, source) pairs; a pair is accepted when r_A + r_B < theta d
+// (tree_geometry.h:343-350), two leaves interact directly, a cell meets
+// itself by splitting into all child pairs, and otherwise the larger radius
+// is split.  Here the same recursion runs level-synchronously over FMM cells
+// (octree nodes with more than L particles, plus buckets: maximal nodes with
+// at most L particles and single particles under larger nodes), one thread
+// per pair with wave-aggregated appends.  Accepted pairs become M2L
+// interactions (order 1: monopole source and first-order local field, which
+// is the reference's scheme at L = 1; order 2: traceless quadrupole source
+// and second-order local expansion), bucket pairs become P2P interactions.
+// Lists are bucketed by sink; M2L runs one wavefront per sink cell with a
+// shuffle reduction, L2L goes down level by level, and one workgroup per
+// bucket evaluates its local expansion and sums its P2P sources from LDS.
+// Centres of mass and radii are the FP64 octree aggregates; multipole and
+// pair arithmetic is FP32 with FP64 position differences.
+#include <cfloat>
+
+#include "fsph_internal.h"
+
+namespace fsph {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kMaxLeaf = 64;
+inline unsigned blocks_for(int64_t n, int t = kT) {
+  return (unsigned)((n + t - 1) / t);
+}
+
+struct Cells {
+  const int32_t *node, *start, *count, *bucket, *child;
+  const double *com, *mass, *rad;
+};
+
+// ---------------------------------------------------------------------------
+// cells
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_cell_flags(
+    const int32_t *__restrict__ count, const int32_t *__restrict__ parent,
+    int64_t nn, const int32_t *__restrict__ leaf_parent, int64_t n, int L,
+    int32_t *__restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nn) {
+    const int32_t p = parent[i];
+    flag[i] = count[i] > L || p < 0 || count[p] > L;
+  } else if (i < nn + n) {
+    flag[i] = count[leaf_parent[i - nn]] > L;
+  }
+}
+
+struct CellOut {
+  int32_t *node, *start, *count, *bucket, *child, *nodecell, *partcell;
+  double *com, *mass, *rad;
+};
+
+__global__ __launch_bounds__(kT) void k_cell_write(
+    const int32_t *__restrict__ flag, const int64_t *__restrict__ pos,
+    int64_t nn, int64_t n, int L, const int32_t *__restrict__ nstart,
+    const int32_t *__restrict__ ncount, const double *__restrict__ ncom,
+    const double *__restrict__ nmass, const double *__restrict__ nrad,
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, CellOut o) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nn + n) return;
+  const int32_t c = flag[i] ? (int32_t)pos[i] : -1;
+  if (i < nn)
+    o.nodecell[i] = c;
+  else
+    o.partcell[i - nn] = c;
+  if (c < 0) return;
+  for (int d = 0; d < 8; ++d) o.child[8 * (int64_t)c + d] = -1;
+  if (i < nn) {
+    o.node[c] = (int32_t)i;
+    o.start[c] = nstart[i];
+    o.count[c] = ncount[i];
+    o.bucket[c] = ncount[i] <= L;
+    for (int d = 0; d < 3; ++d) o.com[3 * (int64_t)c + d] = ncom[3 * i + d];
+    o.mass[c] = nmass[i];
+    o.rad[c] = nrad[i];
+  } else {
+    const int64_t p = i - nn;
+    o.node[c] = -1;
+    o.start[c] = (int32_t)p;
+    o.count[c] = 1;
+    o.bucket[c] = 1;
+    o.com[3 * (int64_t)c] = x[p];
+    o.com[3 * (int64_t)c + 1] = y[p];
+    o.com[3 * (int64_t)c + 2] = z[p];
+    o.mass[c] = m[p];
+    o.rad[c] = 0;
+  }
+}
+
+// children of internal cells, in digit order
+__global__ __launch_bounds__(kT) void k_cell_children(
+    const int32_t *__restrict__ cnode, const int32_t *__restrict__ cbucket,
+    int64_t nc, const int32_t *__restrict__ nchild,
+    const int32_t *__restrict__ nodecell, const int32_t *__restrict__ partcell,
+    int32_t *__restrict__ cchild) {
+  const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (c >= nc || cbucket[c]) return;
+  const int64_t nd = cnode[c];
+  for (int d = 0; d < 8; ++d) {
+    const int32_t ch = nchild[8 * nd + d];
+    int32_t cc = -1;
+    if (ch >= 0)
+      cc = nodecell[ch];
+    else if (ch <= -2)
+      cc = partcell[leaf_of(ch)];
+    cchild[8 * c + d] = cc;
+  }
+}
+
+// traceless quadrupole of point masses about the cell's COM
+__device__ inline void add_quad(double Q[6], double m, double y0, double y1,
+                                double y2) {
+  const double yy = y0 * y0 + y1 * y1 + y2 * y2;
+  Q[0] += m * (3 * y0 * y0 - yy);
+  Q[1] += m * (3 * y0 * y1);
+  Q[2] += m * (3 * y0 * y2);
+  Q[3] += m * (3 * y1 * y1 - yy);
+  Q[4] += m * (3 * y1 * y2);
+  Q[5] += m * (3 * y2 * y2 - yy);
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_p2m(Cells c, int64_t nc,
+                                            const double *__restrict__ x,
+                                            const double *__restrict__ y,
+                                            const double *__restrict__ z,
+                                            const double *__restrict__ m,
+                                            T *__restrict__ Q) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nc || !c.bucket[i]) return;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  const double cx = c.com[3 * i], cy = c.com[3 * i + 1], cz = c.com[3 * i + 2];
+  for (int64_t p = c.start[i]; p < c.start[i] + c.count[i]; ++p)
+    add_quad(q, m[p], x[p] - cx, y[p] - cy, z[p] - cz);
+  for (int k = 0; k < 6; ++k) Q[6 * i + k] = (T)q[k];
+}
+
+// M2M for the internal cells among the nodes of one level
+template <class T>
+__global__ __launch_bounds__(kT) void k_m2m(Cells c,
+                                            const int32_t *__restrict__ nodes,
+                                            int64_t cnt,
+                                            const int32_t *__restrict__ nodecell,
+                                            T *__restrict__ Q) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t cell = nodecell[nodes[t]];
+  if (cell < 0 || c.bucket[cell]) return;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  const double cx = c.com[3 * cell], cy = c.com[3 * cell + 1],
+               cz = c.com[3 * cell + 2];
+  for (int d = 0; d < 8; ++d) {
+    const int32_t ch = c.child[8 * (int64_t)cell + d];
+    if (ch < 0) continue;
+    for (int k = 0; k < 6; ++k) q[k] += (double)Q[6 * (int64_t)ch + k];
+    add_quad(q, c.mass[ch], c.com[3 * ch] - cx, c.com[3 * ch + 1] - cy,
+             c.com[3 * ch + 2] - cz);
+  }
+  for (int k = 0; k < 6; ++k) Q[6 * (int64_t)cell + k] = (T)q[k];
+}
+
+// ---------------------------------------------------------------------------
+// dual-tree pair recursion
+// ---------------------------------------------------------------------------
+__device__ inline int wave_excl_scan(int v, int *total) {
+  const int lane = threadIdx.x & 63;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  *total = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+// Reserves `cnt` slots for this lane at a global counter, one atomic per
+// wavefront.  Every lane of the wave must call it.
+__device__ inline int64_t wave_reserve(unsigned long long *ctr, int cnt) {
+  int total;
+  const int pre = wave_excl_scan(cnt, &total);
+  unsigned long long base = 0;
+  if ((threadIdx.x & 63) == 0 && total)
+    base = atomicAdd(ctr, (unsigned long long)total);
+  base = __shfl(base, 0, 64);
+  return (int64_t)base + pre;
+}
+
+struct PairArgs {
+  Cells c;
+  const int2 *in;
+  int64_t n_in;
+  int2 *out, *m2l, *p2p;
+  unsigned long long *ctr;  // [0] out, [1] m2l, [2] p2p
+  int64_t cap_out, cap_m2l, cap_p2p;
+  double theta;
+  int *flags;
+};
+
+__global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  int A = -1, B = -1, kind = 0, nout = 0;
+  int32_t ch[8];
+  int nch = 0;
+  if (i < a.n_in) {
+    const int2 pr = a.in[i];
+    A = pr.x;
+    B = pr.y;
+    if (A == B) {
+      if (a.c.bucket[A]) {
+        kind = 3;
+      } else {
+        kind = 1;
+        for (int d = 0; d < 8; ++d) {
+          const int32_t q = a.c.child[8 * (int64_t)A + d];
+          if (q >= 0) ch[nch++] = q;
+        }
+        nout = nch * nch;
+      }
+    } else {
+#pragma clang fp contract(off)
+      const double *ca = a.c.com + 3 * (int64_t)A, *cb = a.c.com + 3 * (int64_t)B;
+      double s = 0;
+      for (int d = 0; d < 3; ++d) s += (ca[d] - cb[d]) * (ca[d] - cb[d]);
+      const double ra = a.c.rad[A], rb = a.c.rad[B];
+      const bool ba = a.c.bucket[A], bb = a.c.bucket[B];
+      if (ra + rb < a.theta * sqrt(s)) {
+        kind = 2;
+      } else if (ba && bb) {
+        kind = 3;
+      } else {
+        const bool split_a = !ba && (bb || ra > rb);
+        kind = split_a ? 4 : 5;
+        const int32_t S = split_a ? A : B;
+        for (int d = 0; d < 8; ++d) {
+          const int32_t q = a.c.child[8 * (int64_t)S + d];
+          if (q >= 0) ch[nch++] = q;
+        }
+        nout = nch;
+      }
+    }
+  }
+  const int64_t o = wave_reserve(&a.ctr[0], nout);
+  const int64_t mo = wave_reserve(&a.ctr[1], kind == 2);
+  const int64_t po = wave_reserve(&a.ctr[2], kind == 3);
+  if (kind == 0) return;
+  if (nout) {
+    if (o + nout > a.cap_out) {
+      atomicOr(a.flags, 1);
+      return;
+    }
+    int64_t w = o;
+    if (kind == 1) {
+      for (int k1 = 0; k1 < nch; ++k1) {
+        a.out[w++] = make_int2(ch[k1], ch[k1]);
+        for (int k2 = k1 + 1; k2 < nch; ++k2) {
+          a.out[w++] = make_int2(ch[k1], ch[k2]);
+          a.out[w++] = make_int2(ch[k2], ch[k1]);
+        }
+      }
+    } else if (kind == 4) {
+      for (int k = 0; k < nch; ++k) a.out[w++] = make_int2(ch[k], B);
+    } else {
+      for (int k = 0; k < nch; ++k) a.out[w++] = make_int2(A, ch[k]);
+    }
+  } else if (kind == 2) {
+    if (mo < a.cap_m2l)
+      a.m2l[mo] = make_int2(A, B);
+    else
+      atomicOr(a.flags, 2);
+  } else if (kind == 3) {
+    if (po < a.cap_p2p)
+      a.p2p[po] = make_int2(A, B);
+    else
+      atomicOr(a.flags, 4);
+  }
+}
+
+// counting sort of a pair list by sink: counts, then scatter of sources
+__global__ __launch_bounds__(kT) void k_count_sinks(const int2 *__restrict__ l,
+                                                    int64_t n,
+                                                    int32_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[l[i].x], 1);
+}
+
+__global__ __launch_bounds__(kT) void k_scatter_sinks(
+    const int2 *__restrict__ l, int64_t n,
+    unsigned long long *__restrict__ cursor, int32_t *__restrict__ src) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int2 pr = l[i];
+  src[atomicAdd(&cursor[pr.x], 1ull)] = pr.y;
+}
+
+// ---------------------------------------------------------------------------
+// M2L, L2L, L2P + P2P
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Local expansion (phi, grad phi, Hessian phi) at the sink centre from a
+// source of mass M and traceless quadrupole Q, R = c_sink - c_source:
+// phi = -G [M/R + Q:RR / (2 R^5)].
+template <class T>
+__device__ inline void m2l_add(T G, int order, T R0, T R1, T R2, T M,
+                               const T *Q, T *L) {
+  const T r2 = R0 * R0 + R1 * R1 + R2 * R2;
+  const T ir = T(1) / sqrt(r2), ir2 = ir * ir, ir3 = ir * ir2;
+  const T GM = G * M;
+  L[0] += -GM * ir;
+  L[1] += GM * R0 * ir3;
+  L[2] += GM * R1 * ir3;
+  L[3] += GM * R2 * ir3;
+  if (order < 2) return;
+  const T ir5 = ir3 * ir2, ir7 = ir5 * ir2;
+  const T QR0 = Q[0] * R0 + Q[1] * R1 + Q[2] * R2;
+  const T QR1 = Q[1] * R0 + Q[3] * R1 + Q[4] * R2;
+  const T QR2 = Q[2] * R0 + Q[4] * R1 + Q[5] * R2;
+  const T QRR = QR0 * R0 + QR1 * R1 + QR2 * R2;
+  L[0] += -G * T(0.5) * QRR * ir5;
+  const T f = T(2.5) * QRR * ir7;
+  L[1] += -G * (QR0 * ir5 - f * R0);
+  L[2] += -G * (QR1 * ir5 - f * R1);
+  L[3] += -G * (QR2 * ir5 - f * R2);
+  const T h = -GM * ir5;
+  L[4] += h * (T(3) * R0 * R0 - r2);
+  L[5] += h * (T(3) * R0 * R1);
+  L[6] += h * (T(3) * R0 * R2);
+  L[7] += h * (T(3) * R1 * R1 - r2);
+  L[8] += h * (T(3) * R1 * R2);
+  L[9] += h * (T(3) * R2 * R2 - r2);
+}
+
+template <class T> struct M2LArgs {
+  Cells c;
+  const int64_t *off;
+  const int32_t *src;
+  const T *Q;
+  T *L;
+  T G;
+  int order;
+};
+
+template <class T>
+__global__ __launch_bounds__(64) void k_m2l(M2LArgs<T> a) {
+  const int64_t cell = blockIdx.x;
+  const int64_t b0 = a.off[cell], b1 = a.off[cell + 1];
+  if (b0 == b1) return;
+  T L[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double cx = a.c.com[3 * cell], cy = a.c.com[3 * cell + 1],
+               cz = a.c.com[3 * cell + 2];
+  for (int64_t j = b0 + threadIdx.x; j < b1; j += 64) {
+    const int64_t s = a.src[j];
+    const T R0 = (T)(cx - a.c.com[3 * s]), R1 = (T)(cy - a.c.com[3 * s + 1]),
+            R2 = (T)(cz - a.c.com[3 * s + 2]);
+    T Q[6];
+    for (int k = 0; k < 6; ++k) Q[k] = a.Q[6 * s + k];
+    m2l_add<T>(a.G, a.order, R0, R1, R2, (T)a.c.mass[s], Q, L);
+  }
+  for (int k = 0; k < 10; ++k) L[k] = wave_sum(L[k]);
+  if (threadIdx.x < 10) {
+    T v = L[0];
+    for (int k = 1; k < 10; ++k)
+      if (threadIdx.x == k) v = L[k];
+    a.L[10 * cell + threadIdx.x] = v;
+  }
+}
+
+// value and gradient of a local expansion at offset y
+template <class T>
+__device__ inline void local_eval(const T *L, T y0, T y1, T y2, T *phi,
+                                  T *g0, T *g1, T *g2) {
+  const T Hy0 = L[4] * y0 + L[5] * y1 + L[6] * y2;
+  const T Hy1 = L[5] * y0 + L[7] * y1 + L[8] * y2;
+  const T Hy2 = L[6] * y0 + L[8] * y1 + L[9] * y2;
+  *phi = L[0] + L[1] * y0 + L[2] * y1 + L[3] * y2 +
+         T(0.5) * (Hy0 * y0 + Hy1 * y1 + Hy2 * y2);
+  *g0 = L[1] + Hy0;
+  *g1 = L[2] + Hy1;
+  *g2 = L[3] + Hy2;
+}
+
+template <class T>
+__device__ inline void l2l_one(Cells c, T *Lall, int64_t cell, int64_t par) {
+  const T y0 = (T)(c.com[3 * cell] - c.com[3 * par]);
+  const T y1 = (T)(c.com[3 * cell + 1] - c.com[3 * par + 1]);
+  const T y2 = (T)(c.com[3 * cell + 2] - c.com[3 * par + 2]);
+  const T *Lp = Lall + 10 * par;
+  T *Lc = Lall + 10 * cell;
+  T phi, g0, g1, g2;
+  local_eval(Lp, y0, y1, y2, &phi, &g0, &g1, &g2);
+  Lc[0] += phi;
+  Lc[1] += g0;
+  Lc[2] += g1;
+  Lc[3] += g2;
+  for (int k = 4; k < 10; ++k) Lc[k] += Lp[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_l2l_level(
+    Cells c, const int32_t *__restrict__ nodes, int64_t cnt,
+    const int32_t *__restrict__ nodecell, const int32_t *__restrict__ nparent,
+    T *L) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t nd = nodes[t];
+  const int32_t cell = nodecell[nd];
+  const int32_t pn = nparent[nd];
+  if (cell < 0 || pn < 0) return;
+  l2l_one<T>(c, L, cell, nodecell[pn]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_l2l_single(
+    Cells c, const int32_t *__restrict__ partcell, int64_t n,
+    const int32_t *__restrict__ leaf_parent,
+    const int32_t *__restrict__ nodecell, T *L) {
+  const int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (p >= n) return;
+  const int32_t cell = partcell[p];
+  if (cell < 0) return;
+  l2l_one<T>(c, L, cell, nodecell[leaf_parent[p]]);
+}
+
+template <class T> struct KS {
+  T s = 0, c = 0;
+  __device__ inline void add(T v) {
+    const T y = v - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ inline T get() const { return s - c; }
+};
+
+template <class T> struct EvalArgs {
+  Cells c;
+  int64_t nc;
+  const int64_t *off;
+  const int32_t *src;
+  const T *L;
+  const double *x, *y, *z, *m;
+  T *gx, *gy, *gz, *phi;
+  T G;
+};
+
+// One workgroup per bucket: local expansion plus direct sums over the
+// bucket's P2P sources staged in LDS (unsoftened, skipping the particle
+// itself, fmm.h:37-46, 124-143).
+template <class T>
+__global__ __launch_bounds__(kMaxLeaf) void k_eval(EvalArgs<T> a) {
+  __shared__ double sx[kMaxLeaf], sy[kMaxLeaf], sz[kMaxLeaf];
+  __shared__ T sm[kMaxLeaf];
+  const int64_t cell = blockIdx.x;
+  if (!a.c.bucket[cell]) return;
+  const int cnt = a.c.count[cell];
+  const int64_t st = a.c.start[cell];
+  const int t = threadIdx.x;
+  const int64_t p = st + t;
+  const bool act = t < cnt;
+  double px = 0, py = 0, pz = 0;
+  KS<T> f0, f1, f2, pp;
+  if (act) {
+    px = a.x[p]; py = a.y[p]; pz = a.z[p];
+    T phi, g0, g1, g2;
+    local_eval(a.L + 10 * cell, (T)(px - a.c.com[3 * cell]),
+               (T)(py - a.c.com[3 * cell + 1]), (T)(pz - a.c.com[3 * cell + 2]),
+               &phi, &g0, &g1, &g2);
+    pp.add(phi);
+    f0.add(-g0);
+    f1.add(-g1);
+    f2.add(-g2);
+  }
+  for (int64_t j = a.off[cell]; j < a.off[cell + 1]; ++j) {
+    const int64_t s = a.src[j];
+    const int sc = a.c.count[s];
+    const int64_t ss = a.c.start[s];
+    __syncthreads();
+    if (t < sc) {
+      sx[t] = a.x[ss + t]; sy[t] = a.y[ss + t]; sz[t] = a.z[ss + t];
+      sm[t] = (T)a.m[ss + t];
+    }
+    __syncthreads();
+    if (!act) continue;
+    for (int q = 0; q < sc; ++q) {
+      if (ss + q == p) continue;
+      const T d0 = (T)(px - sx[q]), d1 = (T)(py - sy[q]), d2 = (T)(pz - sz[q]);
+      const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const T ir = T(1) / sqrt(r2);
+      const T gm = a.G * sm[q] * ir;
+      pp.add(-gm);
+      const T fr = -gm * ir * ir;
+      f0.add(fr * d0);
+      f1.add(fr * d1);
+      f2.add(fr * d2);
+    }
+  }
+  if (act) {
+    a.gx[p] = f0.get();
+    a.gy[p] = f1.get();
+    a.gz[p] = f2.get();
+    a.phi[p] = pp.get();
+  }
+}
+
+int bucket_sort(Ctx *c, const int2 *list, int64_t n, int64_t nc, DBuf &off,
+                DBuf &src) {
+  auto &G = c->grav;
+  FSPH_TRY(ensure(c, off, (size_t)(nc + 1) * 8));
+  FSPH_TRY(ensure(c, src, (size_t)(n > 0 ? n : 1) * 4));
+  FSPH_TRY(ensure(c, G.cursor, (size_t)(nc + 1) * 8));
+  int32_t *cnt = G.flag.as<int32_t>();  // scratch of >= nc ints
+  FSPH_HIP(c, hipMemsetAsync(cnt, 0, (size_t)nc * 4, c->stream));
+  if (n) k_count_sinks<<<blocks_for(n), kT, 0, c->stream>>>(list, n, cnt);
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i32_to_i64(c, cnt, off.as<int64_t>(), nc));
+  FSPH_HIP(c, hipMemcpyAsync(G.cursor.p, off.p, (size_t)nc * 8,
+                             hipMemcpyDeviceToDevice, c->stream));
+  if (n)
+    k_scatter_sinks<<<blocks_for(n), kT, 0, c->stream>>>(
+        list, n, G.cursor.as<unsigned long long>(), src.as<int32_t>());
+  return hip_check(c, hipGetLastError(), "k_scatter_sinks");
+}
+
+template <class T> int gravity_impl(Ctx *c) {
+  auto &G = c->grav;
+  const int64_t n = c->n, nn = c->nn;
+  const int L = c->p.leaf_size;
+  const int order = c->p.fmm_order >= 2 ? 2 : 1;
+  // ---- cells
+  const int64_t tot = nn + n;
+  FSPH_TRY(ensure(c, G.flag, (size_t)(tot + 1) * 4));
+  FSPH_TRY(ensure(c, G.pos, (size_t)(tot + 1) * 8));
+  FSPH_TRY(ensure(c, G.nodecell, (size_t)(nn + 1) * 4));
+  FSPH_TRY(ensure(c, G.partcell, (size_t)(n + 1) * 4));
+  k_cell_flags<<<blocks_for(tot), kT, 0, c->stream>>>(
+      c->ncount.as<int32_t>(), c->nparent.as<int32_t>(), nn,
+      c->leaf_parent.as<int32_t>(), n, L, G.flag.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i32_to_i64(c, G.flag.as<int32_t>(), G.pos.as<int64_t>(), tot));
+  int64_t nc = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&nc, G.pos.as<int64_t>() + tot, 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  G.nc = nc;
+  if (nc > G.cap_cells) {
+    const int64_t cap = nc + nc / 4 + 1024;
+    FSPH_TRY(ensure(c, G.cnode, cap * 4));
+    FSPH_TRY(ensure(c, G.cstart, cap * 4));
+    FSPH_TRY(ensure(c, G.ccount, cap * 4));
+    FSPH_TRY(ensure(c, G.cbucket, cap * 4));
+    FSPH_TRY(ensure(c, G.cchild, cap * 32));
+    FSPH_TRY(ensure(c, G.ccom, cap * 24));
+    FSPH_TRY(ensure(c, G.cmass, cap * 8));
+    FSPH_TRY(ensure(c, G.crad, cap * 8));
+    FSPH_TRY(ensure(c, G.cQ, cap * 6 * sizeof(T)));
+    FSPH_TRY(ensure(c, G.cL, cap * 10 * sizeof(T)));
+    G.cap_cells = cap;
+  }
+  CellOut co{G.cnode.as<int32_t>(),   G.cstart.as<int32_t>(),
+             G.ccount.as<int32_t>(),  G.cbucket.as<int32_t>(),
+             G.cchild.as<int32_t>(),  G.nodecell.as<int32_t>(),
+             G.partcell.as<int32_t>(), G.ccom.as<double>(),
+             G.cmass.as<double>(),    G.crad.as<double>()};
+  k_cell_write<<<blocks_for(tot), kT, 0, c->stream>>>(
+      G.flag.as<int32_t>(), G.pos.as<int64_t>(), nn, n, L,
+      c->nstart.as<int32_t>(), c->ncount.as<int32_t>(), c->ncom.as<double>(),
+      c->nmass.as<double>(), c->nrad.as<double>(), c->f[PF_X].as<double>(),
+      c->f[PF_Y].as<double>(), c->f[PF_Z].as<double>(),
+      c->f[PF_M].as<double>(), co);
+  k_cell_children<<<blocks_for(nc), kT, 0, c->stream>>>(
+      G.cnode.as<int32_t>(), G.cbucket.as<int32_t>(), nc,
+      c->nchild.as<int32_t>(), G.nodecell.as<int32_t>(),
+      G.partcell.as<int32_t>(), G.cchild.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  Cells cells{G.cnode.as<int32_t>(), G.cstart.as<int32_t>(),
+              G.ccount.as<int32_t>(), G.cbucket.as<int32_t>(),
+              G.cchild.as<int32_t>(), G.ccom.as<double>(),
+              G.cmass.as<double>(),  G.crad.as<double>()};
+  T *Q = G.cQ.as<T>();
+  T *Lc = G.cL.as<T>();
+  // ---- multipoles: P2M on buckets, M2M up the node levels
+  k_p2m<T><<<blocks_for(nc), kT, 0, c->stream>>>(
+      cells, nc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(), Q);
+  for (int d = c->max_depth; d >= 0; --d) {
+    const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
+    if (!cnt) continue;
+    k_m2m<T><<<blocks_for(cnt), kT, 0, c->stream>>>(
+        cells, c->nlevel.as<int32_t>() + c->level_off[d], cnt,
+        G.nodecell.as<int32_t>(), Q);
+  }
+  FSPH_HIP(c, hipGetLastError());
+  // ---- pair recursion
+  if (G.cap_q == 0) {
+    G.cap_q = std::max<int64_t>(1 << 16, 64 * nc);
+    G.cap_m2l = std::max<int64_t>(1 << 16, 256 * nc);
+    G.cap_p2p = std::max<int64_t>(1 << 16, 64 * nc);
+  }
+  FSPH_TRY(ensure(c, G.ctr, 64));
+  for (int attempt = 0;; ++attempt) {
+    FSPH_TRY(ensure(c, G.qa, (size_t)G.cap_q * 8));
+    FSPH_TRY(ensure(c, G.qb, (size_t)G.cap_q * 8));
+    FSPH_TRY(ensure(c, G.m2l, (size_t)G.cap_m2l * 8));
+    FSPH_TRY(ensure(c, G.p2p, (size_t)G.cap_p2p * 8));
+    unsigned long long *ctr = G.ctr.as<unsigned long long>();
+    FSPH_HIP(c, hipMemsetAsync(ctr, 0, 64, c->stream));
+    FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+    const int2 root = make_int2(0, 0);
+    FSPH_HIP(c, hipMemcpyAsync(G.qa.p, &root, 8, hipMemcpyHostToDevice,
+                               c->stream));
+    int64_t n_in = 1;
+    int2 *qin = G.qa.as<int2>(), *qout = G.qb.as<int2>();
+    unsigned long long h[3] = {0, 0, 0};
+    int fl = 0;
+    while (n_in > 0) {
+      FSPH_HIP(c, hipMemsetAsync(ctr, 0, 8, c->stream));
+      PairArgs pa{cells, qin, n_in, qout, G.m2l.as<int2>(), G.p2p.as<int2>(),
+                  ctr, G.cap_q, G.cap_m2l, G.cap_p2p, c->p.theta,
+                  c->flags.as<int>()};
+      k_fmm_pairs<<<blocks_for(n_in), kT, 0, c->stream>>>(pa);
+      FSPH_HIP(c, hipGetLastError());
+      FSPH_HIP(c, hipMemcpyAsync(h, ctr, 24, hipMemcpyDeviceToHost,
+                                 c->stream));
+      FSPH_TRY(read_flags(c, &fl));
+      if (fl) break;
+      n_in = (int64_t)h[0];
+      std::swap(qin, qout);
+    }
+    if (!fl) {
+      G.n_m2l = (int64_t)h[1];
+      G.n_p2p = (int64_t)h[2];
+      break;
+    }
+    if (attempt >= 8) return fail(c, FSPH_E_CAP, "FMM pair lists overflow");
+    if (fl & 1) G.cap_q *= 2;
+    if (fl & 2) G.cap_m2l = std::max<int64_t>(2 * G.cap_m2l, (int64_t)h[1] + 1);
+    if (fl & 4) G.cap_p2p = std::max<int64_t>(2 * G.cap_p2p, (int64_t)h[2] + 1);
+  }
+  // ---- bucket the lists by sink
+  FSPH_TRY(bucket_sort(c, G.m2l.as<int2>(), G.n_m2l, nc, G.m2l_off,
+                       G.m2l_src));
+  FSPH_TRY(bucket_sort(c, G.p2p.as<int2>(), G.n_p2p, nc, G.p2p_off,
+                       G.p2p_src));
+  // ---- M2L, L2L
+  FSPH_HIP(c, hipMemsetAsync(Lc, 0, (size_t)nc * 10 * sizeof(T), c->stream));
+  M2LArgs<T> ma{cells, G.m2l_off.as<int64_t>(), G.m2l_src.as<int32_t>(), Q, Lc,
+                (T)c->p.G, order};
+  k_m2l<T><<<(unsigned)nc, 64, 0, c->stream>>>(ma);
+  FSPH_HIP(c, hipGetLastError());
+  for (int d = 1; d <= c->max_depth; ++d) {
+    const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
+    if (!cnt) continue;
+    k_l2l_level<T><<<blocks_for(cnt), kT, 0, c->stream>>>(
+        cells, c->nlevel.as<int32_t>() + c->level_off[d], cnt,
+        G.nodecell.as<int32_t>(), c->nparent.as<int32_t>(), Lc);
+  }
+  k_l2l_single<T><<<blocks_for(n), kT, 0, c->stream>>>(
+      cells, G.partcell.as<int32_t>(), n, c->leaf_parent.as<int32_t>(),
+      G.nodecell.as<int32_t>(), Lc);
+  FSPH_HIP(c, hipGetLastError());
+  // ---- L2P + P2P
+  EvalArgs<T> ea{cells, nc, G.p2p_off.as<int64_t>(), G.p2p_src.as<int32_t>(),
+                 Lc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+                 c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(),
+                 c->f[PF_GX].as<T>(), c->f[PF_GY].as<T>(), c->f[PF_GZ].as<T>(),
+                 c->f[PF_PHI].as<T>(), (T)c->p.G};
+  k_eval<T><<<(unsigned)nc, kMaxLeaf, 0, c->stream>>>(ea);
+  return hip_check(c, hipGetLastError(), "k_eval");
+}
+
+}  // namespace
+
+int stage_gravity(Ctx *c) {
+  if (c->p.leaf_size < 1 || c->p.leaf_size > kMaxLeaf)
+    return fail(c, FSPH_E_ARG, "leaf_size must be in [1, %d]", kMaxLeaf);
+  if (!(c->p.theta > 0))
+    return fail(c, FSPH_E_ARG, "fmm_macangle (theta) must be positive");
+  return c->rsize == 4 ? gravity_impl<float>(c) : gravity_impl<double>(c);
+}
+
+void free_gravity(Ctx *c) {
+  auto &G = c->grav;
+  DBuf *all[] = {&G.flag,    &G.pos,     &G.nodecell, &G.partcell, &G.cnode,
+                 &G.cstart,  &G.ccount,  &G.cbucket,  &G.cchild,   &G.ccom,
+                 &G.cmass,   &G.crad,    &G.cQ,       &G.cL,       &G.qa,
+                 &G.qb,      &G.m2l,     &G.p2p,      &G.ctr,      &G.m2l_off,
+                 &G.m2l_src, &G.p2p_off, &G.p2p_src,  &G.cursor};
+  for (DBuf *b : all)
+    if (b->p) {
+      hipFree(b->p);
+      b->p = nullptr;
+      b->bytes = 0;
+    }
+}
+
+}  // namespace fsph

@@ -72,7 +72,10 @@ typedef struct fsph_params {
   int32_t leaf_size;   /* FMM leaf bucket size                              */
   int32_t periodic[3]; /* periodic_boundary_x/y/z                           */
   int32_t n_target;    /* target neighbour count (smoothing-length solvers) */
-  int32_t pad0;
+  int32_t fmm_order;   /* 1: monopole sources, first-order local field (the
+                          reference's scheme, fmm.h:37-159, with leaf_size
+                          1); 2: quadrupole sources, second-order local
+                          expansion                                         */
   double gamma;        /* poly_gamma                                        */
   double poly_K;       /* adiabatic constant of the polytropic EOS          */
   double eta;          /* sph_eta                                           */

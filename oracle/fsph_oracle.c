@@ -1202,10 +1202,400 @@ void fo_direct_gravity(int64_t n, const double *x, const double *y,
   }
 }
 
-int fo_gravity(fo_sys *s) {
-  fo_direct_gravity(s->n, s->x, s->y, s->z, s->m, s->p.G, s->gx, s->gy, s->gz,
-                    s->phi);
+/* --- the reference's scheme, restated literally ------------------------ */
+/* tree_topology::traversal_fmm (tree_topology.h:449-795) with
+   fmm_sub_entities_ = 0 and the callbacks of fmm.h:37-159: a breadth-first
+   queue of (sink, source) key pairs; accepted node sinks accumulate a
+   monopole potential pc and field fc at their COM, accepted particle sinks
+   take the monopole directly, particle pairs are summed directly, and each
+   affected node finally adds fc and pc - r.fc to every particle below it. */
+typedef struct {
+  int32_t a, b;
+} pair32;
+
+typedef struct {
+  pair32 *v;
+  int64_t n, cap;
+} pvec;
+
+static void pvec_push(pvec *q, int32_t a, int32_t b) {
+  if (q->n == q->cap) {
+    q->cap = q->cap ? 2 * q->cap : 1024;
+    q->v = (pair32 *)realloc(q->v, (size_t)q->cap * sizeof(pair32));
+  }
+  q->v[q->n].a = a;
+  q->v[q->n].b = b;
+  q->n++;
+}
+
+/* gravitation_fc (fmm.h:51-85): monopole of mass M at `src` seen at `loc` */
+static void grav_fc(double G, double *pc, double fc[3], const double loc[3],
+                    const double src[3], double M) {
+  double sum = 0;
+  for (int d = 0; d < 3; ++d) sum += (loc[d] - src[d]) * (loc[d] - src[d]);
+  double dd = sqrt(sum);
+  double d3 = dd * dd * dd;
+  *pc += -G * M / dd;
+  for (int m = 0; m < 3; ++m) fc[m] += -G * M * (loc[m] - src[m]) / d3;
+}
+
+static void cell_geom(const fo_sys *s, int32_t c, double X[3], double *r) {
+  if (c >= 0) {
+    X[0] = s->ncom[3 * c]; X[1] = s->ncom[3 * c + 1]; X[2] = s->ncom[3 * c + 2];
+    *r = s->nrad[c];
+  } else {
+    int64_t p = LEAF_OF(c);
+    X[0] = s->x[p]; X[1] = s->y[p]; X[2] = s->z[p];
+    *r = 0;
+  }
+}
+
+static int fmm_reference(fo_sys *s) {
+  const double G = s->p.G, theta = s->p.theta;
+  int64_t nn = s->nn;
+  double *pc = (double *)calloc((size_t)nn, sizeof(double));
+  double *fc = (double *)calloc((size_t)nn * 3, sizeof(double));
+  char *aff = (char *)calloc((size_t)nn, 1);
+  pvec q = {0}, nq = {0}, p2p = {0};
+  pvec_push(&q, 0, 0);
+  while (q.n) {
+    nq.n = 0;
+    for (int64_t i = 0; i < q.n; ++i) {
+      int32_t a = q.v[i].a, b = q.v[i].b;
+      if (a < -1 && b < -1) {
+        pvec_push(&p2p, a, b);
+        continue;
+      }
+      if (a == b) { /* split for self-interaction (tree_topology.h:505-527) */
+        int32_t ch[8];
+        int nc = 0;
+        for (int d = 0; d < 8; ++d)
+          if (s->nchild[a * 8 + d] != CHILD_EMPTY) ch[nc++] = s->nchild[a * 8 + d];
+        for (int k1 = 0; k1 < nc; ++k1) {
+          pvec_push(&nq, ch[k1], ch[k1]);
+          for (int k2 = k1 + 1; k2 < nc; ++k2) {
+            pvec_push(&nq, ch[k1], ch[k2]);
+            pvec_push(&nq, ch[k2], ch[k1]);
+          }
+        }
+        continue;
+      }
+      double c1[3], c2[3], r1, r2;
+      cell_geom(s, a, c1, &r1);
+      cell_geom(s, b, c2, &r2);
+      double sum = 0;
+      for (int d = 0; d < 3; ++d) sum += (c1[d] - c2[d]) * (c1[d] - c2[d]);
+      if (r1 + r2 < theta * sqrt(sum)) { /* mac, tree_geometry.h:343-350 */
+        if (a >= 0) {
+          double M = b >= 0 ? s->nmass[b] : s->m[LEAF_OF(b)];
+          grav_fc(G, &pc[a], &fc[3 * a], c1, c2, M); /* taylor_c2c/p2c */
+          aff[a] = 1;
+        } else { /* fmm_p2p with a node source (fmm.h:124-143) */
+          int64_t p = LEAF_OF(a);
+          double g3[3] = {s->gx[p], s->gy[p], s->gz[p]};
+          grav_fc(G, &s->phi[p], g3, c1, c2, s->nmass[b]);
+          s->gx[p] = g3[0]; s->gy[p] = g3[1]; s->gz[p] = g3[2];
+        }
+      } else if (r1 > r2) { /* split the bigger one (tree_topology.h:614-632) */
+        for (int d = 0; d < 8; ++d)
+          if (s->nchild[a * 8 + d] != CHILD_EMPTY)
+            pvec_push(&nq, s->nchild[a * 8 + d], b);
+      } else {
+        for (int d = 0; d < 8; ++d)
+          if (s->nchild[b * 8 + d] != CHILD_EMPTY)
+            pvec_push(&nq, a, s->nchild[b * 8 + d]);
+      }
+    }
+    pvec t = q; q = nq; nq = t;
+  }
+  /* c2p for affected nodes in depth-first order (tree_topology.h:681-718) */
+  int64_t cap = 1024, top = 0;
+  int32_t *stk = (int32_t *)malloc((size_t)cap * 4);
+  stk[top++] = 0;
+  while (top) {
+    int32_t c = stk[--top];
+    if (c < 0) continue;
+    if (aff[c]) {
+      for (int64_t p = s->nstart[c]; p < s->nstart[c] + s->ncount[c]; ++p) {
+        double r[3] = {s->x[p] - s->ncom[3 * c], s->y[p] - s->ncom[3 * c + 1],
+                       s->z[p] - s->ncom[3 * c + 2]};
+        double pot = pc[c];
+        for (int d = 0; d < 3; ++d) pot += -r[d] * fc[3 * c + d];
+        s->phi[p] = s->phi[p] + pot;
+        s->gx[p] = fc[3 * c] + s->gx[p];
+        s->gy[p] = fc[3 * c + 1] + s->gy[p];
+        s->gz[p] = fc[3 * c + 2] + s->gz[p];
+      }
+    }
+    for (int d = 0; d < 8; ++d) {
+      int32_t ch = s->nchild[c * 8 + d];
+      if (ch == CHILD_EMPTY) continue;
+      if (top == cap) {
+        cap *= 2;
+        stk = (int32_t *)realloc(stk, (size_t)cap * 4);
+      }
+      stk[top++] = ch;
+    }
+  }
+  free(stk);
+  /* direct particle pairs (fmm.h:124-143, gravitation_p2p fmm.h:37-46) */
+  for (int64_t i = 0; i < p2p.n; ++i) {
+    int64_t pa = LEAF_OF(p2p.v[i].a), pb = LEAF_OF(p2p.v[i].b);
+    if (s->id[pa] == s->id[pb]) continue;
+    double d0 = s->x[pa] - s->x[pb], d1 = s->y[pa] - s->y[pb],
+           d2 = s->z[pa] - s->z[pb];
+    double sum = 0;
+    sum += d0 * d0; sum += d1 * d1; sum += d2 * d2;
+    double dist = sqrt(sum);
+    s->phi[pa] += -G * s->m[pb] / dist;
+    double f = -G * s->m[pb] / (dist * dist * dist);
+    s->gx[pa] += d0 * f; s->gy[pa] += d1 * f; s->gz[pa] += d2 * f;
+  }
+  free(q.v); free(nq.v); free(p2p.v);
+  free(pc); free(fc); free(aff);
   return 0;
+}
+
+/* --- the FMM engine the HIP path follows ------------------------------- */
+/* Cells are octree nodes with more than L particles (internal) and the
+   maximal nodes with at most L particles or single particles under a larger
+   node (buckets).  Order 1: monopole sources and a first-order local field
+   (the reference's scheme when L = 1); order 2: traceless quadrupole
+   sources, Q_ij = sum m (3 y_i y_j - y^2 delta_ij) about the COM, and a
+   second-order local expansion (L0, grad, Hessian of the potential).  The
+   (sink, source) pair recursion, the MAC r_A + r_B < theta d and the rule
+   "split the larger radius" are the reference's. */
+typedef struct {
+  int64_t nc;
+  int32_t *node, *start, *count, *bucket, *child; /* child: 8 per cell */
+  double *com, *mass, *rad, *Q, *L;               /* Q: 6, L: 10 per cell */
+} fmm_cells;
+
+static int32_t make_cell(fo_sys *s, fmm_cells *C, int32_t code, int L) {
+  int32_t me = (int32_t)C->nc++;
+  for (int d = 0; d < 8; ++d) C->child[8 * me + d] = -1;
+  for (int k = 0; k < 6; ++k) C->Q[6 * me + k] = 0;
+  for (int k = 0; k < 10; ++k) C->L[10 * me + k] = 0;
+  if (code < -1) {
+    int64_t p = LEAF_OF(code);
+    C->node[me] = -1; C->start[me] = (int32_t)p; C->count[me] = 1;
+    C->bucket[me] = 1;
+    C->com[3 * me] = s->x[p]; C->com[3 * me + 1] = s->y[p];
+    C->com[3 * me + 2] = s->z[p];
+    C->mass[me] = s->m[p]; C->rad[me] = 0;
+    return me;
+  }
+  C->node[me] = code; C->start[me] = s->nstart[code];
+  C->count[me] = s->ncount[code];
+  C->bucket[me] = s->ncount[code] <= L;
+  for (int d = 0; d < 3; ++d) C->com[3 * me + d] = s->ncom[3 * code + d];
+  C->mass[me] = s->nmass[code];
+  C->rad[me] = s->nrad[code];
+  if (!C->bucket[me])
+    for (int d = 0; d < 8; ++d) {
+      int32_t ch = s->nchild[code * 8 + d];
+      if (ch != CHILD_EMPTY) C->child[8 * me + d] = make_cell(s, C, ch, L);
+    }
+  return me;
+}
+
+static void add_quad(double Q[6], double m, const double y[3]) {
+  double y2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+  Q[0] += m * (3 * y[0] * y[0] - y2);
+  Q[1] += m * (3 * y[0] * y[1]);
+  Q[2] += m * (3 * y[0] * y[2]);
+  Q[3] += m * (3 * y[1] * y[1] - y2);
+  Q[4] += m * (3 * y[1] * y[2]);
+  Q[5] += m * (3 * y[2] * y[2] - y2);
+}
+
+/* P2M for buckets, M2M (parallel-axis shift) for internal cells; children
+   have larger ids than parents. */
+static void cell_multipoles(fo_sys *s, fmm_cells *C) {
+  for (int64_t c = C->nc - 1; c >= 0; --c) {
+    double *Q = C->Q + 6 * c, *X = C->com + 3 * c;
+    if (C->bucket[c]) {
+      for (int64_t p = C->start[c]; p < C->start[c] + C->count[c]; ++p) {
+        double y[3] = {s->x[p] - X[0], s->y[p] - X[1], s->z[p] - X[2]};
+        add_quad(Q, s->m[p], y);
+      }
+    } else {
+      for (int d = 0; d < 8; ++d) {
+        int32_t ch = C->child[8 * c + d];
+        if (ch < 0) continue;
+        double y[3] = {C->com[3 * ch] - X[0], C->com[3 * ch + 1] - X[1],
+                       C->com[3 * ch + 2] - X[2]};
+        for (int k = 0; k < 6; ++k) Q[k] += C->Q[6 * ch + k];
+        add_quad(Q, C->mass[ch], y);
+      }
+    }
+  }
+}
+
+/* Local expansion at A's centre from B's multipole, R = c_A - c_B.
+   phi = -G [M/R + Q:RR/(2 R^5)];  L = (phi, grad phi, Hessian phi). */
+static void m2l(double G, int order, const double R[3], double M,
+                const double *Q, double *L) {
+  double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+  double r = sqrt(r2), ir = 1.0 / r, ir2 = ir * ir, ir3 = ir * ir2;
+  L[0] += -G * M * ir;
+  for (int k = 0; k < 3; ++k) L[1 + k] += G * M * R[k] * ir3;
+  if (order < 2) return;
+  double ir5 = ir3 * ir2, ir7 = ir5 * ir2;
+  double QR[3] = {Q[0] * R[0] + Q[1] * R[1] + Q[2] * R[2],
+                  Q[1] * R[0] + Q[3] * R[1] + Q[4] * R[2],
+                  Q[2] * R[0] + Q[4] * R[1] + Q[5] * R[2]};
+  double QRR = QR[0] * R[0] + QR[1] * R[1] + QR[2] * R[2];
+  L[0] += -G * 0.5 * QRR * ir5;
+  for (int k = 0; k < 3; ++k)
+    L[1 + k] += -G * (QR[k] * ir5 - 2.5 * QRR * R[k] * ir7);
+  /* Hessian of -G M / R: -G M (3 R_k R_l - R^2 delta_kl) / R^5 */
+  static const int kk[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int q = 0; q < 6; ++q) {
+    int a = kk[q][0], b = kk[q][1];
+    L[4 + q] += -G * M * (3 * R[a] * R[b] - (a == b ? r2 : 0.0)) * ir5;
+  }
+}
+
+/* value and gradient of the local expansion L at offset y */
+static void local_eval(const double *L, const double y[3], double *phi,
+                       double grad[3]) {
+  double H[3][3] = {{L[4], L[5], L[6]}, {L[5], L[7], L[8]}, {L[6], L[8], L[9]}};
+  double Hy[3];
+  for (int a = 0; a < 3; ++a) Hy[a] = H[a][0] * y[0] + H[a][1] * y[1] + H[a][2] * y[2];
+  *phi = L[0] + L[1] * y[0] + L[2] * y[1] + L[3] * y[2] +
+         0.5 * (Hy[0] * y[0] + Hy[1] * y[1] + Hy[2] * y[2]);
+  for (int a = 0; a < 3; ++a) grad[a] = L[1 + a] + Hy[a];
+}
+
+static int fmm_engine(fo_sys *s) {
+  const double G = s->p.G, theta = s->p.theta;
+  const int order = s->p.fmm_order >= 2 ? 2 : 1;
+  const int L = s->p.leaf_size > 0 ? s->p.leaf_size : 1;
+  fmm_cells C = {0};
+  int64_t cap = s->nn + s->n + 1;
+  C.node = (int32_t *)malloc((size_t)cap * 4);
+  C.start = (int32_t *)malloc((size_t)cap * 4);
+  C.count = (int32_t *)malloc((size_t)cap * 4);
+  C.bucket = (int32_t *)malloc((size_t)cap * 4);
+  C.child = (int32_t *)malloc((size_t)cap * 32);
+  C.com = (double *)malloc((size_t)cap * 24);
+  C.mass = (double *)malloc((size_t)cap * 8);
+  C.rad = (double *)malloc((size_t)cap * 8);
+  C.Q = (double *)malloc((size_t)cap * 48);
+  C.L = (double *)malloc((size_t)cap * 80);
+  make_cell(s, &C, 0, L);
+  cell_multipoles(s, &C);
+  pvec q = {0}, nq = {0}, m2lv = {0}, p2p = {0};
+  pvec_push(&q, 0, 0);
+  while (q.n) {
+    nq.n = 0;
+    for (int64_t i = 0; i < q.n; ++i) {
+      int32_t a = q.v[i].a, b = q.v[i].b;
+      if (a == b) {
+        if (C.bucket[a]) {
+          pvec_push(&p2p, a, a);
+          continue;
+        }
+        int32_t ch[8];
+        int nc = 0;
+        for (int d = 0; d < 8; ++d)
+          if (C.child[8 * a + d] >= 0) ch[nc++] = C.child[8 * a + d];
+        for (int k1 = 0; k1 < nc; ++k1) {
+          pvec_push(&nq, ch[k1], ch[k1]);
+          for (int k2 = k1 + 1; k2 < nc; ++k2) {
+            pvec_push(&nq, ch[k1], ch[k2]);
+            pvec_push(&nq, ch[k2], ch[k1]);
+          }
+        }
+        continue;
+      }
+      const double *ca = C.com + 3 * a, *cb = C.com + 3 * b;
+      double sum = 0;
+      for (int d = 0; d < 3; ++d) sum += (ca[d] - cb[d]) * (ca[d] - cb[d]);
+      if (C.rad[a] + C.rad[b] < theta * sqrt(sum)) {
+        pvec_push(&m2lv, a, b);
+      } else if (C.bucket[a] && C.bucket[b]) {
+        pvec_push(&p2p, a, b);
+      } else if (!C.bucket[a] && (C.bucket[b] || C.rad[a] > C.rad[b])) {
+        for (int d = 0; d < 8; ++d)
+          if (C.child[8 * a + d] >= 0) pvec_push(&nq, C.child[8 * a + d], b);
+      } else {
+        for (int d = 0; d < 8; ++d)
+          if (C.child[8 * b + d] >= 0) pvec_push(&nq, a, C.child[8 * b + d]);
+      }
+    }
+    pvec t = q; q = nq; nq = t;
+  }
+  for (int64_t i = 0; i < m2lv.n; ++i) {
+    int32_t a = m2lv.v[i].a, b = m2lv.v[i].b;
+    double R[3] = {C.com[3 * a] - C.com[3 * b], C.com[3 * a + 1] - C.com[3 * b + 1],
+                   C.com[3 * a + 2] - C.com[3 * b + 2]};
+    m2l(G, order, R, C.mass[b], C.Q + 6 * b, C.L + 10 * a);
+  }
+  /* L2L: parents precede children */
+  for (int64_t c = 0; c < C.nc; ++c) {
+    if (C.bucket[c]) continue;
+    for (int d = 0; d < 8; ++d) {
+      int32_t ch = C.child[8 * c + d];
+      if (ch < 0) continue;
+      double y[3] = {C.com[3 * ch] - C.com[3 * c], C.com[3 * ch + 1] - C.com[3 * c + 1],
+                     C.com[3 * ch + 2] - C.com[3 * c + 2]};
+      double phi, grad[3];
+      const double *Lp = C.L + 10 * c;
+      double *Lc = C.L + 10 * ch;
+      local_eval(Lp, y, &phi, grad);
+      Lc[0] += phi;
+      for (int k = 0; k < 3; ++k) Lc[1 + k] += grad[k];
+      for (int k = 0; k < 6; ++k) Lc[4 + k] += Lp[4 + k];
+    }
+  }
+  /* L2P */
+  for (int64_t c = 0; c < C.nc; ++c) {
+    if (!C.bucket[c]) continue;
+    for (int64_t p = C.start[c]; p < C.start[c] + C.count[c]; ++p) {
+      double y[3] = {s->x[p] - C.com[3 * c], s->y[p] - C.com[3 * c + 1],
+                     s->z[p] - C.com[3 * c + 2]};
+      double phi, grad[3];
+      local_eval(C.L + 10 * c, y, &phi, grad);
+      s->phi[p] += phi;
+      s->gx[p] -= grad[0]; s->gy[p] -= grad[1]; s->gz[p] -= grad[2];
+    }
+  }
+  /* P2P (unsoftened, skipping the particle itself) */
+  for (int64_t i = 0; i < p2p.n; ++i) {
+    int32_t a = p2p.v[i].a, b = p2p.v[i].b;
+    for (int64_t pa = C.start[a]; pa < C.start[a] + C.count[a]; ++pa)
+      for (int64_t pb = C.start[b]; pb < C.start[b] + C.count[b]; ++pb) {
+        if (pa == pb) continue;
+        double d0 = s->x[pa] - s->x[pb], d1 = s->y[pa] - s->y[pb],
+               d2 = s->z[pa] - s->z[pb];
+        double r2 = d0 * d0 + d1 * d1 + d2 * d2, r = sqrt(r2);
+        s->phi[pa] += -G * s->m[pb] / r;
+        double f = -G * s->m[pb] / (r2 * r);
+        s->gx[pa] += d0 * f; s->gy[pa] += d1 * f; s->gz[pa] += d2 * f;
+      }
+  }
+  free(q.v); free(nq.v); free(m2lv.v); free(p2p.v);
+  free(C.node); free(C.start); free(C.count); free(C.bucket); free(C.child);
+  free(C.com); free(C.mass); free(C.rad); free(C.Q); free(C.L);
+  return 0;
+}
+
+/* Gravity of the current tree: the reference's own traversal when
+   leaf_size == 1 and fmm_order == 1, the bucketed engine otherwise.  Adds
+   to g and phi (compute_acceleration zeroes them first). */
+int fo_gravity(fo_sys *s) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  if (s->p.leaf_size <= 1 && s->p.fmm_order <= 1) return fmm_reference(s);
+  return fmm_engine(s);
+}
+
+/* The bucketed engine regardless of the settings (for comparing it with
+   the reference's traversal at leaf_size 1, order 1). */
+int fo_gravity_engine(fo_sys *s) {
+  if (!s->have_tree) return FSPH_E_ARG;
+  return fmm_engine(s);
 }
 
 /* Exported for the pin tests against the reference's tree_geometry.h. */

@@ -30,6 +30,7 @@ int fo_eos_init(fo_sys *s);
 int fo_density_eos(fo_sys *s);
 int fo_acceleration(fo_sys *s);
 int fo_gravity(fo_sys *s);
+int fo_gravity_engine(fo_sys *s);
 int fo_dudt(fo_sys *s);
 int fo_integrate(fo_sys *s, uint32_t what);
 int fo_update_h(fo_sys *s);

@@ -50,6 +50,7 @@ def load():
         _LIB.fo_sphere_box.argtypes = [pd, pd, pd, C.c_double]
         _LIB.fo_box_box.argtypes = [pd, pd, pd, pd]
         _LIB.fo_within_distance2.argtypes = [pd, pd, C.c_double]
+        _LIB.fo_gravity_engine.argtypes = [C.c_void_p]
         _LIB.fo_get_node_data.argtypes = [C.c_void_p] + [
             C.POINTER(C.c_double)] * 6
     return _LIB
@@ -67,6 +68,11 @@ class OracleSystem(_System):
         if not h:
             raise FsphError("fo_create", _abi.FSPH_E_OOM)
         super().__init__(_abi.bind(lib, "fo_"), C.c_void_p(h), params)
+
+    def gravity_engine(self):
+        rc = load().fo_gravity_engine(self._h)
+        if rc:
+            raise FsphError("fo_gravity_engine", rc)
 
     def node_data(self):
         t = self.tree()
