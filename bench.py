@@ -40,9 +40,22 @@ KERNEL_BYTES = {
     # x,y,z,h read (32); mask bits written (not counted)
     "masks": 32,
 }
-# Compulsory particle-state traffic of one hydro step (bytes per particle),
-# the B_hydro of the benchmark definition.
+# Compulsory particle-state traffic of one step (bytes per particle), the
+# B_hydro / B_grav of the benchmark definition.
 B_HYDRO = 864
+B_GRAV = 968
+
+
+def workload(name):
+    """(particles, params, label, step bytes) of a named workload."""
+    from flecsph_amd import generators
+    if name == "sedov64":
+        parts, prm = generators.sedov(n_side=64)
+        return parts, prm, "sedov3d_64^3_periodic", B_HYDRO
+    if name == "polytrope1m":
+        parts, prm = generators.polytrope(n=1_000_000)
+        return parts, prm, "polytrope_n1_1M_fmm_theta0.5_quad_leaf32", B_GRAV
+    raise SystemExit(f"unknown workload {name}")
 
 
 def parse():
@@ -50,20 +63,21 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n-side", type=int, default=64)
+    ap.add_argument("--workload", default="sedov64",
+                    choices=["sedov64", "polytrope1m"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=3)
     return ap.parse_args()
 
 
-def cpu_baseline(n_side, threads, steps):
+def cpu_baseline(name, threads, steps):
     """The FP64 CPU oracle (a port of the reference path, OpenMP over
-    groups) on the same workload: one initial iteration and one warm step
-    untimed, then `steps` timed steps."""
-    from flecsph_amd import _abi, generators
+    groups and FMM sinks) on the same workload: one initial iteration and
+    one warm step untimed, then `steps` timed steps."""
+    from flecsph_amd import _abi
     from oracle import oracle
-    parts, prm = generators.sedov(n_side=n_side)
+    parts, prm, label, _ = workload(name)
     prm["precision"] = _abi.PREC_FP64
     s = oracle.OracleSystem(_abi.default_params(**prm), threads=threads)
     s.upload(parts)
@@ -76,9 +90,8 @@ def cpu_baseline(n_side, threads, steps):
     n_real = s.count()[1]
     return dict(value=n_real * steps / dt, unit="particle-steps/s",
                 cores=threads, kind="port",
-                sample=f"sedov {n_side}^3 periodic, {steps} timed steps after "
-                       f"2 untimed, FP64 oracle, {threads} OpenMP threads, "
-                       f"{dt:.1f} s")
+                sample=f"{label}, {steps} timed steps after 2 untimed, FP64 "
+                       f"oracle, {threads} OpenMP threads, {dt:.1f} s")
 
 
 def main():
@@ -87,9 +100,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "--gpus must be 1"}))
         sys.exit(2)
-    from flecsph_amd import BodySystem, _abi, generators
+    from flecsph_amd import BodySystem, _abi
 
-    parts, prm = generators.sedov(n_side=args.n_side)
+    parts, prm, label, step_bytes = workload(args.workload)
     g = BodySystem(_abi.default_params(**prm), device=0)
     g.upload(parts)
     for _ in range(args.warmup):
@@ -129,7 +142,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"sedov3d_{args.n_side}^3_periodic",
+            "workload": label,
             "n_particles": n_real,
             "n_with_periodic_copies": round(avg_ntot),
             "kernel": "wendland_c6", "n_ngb": 100,
@@ -145,13 +158,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
         },
-        "step_hbm_frac": B_HYDRO * value / (HBM_PEAK_GBS * 1e9),
+        "step_hbm_frac": step_bytes * value / (HBM_PEAK_GBS * 1e9),
         "phase_ms": {k: round(v, 4) for k, v in per_phase.items()},
         "sim_state": {"iteration": it, "dt": dt, "time": tt},
     }
     if not args.no_cpu_baseline:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(args.n_side, thr, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.workload, thr, args.cpu_steps)
     print(json.dumps(out))
 
 

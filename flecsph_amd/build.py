@@ -44,9 +44,26 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, OUT)
+    build_driver()
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {DRIVER}")
     return OUT
+
+
+DRIVER = os.path.join(HERE, "fsph_hydro_driver")
+
+
+def build_driver():
+    """The C++ hydro driver over host/body_system.hpp, linked to
+    libfsph.so (host compiler only; the device code is in the library)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "hydro_driver.cpp"), "-L", HERE,
+           "-lfsph", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr[-4000:]}")
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -182,16 +182,33 @@ __device__ inline int wave_excl_scan(int v, int *total) {
   return inc - v;
 }
 
-// Reserves `cnt` slots for this lane at a global counter, one atomic per
-// wavefront.  Every lane of the wave must call it.
-__device__ inline int64_t wave_reserve(unsigned long long *ctr, int cnt) {
-  int total;
-  const int pre = wave_excl_scan(cnt, &total);
-  unsigned long long base = 0;
-  if ((threadIdx.x & 63) == 0 && total)
-    base = atomicAdd(ctr, (unsigned long long)total);
-  base = __shfl(base, 0, 64);
-  return (int64_t)base + pre;
+// Reserves c[q] slots for this thread in each of three global lists, one
+// atomic per list per workgroup.  Every thread of the block must call it.
+__device__ inline void block_reserve3(unsigned long long *ctr, const int c[3],
+                                      int64_t o[3]) {
+  constexpr int kWaves = kT / 64;
+  __shared__ int wsum[3][kWaves];
+  __shared__ unsigned long long base[3][kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int pre[3];
+  for (int q = 0; q < 3; ++q) {
+    int tot;
+    pre[q] = wave_excl_scan(c[q], &tot);
+    if (lane == 0) wsum[q][w] = tot;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    unsigned long long tot = 0;
+    for (int i = 0; i < kWaves; ++i) tot += (unsigned long long)wsum[q][i];
+    unsigned long long b = tot ? atomicAdd(&ctr[q], tot) : 0ull;
+    for (int i = 0; i < kWaves; ++i) {
+      base[q][i] = b;
+      b += (unsigned long long)wsum[q][i];
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < 3; ++q) o[q] = (int64_t)base[q][w] + pre[q];
 }
 
 struct PairArgs {
@@ -248,9 +265,10 @@ __global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
       }
     }
   }
-  const int64_t o = wave_reserve(&a.ctr[0], nout);
-  const int64_t mo = wave_reserve(&a.ctr[1], kind == 2);
-  const int64_t po = wave_reserve(&a.ctr[2], kind == 3);
+  const int cnts[3] = {nout, kind == 2, kind == 3};
+  int64_t offs[3];
+  block_reserve3(a.ctr, cnts, offs);
+  const int64_t o = offs[0], mo = offs[1], po = offs[2];
   if (kind == 0) return;
   if (nout) {
     if (o + nout > a.cap_out) {
@@ -456,62 +474,107 @@ template <class T> struct EvalArgs {
   T G;
 };
 
-// One workgroup per bucket: local expansion plus direct sums over the
-// bucket's P2P sources staged in LDS (unsoftened, skipping the particle
-// itself, fmm.h:37-46, 124-143).
+template <class T> __device__ inline T lane_sum(T v, int LP) {
+  for (int o = LP >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double fast_rsq(double x) { return 1.0 / sqrt(x); }
+
+constexpr int kEvalB = 64;            // threads per bucket workgroup
+constexpr int kStageBuckets = 16;     // at most this many source buckets
+constexpr int kStage = 256;           // and particles staged per round
+
+// One wavefront per bucket: its local expansion, then direct sums over its
+// P2P source buckets (unsoftened, skipping the particle itself, fmm.h:37-46,
+// 124-143).  Sources are staged in LDS kStageBuckets buckets at a time; a
+// bucket of c particles gets LP = 64/c lanes per particle (a power of two),
+// which split the staged sources and combine by xor shuffles.
 template <class T>
-__global__ __launch_bounds__(kMaxLeaf) void k_eval(EvalArgs<T> a) {
-  __shared__ double sx[kMaxLeaf], sy[kMaxLeaf], sz[kMaxLeaf];
-  __shared__ T sm[kMaxLeaf];
+__global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
+  __shared__ double sx[kStage], sy[kStage], sz[kStage];
+  __shared__ T sm[kStage];
+  __shared__ int32_t sidx[kStage];
+  __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
   const int64_t cell = blockIdx.x;
   if (!a.c.bucket[cell]) return;
   const int cnt = a.c.count[cell];
   const int64_t st = a.c.start[cell];
   const int t = threadIdx.x;
-  const int64_t p = st + t;
-  const bool act = t < cnt;
+  int LP = 1;
+  while (LP < 64 && cnt * LP * 2 <= kEvalB) LP <<= 1;
+  const int k = t / LP, r = t & (LP - 1);
+  const bool act = k < cnt;
+  const int64_t p = st + k;
   double px = 0, py = 0, pz = 0;
   KS<T> f0, f1, f2, pp;
   if (act) {
     px = a.x[p]; py = a.y[p]; pz = a.z[p];
-    T phi, g0, g1, g2;
-    local_eval(a.L + 10 * cell, (T)(px - a.c.com[3 * cell]),
-               (T)(py - a.c.com[3 * cell + 1]), (T)(pz - a.c.com[3 * cell + 2]),
-               &phi, &g0, &g1, &g2);
-    pp.add(phi);
-    f0.add(-g0);
-    f1.add(-g1);
-    f2.add(-g2);
-  }
-  for (int64_t j = a.off[cell]; j < a.off[cell + 1]; ++j) {
-    const int64_t s = a.src[j];
-    const int sc = a.c.count[s];
-    const int64_t ss = a.c.start[s];
-    __syncthreads();
-    if (t < sc) {
-      sx[t] = a.x[ss + t]; sy[t] = a.y[ss + t]; sz[t] = a.z[ss + t];
-      sm[t] = (T)a.m[ss + t];
-    }
-    __syncthreads();
-    if (!act) continue;
-    for (int q = 0; q < sc; ++q) {
-      if (ss + q == p) continue;
-      const T d0 = (T)(px - sx[q]), d1 = (T)(py - sy[q]), d2 = (T)(pz - sz[q]);
-      const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      const T ir = T(1) / sqrt(r2);
-      const T gm = a.G * sm[q] * ir;
-      pp.add(-gm);
-      const T fr = -gm * ir * ir;
-      f0.add(fr * d0);
-      f1.add(fr * d1);
-      f2.add(fr * d2);
+    if (r == 0) {
+      T phi, g0, g1, g2;
+      local_eval(a.L + 10 * cell, (T)(px - a.c.com[3 * cell]),
+                 (T)(py - a.c.com[3 * cell + 1]),
+                 (T)(pz - a.c.com[3 * cell + 2]), &phi, &g0, &g1, &g2);
+      pp.add(phi);
+      f0.add(-g0);
+      f1.add(-g1);
+      f2.add(-g2);
     }
   }
-  if (act) {
-    a.gx[p] = f0.get();
-    a.gy[p] = f1.get();
-    a.gz[p] = f2.get();
-    a.phi[p] = pp.get();
+  const int64_t j1 = a.off[cell + 1];
+  for (int64_t j0 = a.off[cell]; j0 < j1;) {
+    if (t == 0) {  // whole buckets while they fit (a bucket has <= kStage)
+      int acc = 0, i = 0;
+      while (i < kStageBuckets && j0 + i < j1) {
+        const int32_t s = a.src[j0 + i];
+        const int sc = a.c.count[s];
+        if (acc + sc > kStage) break;
+        bst[i] = a.c.start[s];
+        boff[i] = acc;
+        acc += sc;
+        ++i;
+      }
+      boff[i] = acc;
+      snb = i;
+    }
+    __syncthreads();
+    const int nb = snb;
+    const int nst = boff[nb];
+    j0 += nb;
+    for (int q = t; q < nst; q += kEvalB) {
+      int i = 0;
+      while (boff[i + 1] <= q) ++i;
+      const int64_t pi = bst[i] + (q - boff[i]);
+      sx[q] = a.x[pi]; sy[q] = a.y[pi]; sz[q] = a.z[pi];
+      sm[q] = (T)a.m[pi];
+      sidx[q] = (int32_t)pi;
+    }
+    __syncthreads();
+    if (act)
+      for (int q = r; q < nst; q += LP) {
+        if (sidx[q] == (int32_t)p) continue;
+        const T d0 = (T)(px - sx[q]), d1 = (T)(py - sy[q]),
+                d2 = (T)(pz - sz[q]);
+        const T ir = fast_rsq(d0 * d0 + d1 * d1 + d2 * d2);
+        const T gm = a.G * sm[q] * ir;
+        pp.add(-gm);
+        const T fr = -gm * ir * ir;
+        f0.add(fr * d0);
+        f1.add(fr * d1);
+        f2.add(fr * d2);
+      }
+    __syncthreads();
+  }
+  const T g0 = lane_sum(f0.s, LP) - lane_sum(f0.c, LP);
+  const T g1 = lane_sum(f1.s, LP) - lane_sum(f1.c, LP);
+  const T g2 = lane_sum(f2.s, LP) - lane_sum(f2.c, LP);
+  const T ph = lane_sum(pp.s, LP) - lane_sum(pp.c, LP);
+  if (act && r == 0) {
+    a.gx[p] = g0;
+    a.gy[p] = g1;
+    a.gz[p] = g2;
+    a.phi[p] = ph;
   }
 }
 
@@ -677,7 +740,7 @@ template <class T> int gravity_impl(Ctx *c) {
                  c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(),
                  c->f[PF_GX].as<T>(), c->f[PF_GY].as<T>(), c->f[PF_GZ].as<T>(),
                  c->f[PF_PHI].as<T>(), (T)c->p.G};
-  k_eval<T><<<(unsigned)nc, kMaxLeaf, 0, c->stream>>>(ea);
+  k_eval<T><<<(unsigned)nc, kEvalB, 0, c->stream>>>(ea);
   return hip_check(c, hipGetLastError(), "k_eval");
 }
 

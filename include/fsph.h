@@ -171,6 +171,14 @@ int fsph_eos_init(fsph_ctx *ctx);
    (default_physics.h:349-408): hydro acceleration; zeroes g and phi. */
 int fsph_acceleration(fsph_ctx *ctx);
 
+/* body_system::gravitation_fmm (bodies_system.h:282-288 ->
+   tree_topology.h:449-795 with the callbacks of fmm.h:37-159): FMM
+   self-gravity of the current tree into g and phi, with opening angle
+   theta, bucket size leaf_size and multipole order fmm_order.  Call it
+   after fsph_acceleration, which zeroes g and phi as compute_acceleration
+   does (default_physics.h:406-407). */
+int fsph_gravity(fsph_ctx *ctx);
+
 /* apply_in_smoothinglength(physics::compute_dudt) (default_physics.h:
    422-487) followed by apply_all(recompute_pressure_soundspeed)
    (default_physics.h:154-161) with the context's current dt. */

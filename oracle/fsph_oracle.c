@@ -1468,6 +1468,20 @@ static void local_eval(const double *L, const double y[3], double *phi,
   for (int a = 0; a < 3; ++a) grad[a] = L[1 + a] + Hy[a];
 }
 
+static int cmp_pair_sink(const void *pa, const void *pb) {
+  const pair32 *a = (const pair32 *)pa, *b = (const pair32 *)pb;
+  if (a->a != b->a) return a->a < b->a ? -1 : 1;
+  return (a->b > b->b) - (a->b < b->b);
+}
+
+/* offsets of each sink's run in a list sorted by sink */
+static int64_t *sink_offsets(const pvec *v, int64_t nc) {
+  int64_t *off = (int64_t *)calloc((size_t)nc + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < v->n; ++i) off[v->v[i].a + 1]++;
+  for (int64_t c = 0; c < nc; ++c) off[c + 1] += off[c];
+  return off;
+}
+
 static int fmm_engine(fo_sys *s) {
   const double G = s->p.G, theta = s->p.theta;
   const int order = s->p.fmm_order >= 2 ? 2 : 1;
@@ -1527,12 +1541,19 @@ static int fmm_engine(fo_sys *s) {
     }
     pvec t = q; q = nq; nq = t;
   }
-  for (int64_t i = 0; i < m2lv.n; ++i) {
-    int32_t a = m2lv.v[i].a, b = m2lv.v[i].b;
-    double R[3] = {C.com[3 * a] - C.com[3 * b], C.com[3 * a + 1] - C.com[3 * b + 1],
-                   C.com[3 * a + 2] - C.com[3 * b + 2]};
-    m2l(G, order, R, C.mass[b], C.Q + 6 * b, C.L + 10 * a);
-  }
+  /* group both lists by sink so sinks can be processed in parallel */
+  qsort(m2lv.v, (size_t)m2lv.n, sizeof(pair32), cmp_pair_sink);
+  qsort(p2p.v, (size_t)p2p.n, sizeof(pair32), cmp_pair_sink);
+  int64_t *moff = sink_offsets(&m2lv, C.nc), *poff = sink_offsets(&p2p, C.nc);
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 64)
+  for (int64_t a = 0; a < C.nc; ++a)
+    for (int64_t i = moff[a]; i < moff[a + 1]; ++i) {
+      int32_t b = m2lv.v[i].b;
+      double R[3] = {C.com[3 * a] - C.com[3 * b],
+                     C.com[3 * a + 1] - C.com[3 * b + 1],
+                     C.com[3 * a + 2] - C.com[3 * b + 2]};
+      m2l(G, order, R, C.mass[b], C.Q + 6 * b, C.L + 10 * a);
+    }
   /* L2L: parents precede children */
   for (int64_t c = 0; c < C.nc; ++c) {
     if (C.bucket[c]) continue;
@@ -1563,8 +1584,10 @@ static int fmm_engine(fo_sys *s) {
     }
   }
   /* P2P (unsoftened, skipping the particle itself) */
-  for (int64_t i = 0; i < p2p.n; ++i) {
-    int32_t a = p2p.v[i].a, b = p2p.v[i].b;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 16)
+  for (int64_t a = 0; a < C.nc; ++a)
+  for (int64_t i = poff[a]; i < poff[a + 1]; ++i) {
+    int32_t b = p2p.v[i].b;
     for (int64_t pa = C.start[a]; pa < C.start[a] + C.count[a]; ++pa)
       for (int64_t pb = C.start[b]; pb < C.start[b] + C.count[b]; ++pb) {
         if (pa == pb) continue;
@@ -1576,6 +1599,7 @@ static int fmm_engine(fo_sys *s) {
         s->gx[pa] += d0 * f; s->gy[pa] += d1 * f; s->gz[pa] += d2 * f;
       }
   }
+  free(moff); free(poff);
   free(q.v); free(nq.v); free(m2lv.v); free(p2p.v);
   free(C.node); free(C.start); free(C.count); free(C.bucket); free(C.child);
   free(C.com); free(C.mass); free(C.rad); free(C.Q); free(C.L);
