@@ -32,6 +32,10 @@ PREC_FP64 = 1
 GRAV_NONE = 0
 GRAV_FMM = 1
 
+H_CLOSED = 0
+H_NEWTON = 1
+H_LEARNED = 2
+
 KICK_V = 1
 KICK_U = 2
 SAVE_VHALF = 4
@@ -39,7 +43,7 @@ DRIFT = 8
 
 FIELDS = ["x", "y", "z", "vx", "vy", "vz", "vhx", "vhy", "vhz", "u", "h", "m",
           "rho", "P", "c", "vsig", "ax", "ay", "az", "dudt", "gx", "gy", "gz",
-          "phi", "id", "key", "nngb"]
+          "phi", "id", "key", "nngb", "hngb", "hiter"]
 FIELD_INDEX = {name: i for i, name in enumerate(FIELDS)}
 
 TIMERS = ["integrate", "boundary", "range_keys", "sort", "tree", "traverse",
@@ -58,6 +62,8 @@ class FsphParams(C.Structure):
         ("visc_eps", C.c_double), ("cfl", C.c_double),
         ("initial_dt", C.c_double), ("G", C.c_double), ("theta", C.c_double),
         ("box", C.c_double * 3),
+        ("h_mode", C.c_int32), ("newton_maxit", C.c_int32),
+        ("newton_tol", C.c_double), ("h_search", C.c_double),
     ]
 
 
@@ -92,6 +98,10 @@ def default_params(**kw):
     p.initial_dt = 0.001
     p.G = 1.0
     p.theta = 0.0
+    p.h_mode = H_CLOSED
+    p.newton_maxit = 20
+    p.newton_tol = 1e-4
+    p.h_search = 1.3
     for k in range(3):
         p.periodic[k] = 0
         p.box[k] = 1.0

@@ -48,7 +48,7 @@ int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
 
 size_t field_size(const Ctx *c, int f) {
   if (f < PF_NDBL || f == PF_ID) return 8;
-  if (f == PF_NNGB) return 4;
+  if (f == PF_NNGB || f == PF_HNGB || f == PF_HITER) return 4;
   return c->rsize;
 }
 
@@ -133,6 +133,17 @@ int do_advance(Ctx *c) {
 
 int do_gravity(Ctx *c);
 
+// The end-of-step smoothing-length update: closed form (the reference's
+// get_all(compute_smoothinglength)) or Newton-h over fresh gather tiles.
+int do_update_h(Ctx *c) {
+  if (c->p.h_mode == FSPH_H_NEWTON) {
+    FSPH_TRY(need_tree(c));
+    return stage_newton_h(c, nullptr);
+  }
+  PhaseTimer t(c, FSPH_T_H);
+  return stage_update_h(c);
+}
+
 int do_step(Ctx *c) {
   c->ev_used = 0;
   if (!c->initial_done) {
@@ -178,10 +189,7 @@ int do_step(Ctx *c) {
       FSPH_TRY(stage_integrate(c, FSPH_KICK_U));
     }
   }
-  {
-    PhaseTimer t(c, FSPH_T_H);
-    FSPH_TRY(stage_update_h(c));  // hydro/main_driver.cc:247-256
-  }
+  FSPH_TRY(do_update_h(c));  // hydro/main_driver.cc:247-256
   {
     PhaseTimer t(c, FSPH_T_DT);
     FSPH_TRY(do_timestep(c));  // hydro/main_driver.cc:267-273
@@ -220,6 +228,10 @@ void fsph_default_params(fsph_params *p) {
   p->G = 1.0;
   p->theta = 0.0;
   for (int d = 0; d < 3; ++d) p->box[d] = 1.0;
+  p->h_mode = FSPH_H_CLOSED;
+  p->newton_maxit = 20;
+  p->newton_tol = 1e-4;
+  p->h_search = 1.3;
 }
 
 int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
@@ -326,7 +338,8 @@ int fsph_upload(fsph_ctx *ctx, const fsph_particles *in) {
   }
   const int zero_f[] = {fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY, fsph::PF_AZ,
                         fsph::PF_DUDT, fsph::PF_GX, fsph::PF_GY, fsph::PF_GZ,
-                        fsph::PF_PHI, fsph::PF_COLD, fsph::PF_NNGB};
+                        fsph::PF_PHI, fsph::PF_COLD, fsph::PF_NNGB,
+                        fsph::PF_HNGB, fsph::PF_HITER};
   for (int f : zero_f)
     FSPH_HIP(c, hipMemsetAsync(c->f[f].p, 0, (size_t)n * fsph::field_size(c, f),
                                c->stream));
@@ -361,14 +374,15 @@ int fsph_download(fsph_ctx *ctx, int field, void *dst) {
       fsph::PF_H,   fsph::PF_M,   fsph::PF_RHO,  fsph::PF_P,    fsph::PF_C,
       fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY,   fsph::PF_AZ,   fsph::PF_DUDT,
       fsph::PF_GX,  fsph::PF_GY,  fsph::PF_GZ,   fsph::PF_PHI,  fsph::PF_ID,
-      -1 /* key */, fsph::PF_NNGB};
+      -1 /* key */, fsph::PF_NNGB, fsph::PF_HNGB, fsph::PF_HITER};
   if (field == FSPH_F_KEY) {
     if (!c->key.p) return fsph::fail(c, FSPH_E_ARG, "no keys yet");
     FSPH_HIP(c, hipMemcpyAsync(dst, c->key.p, (size_t)n * 8,
                                hipMemcpyDeviceToHost, c->stream));
-  } else if (field == FSPH_F_NNGB) {
+  } else if (field == FSPH_F_NNGB || field == FSPH_F_HNGB ||
+             field == FSPH_F_HITER) {
     std::vector<int32_t> t(n);
-    FSPH_HIP(c, hipMemcpyAsync(t.data(), c->f[fsph::PF_NNGB].p, (size_t)n * 4,
+    FSPH_HIP(c, hipMemcpyAsync(t.data(), c->f[map[field]].p, (size_t)n * 4,
                                hipMemcpyDeviceToHost, c->stream));
     FSPH_HIP(c, hipStreamSynchronize(c->stream));
     double *d = static_cast<double *>(dst);
@@ -446,7 +460,7 @@ int fsph_integrate(fsph_ctx *ctx, uint32_t what) {
 
 int fsph_update_h(fsph_ctx *ctx) {
   CTX_OR_FAIL;
-  FSPH_TRY(fsph::stage_update_h(c));
+  FSPH_TRY(fsph::do_update_h(c));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
   return FSPH_OK;
 }

@@ -51,6 +51,7 @@ enum PField : int {
   PF_RHO, PF_P, PF_C, PF_COLD, PF_VSIG, PF_AX, PF_AY, PF_AZ, PF_DUDT, PF_GX,
   PF_GY, PF_GZ, PF_PHI, // real-typed fields
   PF_NNGB,              // int32
+  PF_HNGB, PF_HITER,    // int32: smoothing-length solver outputs
   PF_COUNT
 };
 
@@ -94,6 +95,11 @@ struct Ctx {
   DBuf mask;                 // uint32 words
   int64_t mask_words = 0;
   bool have_tree = false;
+  // gather tiles of the smoothing-length solver (radius h_search * h)
+  int64_t gccap = 4096;
+  DBuf gcand, gccount, gmoff, gmask;
+  int64_t gmask_words = 0;
+  DBuf h_prop;               // learned proposal of h (float), or empty
 
   // FMM gravity: cells (internal nodes and buckets), pair lists
   struct Grav {
@@ -170,6 +176,9 @@ int permute_particles(Ctx *c, const uint32_t *perm, int64_t n_out);
 int stage_tree(Ctx *c);                // k_tree.hip
 int stage_groups(Ctx *c);              // k_tree.hip
 int stage_neighbors(Ctx *c);           // k_neighbors.hip
+// gather tiles: all b with r_ab < scale * h_a per group particle a
+int stage_gather_tiles(Ctx *c, double scale);  // k_neighbors.hip
+int stage_newton_h(Ctx *c, const float *h0);   // k_hsolve.hip
 int stage_density(Ctx *c);             // k_sph.hip
 int stage_acceleration(Ctx *c);        // k_sph.hip
 int stage_dudt(Ctx *c);                // k_sph.hip

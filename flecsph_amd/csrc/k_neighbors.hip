@@ -75,9 +75,14 @@ struct TravArgs {
   int32_t *ccount;
   int64_t ccap;
   int *flags;
+  // gather mode (smoothing-length solver): sphere radius rscale * h_k and
+  // no group-box test; the SPH mode uses rscale = 1 with the box test.
+  double rscale;
+  int gather;
 };
 
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
+#pragma clang fp contract(off)
   __shared__ double sx[kMaxGroup], sy[kMaxGroup], sz[kMaxGroup],
       sh[kMaxGroup];
   __shared__ int32_t stk[kStack];
@@ -85,12 +90,12 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
   const int64_t g = blockIdx.x;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
-  const int gn = a.gnode[g];
+  const int gn = a.gather ? -1 : a.gnode[g];
   for (int k = lane; k < gc; k += kW) {
     sx[k] = a.x[gs + k];
     sy[k] = a.y[gs + k];
     sz[k] = a.z[gs + k];
-    sh[k] = a.h[gs + k];
+    sh[k] = a.gather ? a.rscale * a.h[gs + k] : a.h[gs + k];
   }
   double gb[6];
   if (gn >= 0)
@@ -172,17 +177,21 @@ struct MaskArgs {
   const int64_t *moff;
   int64_t ccap;
   uint32_t *mask;
-  int32_t *nngb;
+  int32_t *nngb;  // may be null
+  // gather mode: bit set iff r_kb < rscale * h_k (strict, own radius only)
+  double rscale;
+  int gather;
 };
 
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
+#pragma clang fp contract(off)
   __shared__ double cx[32], cy[32], cz[32], chh[32];
   __shared__ int cok[32];
   const int lane = threadIdx.x;
   const int64_t g = blockIdx.x;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
-  const int gn = a.gnode[g];
+  const int gn = a.gather ? -1 : a.gnode[g];
   const int cc = a.ccount[g];
   const int64_t mo = a.moff[g];
   const int64_t cbase = g * a.ccap;
@@ -193,7 +202,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       px[s] = a.x[gs + k];
       py[s] = a.y[gs + k];
       pz[s] = a.z[gs + k];
-      ph[s] = a.h[gs + k];
+      ph[s] = a.gather ? a.rscale * a.h[gs + k] : a.h[gs + k];
     }
   }
   double gcom[3] = {0, 0, 0}, gext_lap = 0, grad = 0;
@@ -233,40 +242,53 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       uint32_t word = 0;
       for (int j = 0; j < 32; ++j) {
         if (!cok[j]) continue;
-        // tree_topology.h:372-379: d^2 <= max(h_k, h_b)^2
-        const double r = (ph[s] < chh[j]) ? chh[j] : ph[s];
-        if (within_distance2(px[s], py[s], pz[s], cx[j], cy[j], cz[j], r))
-          word |= 1u << j;
+        bool in;
+        if (a.gather) {
+          const double d0 = px[s] - cx[j], d1 = py[s] - cy[j],
+                       d2 = pz[s] - cz[j];
+          in = d0 * d0 + d1 * d1 + d2 * d2 < ph[s] * ph[s];
+        } else {
+          // tree_topology.h:372-379: d^2 <= max(h_k, h_b)^2
+          const double r = (ph[s] < chh[j]) ? chh[j] : ph[s];
+          in = within_distance2(px[s], py[s], pz[s], cx[j], cy[j], cz[j], r);
+        }
+        if (in) word |= 1u << j;
       }
       a.mask[mo + (int64_t)w * gc + k] = word;
       cnt[s] += __popc(word);
     }
     __syncthreads();
   }
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k < gc) a.nngb[gs + k] = cnt[s];
-  }
+  if (a.nngb)
+    for (int s = 0; s < 2; ++s) {
+      const int k = lane + kW * s;
+      if (k < gc) a.nngb[gs + k] = cnt[s];
+    }
 }
 
-}  // namespace
+struct TileBufs {
+  DBuf &cand, &ccount, &moff, &mask;
+  int64_t &ccap, &words;
+};
 
-int stage_neighbors(Ctx *c) {
+int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
+                int32_t *nngb) {
   const int64_t ng = c->ng;
-  FSPH_TRY(ensure(c, c->ccount, (size_t)ng * 4));
-  FSPH_TRY(ensure(c, c->moff, (size_t)(ng + 1) * 8));
+  FSPH_TRY(ensure(c, tb.ccount, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, tb.moff, (size_t)(ng + 1) * 8));
   for (int attempt = 0;; ++attempt) {
-    FSPH_TRY(ensure(c, c->cand, (size_t)ng * c->ccap * 4));
+    FSPH_TRY(ensure(c, tb.cand, (size_t)ng * tb.ccap * 4));
     FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
     TravArgs t{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
                c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
                c->nbox.as<double>(),    c->nchild.as<int32_t>(),
                c->gstart.as<int32_t>(), c->gcount.as<int32_t>(),
-               c->gnode.as<int32_t>(),  c->cand.as<int32_t>(),
-               c->ccount.as<int32_t>(), c->ccap,
-               c->flags.as<int>()};
+               c->gnode.as<int32_t>(),  tb.cand.as<int32_t>(),
+               tb.ccount.as<int32_t>(), tb.ccap,
+               c->flags.as<int>(),      scale,
+               gather ? 1 : 0};
     {
-      PhaseTimer tm(c, FSPH_T_TRAVERSE);
+      PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
       k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
     }
     FSPH_HIP(c, hipGetLastError());
@@ -277,36 +299,50 @@ int stage_neighbors(Ctx *c) {
     if (fl & kFlagCand) {
       if (attempt >= 6)
         return fail(c, FSPH_E_CAP, "candidate tile overflow (cap %lld)",
-                    (long long)c->ccap);
-      c->ccap *= 2;
+                    (long long)tb.ccap);
+      tb.ccap *= 2;
       continue;
     }
     break;
   }
   k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
-      c->ccount.as<int32_t>(), c->gcount.as<int32_t>(), ng,
-      c->moff.as<int64_t>());
+      tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), ng,
+      tb.moff.as<int64_t>());
   FSPH_HIP(c, hipGetLastError());
-  FSPH_TRY(scan_i64(c, c->moff.as<int64_t>(), c->moff.as<int64_t>(), ng));
+  FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), ng));
   int64_t words = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&words, c->moff.as<int64_t>() + ng, 8,
+  FSPH_HIP(c, hipMemcpyAsync(&words, tb.moff.as<int64_t>() + ng, 8,
                              hipMemcpyDeviceToHost, c->stream));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  c->mask_words = words;
-  FSPH_TRY(ensure(c, c->mask, (size_t)(words > 0 ? words : 1) * 4));
+  tb.words = words;
+  FSPH_TRY(ensure(c, tb.mask, (size_t)(words > 0 ? words : 1) * 4));
   MaskArgs m{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
              c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
              c->ncom.as<double>(),    c->nrad.as<double>(),
              c->nlap.as<double>(),    c->gstart.as<int32_t>(),
              c->gcount.as<int32_t>(), c->gnode.as<int32_t>(),
-             c->cand.as<int32_t>(),   c->ccount.as<int32_t>(),
-             c->moff.as<int64_t>(),   c->ccap,
-             c->mask.as<uint32_t>(),  c->f[PF_NNGB].as<int32_t>()};
+             tb.cand.as<int32_t>(),   tb.ccount.as<int32_t>(),
+             tb.moff.as<int64_t>(),   tb.ccap,
+             tb.mask.as<uint32_t>(),  nngb,
+             scale,                   gather ? 1 : 0};
   {
-    PhaseTimer tm(c, FSPH_T_MASKS);
+    PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
     k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
   }
   return hip_check(c, hipGetLastError(), "k_masks");
+}
+
+}  // namespace
+
+int stage_neighbors(Ctx *c) {
+  TileBufs tb{c->cand, c->ccount, c->moff, c->mask, c->ccap, c->mask_words};
+  return build_tiles(c, tb, false, 1.0, c->f[PF_NNGB].as<int32_t>());
+}
+
+int stage_gather_tiles(Ctx *c, double scale) {
+  TileBufs tb{c->gcand, c->gccount, c->gmoff, c->gmask, c->gccap,
+              c->gmask_words};
+  return build_tiles(c, tb, true, scale, nullptr);
 }
 
 }  // namespace fsph

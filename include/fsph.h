@@ -88,7 +88,18 @@ typedef struct fsph_params {
   double theta;        /* fmm_macangle                                      */
   double box[3];       /* box_length, box_width, box_height (periodic box
                           centred on the origin, boundary.h:134-144)        */
+  /* smoothing-length update at the end of a step (fsph_update_h) */
+  int32_t h_mode;      /* FSPH_H_CLOSED | FSPH_H_NEWTON | FSPH_H_LEARNED    */
+  int32_t newton_maxit;/* Newton-h iteration cap (20)                       */
+  double newton_tol;   /* stop when |dh|/h < newton_tol (1e-4)              */
+  double h_search;     /* gather radius of the Newton solve, in units of
+                          the current h (1.3); the solution is clamped to
+                          it and counted as unconverged                     */
 } fsph_params;
+
+#define FSPH_H_CLOSED 0  /* h = eta kw (m/rho)^(1/3), default_physics.h:826 */
+#define FSPH_H_NEWTON 1  /* Newton on rho~(h) h^3 = m (eta kw)^3            */
+#define FSPH_H_LEARNED 2 /* MLP proposal, Newton fallback                   */
 
 /* Fills p with the reference's defaults (params.h DECLARE_PARAM values,
    kernel = cubic spline, eos = ideal, gamma 1.4, group_max 128). */
@@ -120,6 +131,9 @@ enum fsph_field {
   FSPH_F_DUDT,
   FSPH_F_GX, FSPH_F_GY, FSPH_F_GZ, FSPH_F_PHI,
   FSPH_F_ID, FSPH_F_KEY, FSPH_F_NNGB,
+  FSPH_F_HNGB,   /* #{b : r_ab < h_a} after the last Newton/learned update */
+  FSPH_F_HITER,  /* Newton iterations of the last update (0: accepted
+                    learned proposal)                                      */
   FSPH_NFIELDS
 };
 

@@ -47,6 +47,7 @@ struct fo_sys {
   int64_t *id;
   uint64_t *key;
   int32_t *nngb;
+  int32_t *hngb, *hiter; /* smoothing-length solver outputs */
   /* tree */
   int64_t nn, ncap;
   uint64_t *nkey;
@@ -102,7 +103,10 @@ static int reserve(fo_sys *s, int64_t cap) {
   s->id = (int64_t *)realloc(s->id, (size_t)cap * sizeof(int64_t));
   s->key = (uint64_t *)realloc(s->key, (size_t)cap * sizeof(uint64_t));
   s->nngb = (int32_t *)realloc(s->nngb, (size_t)cap * sizeof(int32_t));
-  if (!s->id || !s->key || !s->nngb) return FSPH_E_OOM;
+  s->hngb = (int32_t *)realloc(s->hngb, (size_t)cap * sizeof(int32_t));
+  s->hiter = (int32_t *)realloc(s->hiter, (size_t)cap * sizeof(int32_t));
+  if (!s->id || !s->key || !s->nngb || !s->hngb || !s->hiter)
+    return FSPH_E_OOM;
   s->cap = cap;
   return 0;
 }
@@ -135,7 +139,7 @@ void fo_destroy(fo_sys *s) {
   double **f[NDBL];
   dbl_fields(s, f);
   for (int i = 0; i < NDBL; ++i) free(*f[i]);
-  free(s->id); free(s->key); free(s->nngb);
+  free(s->id); free(s->key); free(s->nngb); free(s->hngb); free(s->hiter);
   free_tree(s);
   free(s);
 }
@@ -158,7 +162,7 @@ int fo_upload(fo_sys *s, const fsph_particles *in) {
     s->c[i] = in->c ? in->c[i] : 0.0;
     s->vsig[i] = 0; s->ax[i] = s->ay[i] = s->az[i] = 0; s->dudt[i] = 0;
     s->gx[i] = s->gy[i] = s->gz[i] = s->phi[i] = 0;
-    s->key[i] = 0; s->nngb[i] = 0;
+    s->key[i] = 0; s->nngb[i] = 0; s->hngb[i] = 0; s->hiter[i] = 0;
   }
   free_tree(s);
   s->iteration = 0;
@@ -205,9 +209,14 @@ int fo_download(fo_sys *s, int field, void *dst) {
     case FSPH_F_PHI: src = s->phi; break;
     case FSPH_F_ID: memcpy(dst, s->id, (size_t)s->n * 8); return 0;
     case FSPH_F_KEY: memcpy(dst, s->key, (size_t)s->n * 8); return 0;
-    case FSPH_F_NNGB: {
+    case FSPH_F_NNGB:
+    case FSPH_F_HNGB:
+    case FSPH_F_HITER: {
+      const int32_t *src = field == FSPH_F_NNGB   ? s->nngb
+                           : field == FSPH_F_HNGB ? s->hngb
+                                                  : s->hiter;
       double *d = (double *)dst;
-      for (int64_t i = 0; i < s->n; ++i) d[i] = s->nngb[i];
+      for (int64_t i = 0; i < s->n; ++i) d[i] = src[i];
       return 0;
     }
     default: return FSPH_E_ARG;
@@ -1012,10 +1021,124 @@ int fo_integrate(fo_sys *s, uint32_t what) {
 }
 
 /* A15: compute_smoothinglength, default_physics.h:826-848 (3D) */
-int fo_update_h(fo_sys *s) {
+static void closed_form_h(fo_sys *s) {
   double kw = kernel_width(&s->p);
   for (int64_t i = 0; i < s->n; ++i)
     s->h[i] = cbrt(s->m[i] / s->rho[i]) * s->p.eta * kw;
+}
+
+/* A19 (new; the reference updates h in closed form only): Newton-h.
+   Gather set S_a = {b : r_ab < H_a}, H_a = h_search * h_a, collected by a
+   depth-first walk of the tree (node boxes hold their particles, so the
+   sphere-box test keeps every b within H_a).  The equation is the
+   kernel-weighted neighbour count without the particle itself,
+     (4 pi / 3) h^3 rho~(h) / m_a = N_target - 1,
+     rho~(h) = sum_{b in S_a, b != a} m_b W(r_ab, h),
+   whose hard count #{b : r_ab < h} (self included) averages N_target for
+   smooth data; with the self term, as in rho h^3 = m (eta kw)^3 of
+   default_physics.h:826-848, the count would sit (4 pi/3) W(0) h^3 below
+   the target (about 11 for the cubic spline, 28 for Wendland C6).  In FP64:
+     f(h) = rho~ h^3 - m_a (3 / 4 pi) (N_target - 1),
+     f'(h) = 3 h^2 rho~ + h^3 sum m_b dW/dh,  dW/dh = -(3 W + r dW/dr) / h,
+   Newton steps inside the bracket [lo, hi] (bisection when a step leaves
+   it), from h_a, until |dh| < tol h or maxit steps; hi starts at H_a and a
+   root beyond it is clamped there.  Records #{b : r_ab < h} and the
+   iteration count. */
+static int gather_set(const fo_sys *s, int64_t a, double H, ivec *out,
+                      ivec *stk) {
+  out->n = 0;
+  stk->n = 0;
+  ivec_push(stk, 0);
+  while (stk->n) {
+    int32_t c = stk->v[--stk->n];
+    if (c >= 0) {
+      if (!sphere_box(s->nbmin + 3 * c, s->nbmax + 3 * c, s->x[a], s->y[a],
+                      s->z[a], H))
+        continue;
+      for (int d = 0; d < 8; ++d)
+        if (s->nchild[c * 8 + d] != CHILD_EMPTY)
+          ivec_push(stk, s->nchild[c * 8 + d]);
+    } else {
+      int64_t b = LEAF_OF(c);
+      double d0 = s->x[a] - s->x[b], d1 = s->y[a] - s->y[b],
+             d2 = s->z[a] - s->z[b];
+      if (d0 * d0 + d1 * d1 + d2 * d2 < H * H) ivec_push(out, (int32_t)b);
+    }
+  }
+  return 0;
+}
+
+static void newton_eval(const fo_sys *s, int64_t a, const ivec *S, double h,
+                        double *rho, double *drho, int *nngb) {
+  double r0 = 0, r1 = 0;
+  int cnt = 0;
+  for (int64_t j = 0; j < S->n; ++j) {
+    int64_t b = S->v[j];
+    double v[3] = {s->x[a] - s->x[b], s->y[a] - s->y[b], s->z[a] - s->z[b]};
+    double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (!(r2 < h * h)) continue;
+    ++cnt;
+    if (b == a) continue;
+    double r = sqrt(r2);
+    double W = fo_kernel(s->p.kernel, r, h), g[3];
+    fo_kernel_gradient(s->p.kernel, v, h, g);
+    double rdWdr = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
+    r0 += s->m[b] * W;
+    r1 += s->m[b] * (-(3.0 * W + rdWdr) / h);
+  }
+  *rho = r0;
+  *drho = r1;
+  *nngb = cnt;
+}
+
+static int newton_h(fo_sys *s) {
+  const double c3 = 3.0 / (4.0 * M_PI) * (s->p.n_target - 1);
+  const int maxit = s->p.newton_maxit > 0 ? s->p.newton_maxit : 20;
+  const double tol = s->p.newton_tol > 0 ? s->p.newton_tol : 1e-4;
+  const double kappa = s->p.h_search > 1 ? s->p.h_search : 1.3;
+  int64_t n = s->n;
+  double *hn = (double *)malloc((size_t)n * 8);
+#pragma omp parallel num_threads(g_threads)
+  {
+    ivec S = {0}, stk = {0};
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t a = 0; a < n; ++a) {
+      const double H = kappa * s->h[a];
+      gather_set(s, a, H, &S, &stk);
+      const double target = s->m[a] * c3;
+      double lo = 0, hi = H, h = s->h[a], rho, drho;
+      int it = 0, nb = 0;
+      for (;;) {
+        newton_eval(s, a, &S, h, &rho, &drho, &nb);
+        double f = rho * h * h * h - target;
+        double fp = 3 * h * h * rho + h * h * h * drho;
+        if (f < 0) lo = h; else hi = h;
+        double hn1 = (fp > 0) ? h - f / fp : -1;
+        if (!(hn1 > lo && hn1 < hi)) hn1 = 0.5 * (lo + hi);
+        ++it;
+        double dh = fabs(hn1 - h);
+        h = hn1;
+        if (dh < tol * h || it >= maxit) break;
+      }
+      newton_eval(s, a, &S, h, &rho, &drho, &nb);
+      hn[a] = h;
+      s->hngb[a] = nb;
+      s->hiter[a] = it;
+    }
+    free(S.v);
+    free(stk.v);
+  }
+  memcpy(s->h, hn, (size_t)n * 8);
+  free(hn);
+  return 0;
+}
+
+int fo_update_h(fo_sys *s) {
+  if (s->p.h_mode == FSPH_H_NEWTON) {
+    if (!s->have_tree) return FSPH_E_ARG;
+    return newton_h(s);
+  }
+  closed_form_h(s);
   return 0;
 }
 
