@@ -64,6 +64,7 @@ class FsphParams(C.Structure):
         ("box", C.c_double * 3),
         ("h_mode", C.c_int32), ("newton_maxit", C.c_int32),
         ("newton_tol", C.c_double), ("h_search", C.c_double),
+        ("learn_steps", C.c_int32), ("n_tol", C.c_int32),
     ]
 
 
@@ -102,6 +103,8 @@ def default_params(**kw):
     p.newton_maxit = 20
     p.newton_tol = 1e-4
     p.h_search = 1.3
+    p.learn_steps = 50
+    p.n_tol = 5
     for k in range(3):
         p.periodic[k] = 0
         p.box[k] = 1.0
@@ -119,6 +122,7 @@ _PD = C.POINTER(C.c_double)
 _PI64 = C.POINTER(C.c_int64)
 _PU64 = C.POINTER(C.c_uint64)
 _PI32 = C.POINTER(C.c_int32)
+_PF = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); "ctx" functions take the handle first.
 PROTOTYPES = {
@@ -144,6 +148,13 @@ PROTOTYPES = {
     "get_tree": (C.c_int, [_P, _PI64, _PU64, _PI32, _PI32, _PI32, _PI64,
                            _PI32, _PI32]),
     "neighbor_lists": (C.c_int, [_P, _PI64, _PI64, C.c_int64]),
+    "mlp_get_params": (C.c_int, [_P, _PF, C.c_int64]),
+    "mlp_set_params": (C.c_int, [_P, _PF, C.c_int64]),
+    "mlp_forward": (C.c_int, [_P, _PF, C.c_int64, _PF]),
+    "mlp_gradient": (C.c_int, [_P, _PF, _PF, C.c_int64, _PF, _PF]),
+    "mlp_train": (C.c_int, [_P, _PF, _PF, C.c_int64, C.c_int, C.c_float,
+                            _PF]),
+    "h_stats": (C.c_int, [_P, _PI64, _PI64, _PI32, _PF]),
 }
 
 

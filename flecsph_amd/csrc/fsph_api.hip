@@ -140,6 +140,10 @@ int do_update_h(Ctx *c) {
     FSPH_TRY(need_tree(c));
     return stage_newton_h(c, nullptr);
   }
+  if (c->p.h_mode == FSPH_H_LEARNED) {
+    FSPH_TRY(need_tree(c));
+    return stage_learned_h(c);
+  }
   PhaseTimer t(c, FSPH_T_H);
   return stage_update_h(c);
 }
@@ -232,6 +236,8 @@ void fsph_default_params(fsph_params *p) {
   p->newton_maxit = 20;
   p->newton_tol = 1e-4;
   p->h_search = 1.3;
+  p->learn_steps = 50;
+  p->n_tol = 5;
 }
 
 int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
@@ -274,10 +280,14 @@ void fsph_destroy(fsph_ctx *ctx) {
                        &c->nchild, &c->nbox, &c->ncom, &c->nmass, &c->nrad,
                        &c->nlap, &c->nlevel, &c->leaf_parent, &c->gstart,
                        &c->gcount, &c->gnode, &c->gflag, &c->gscan, &c->cand,
-                       &c->ccount, &c->moff, &c->mask};
+                       &c->ccount, &c->moff, &c->mask, &c->gcand,
+                       &c->gccount, &c->gmoff, &c->gmask, &c->h_prop,
+                       &c->h_prev, &c->hflag, &c->glist, &c->hgflag,
+                       &c->hgscan, &c->hcnt};
   for (auto *b : all)
     if (b->p) hipFree(b->p);
   fsph::free_gravity(c);
+  fsph::free_mlp(c);
   for (int f = 0; f < fsph::PF_COUNT; ++f) {
     if (c->f[f].p) hipFree(c->f[f].p);
     if (c->g[f].p) hipFree(c->g[f].p);
@@ -339,7 +349,8 @@ int fsph_upload(fsph_ctx *ctx, const fsph_particles *in) {
   const int zero_f[] = {fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY, fsph::PF_AZ,
                         fsph::PF_DUDT, fsph::PF_GX, fsph::PF_GY, fsph::PF_GZ,
                         fsph::PF_PHI, fsph::PF_COLD, fsph::PF_NNGB,
-                        fsph::PF_HNGB, fsph::PF_HITER};
+                        fsph::PF_HNGB, fsph::PF_HITER, fsph::PF_DIVV,
+                        fsph::PF_DLH};
   for (int f : zero_f)
     FSPH_HIP(c, hipMemsetAsync(c->f[f].p, 0, (size_t)n * fsph::field_size(c, f),
                                c->stream));
@@ -622,6 +633,130 @@ int fsph_enable_timing(fsph_ctx *ctx, int on) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c) return FSPH_E_ARG;
   c->timing = on != 0;
+  return FSPH_OK;
+}
+
+int fsph_mlp_num_params(void) { return fsph::mlp_num_params(); }
+
+int fsph_mlp_get_params(fsph_ctx *ctx, float *p, int64_t n) {
+  CTX_OR_FAIL;
+  const int np = fsph::mlp_num_params();
+  if (!p || n < np) return fsph::fail(c, FSPH_E_ARG, "need %d parameters", np);
+  FSPH_TRY(fsph::mlp_ensure(c));
+  FSPH_HIP(c, hipMemcpyAsync(p, c->mlp.p.p, (size_t)np * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_mlp_set_params(fsph_ctx *ctx, const float *p, int64_t n) {
+  CTX_OR_FAIL;
+  const int np = fsph::mlp_num_params();
+  if (!p || n < np) return fsph::fail(c, FSPH_E_ARG, "need %d parameters", np);
+  FSPH_TRY(fsph::mlp_ensure(c));
+  // rows 6 and 7 of W1 multiply zero inputs; they are kept at zero
+  std::vector<float> h(p, p + np);
+  for (int i = 6 * 64; i < 8 * 64; ++i) h[i] = 0.f;
+  auto &M = c->mlp;
+  FSPH_HIP(c, hipMemcpyAsync(M.p.p, h.data(), (size_t)np * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  FSPH_HIP(c, hipMemsetAsync(M.m.p, 0, (size_t)np * 4, c->stream));
+  FSPH_HIP(c, hipMemsetAsync(M.v.p, 0, (size_t)np * 4, c->stream));
+  M.step = 0;
+  FSPH_TRY(fsph::mlp_refresh(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+namespace {
+// Uploads x[B][6] (padded to 8 columns) and optionally t[B] to the MLP's
+// host-call staging buffers.
+int stage_batch(Ctx *c, const float *x, const float *t, int64_t B) {
+  auto &M = c->mlp;
+  FSPH_TRY(fsph::ensure(c, M.Xb, (size_t)B * 8 * 4));
+  FSPH_TRY(fsph::ensure(c, M.T, (size_t)B * 4));
+  std::vector<float> X((size_t)B * 8, 0.f);
+  for (int64_t b = 0; b < B; ++b)
+    for (int k = 0; k < 6; ++k) X[(size_t)b * 8 + k] = x[b * 6 + k];
+  FSPH_HIP(c, hipMemcpyAsync(M.Xb.p, X.data(), X.size() * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  if (t)
+    FSPH_HIP(c, hipMemcpyAsync(M.T.p, t, (size_t)B * 4,
+                               hipMemcpyHostToDevice, c->stream));
+  // the host vector must outlive the copy
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int read_loss(Ctx *c, float *loss) {
+  FSPH_HIP(c, hipMemcpyAsync(loss, c->mlp.g.as<float>() + fsph::mlp_num_params(),
+                             4, hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+}  // namespace
+
+int fsph_mlp_forward(fsph_ctx *ctx, const float *x, int64_t B, float *y) {
+  CTX_OR_FAIL;
+  if (B < 0 || (B > 0 && (!x || !y)))
+    return fsph::fail(c, FSPH_E_ARG, "bad MLP batch");
+  if (B == 0) return FSPH_OK;
+  FSPH_TRY(fsph::mlp_ensure(c));
+  FSPH_TRY(stage_batch(c, x, nullptr, B));
+  auto &M = c->mlp;
+  FSPH_TRY(fsph::ensure(c, M.Y, (size_t)B * 4));
+  FSPH_TRY(fsph::mlp_forward(c, M.Xb.as<float>(), B, M.Y.as<float>()));
+  FSPH_HIP(c, hipMemcpyAsync(y, M.Y.p, (size_t)B * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_mlp_gradient(fsph_ctx *ctx, const float *x, const float *t,
+                      int64_t B, float *grad, float *loss) {
+  CTX_OR_FAIL;
+  if (B <= 0 || !x || !t) return fsph::fail(c, FSPH_E_ARG, "bad MLP batch");
+  FSPH_TRY(fsph::mlp_ensure(c));
+  FSPH_TRY(stage_batch(c, x, t, B));
+  auto &M = c->mlp;
+  FSPH_TRY(fsph::mlp_gradient(c, M.Xb.as<float>(), M.T.as<float>(), B));
+  if (grad)
+    FSPH_HIP(c, hipMemcpyAsync(grad, M.g.p, (size_t)fsph::mlp_num_params() * 4,
+                               hipMemcpyDeviceToHost, c->stream));
+  float l = 0;
+  FSPH_TRY(read_loss(c, &l));
+  if (loss) *loss = l;
+  return FSPH_OK;
+}
+
+int fsph_mlp_train(fsph_ctx *ctx, const float *x, const float *t, int64_t B,
+                   int iters, float lr, float *loss) {
+  CTX_OR_FAIL;
+  if (B <= 0 || !x || !t || iters < 0 || !(lr > 0))
+    return fsph::fail(c, FSPH_E_ARG, "bad MLP training call");
+  FSPH_TRY(fsph::mlp_ensure(c));
+  FSPH_TRY(stage_batch(c, x, t, B));
+  auto &M = c->mlp;
+  for (int i = 0; i < iters; ++i) {
+    FSPH_TRY(fsph::mlp_gradient(c, M.Xb.as<float>(), M.T.as<float>(), B));
+    FSPH_TRY(fsph::mlp_adam(c, lr));
+  }
+  // loss after the last update
+  FSPH_TRY(fsph::mlp_gradient(c, M.Xb.as<float>(), M.T.as<float>(), B));
+  float l = 0;
+  FSPH_TRY(read_loss(c, &l));
+  if (loss) *loss = l;
+  return FSPH_OK;
+}
+
+int fsph_h_stats(fsph_ctx *ctx, int64_t *fallback, int64_t *clamped,
+                 int32_t *trained_steps, float *last_loss) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  if (fallback) *fallback = c->h_fallback;
+  if (clamped) *clamped = c->h_clamped;
+  if (trained_steps) *trained_steps = c->mlp.trained_steps;
+  if (last_loss) *last_loss = c->mlp.last_loss;
   return FSPH_OK;
 }
 

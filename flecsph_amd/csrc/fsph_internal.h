@@ -52,6 +52,8 @@ enum PField : int {
   PF_GY, PF_GZ, PF_PHI, // real-typed fields
   PF_NNGB,              // int32
   PF_HNGB, PF_HITER,    // int32: smoothing-length solver outputs
+  PF_DIVV,              // real: velocity divergence (learned-h feature)
+  PF_DLH,               // real: ln(h_new / h_old) of the last h update
   PF_COUNT
 };
 
@@ -100,6 +102,29 @@ struct Ctx {
   DBuf gcand, gccount, gmoff, gmask;
   int64_t gmask_words = 0;
   DBuf h_prop;               // learned proposal of h (float), or empty
+  DBuf h_prev;               // h before the update (double)
+  DBuf hflag;                // per-particle fallback flags (int32)
+  DBuf glist;                // compacted group list (int32)
+  DBuf hgflag, hgscan;       // per-group fallback flags and their scan
+  DBuf hcnt;                 // int32 counters: [0] flagged, [1] clamped
+
+  // learned smoothing-length predictor (k_mlp.hip)
+  struct Mlp {
+    bool ready = false, norm_set = false;
+    int step = 0;          // Adam steps taken
+    int trained_steps = 0; // simulation steps used for training
+    float mean[6] = {0}, inv_std[6] = {0};
+    float last_loss = 0;
+    float zero_loss = 0;   // mean target^2 of the last batch
+    bool use_model = false;  // the model beats the zero (no-change) proposal
+    DBuf p, m, v, g, W1b, W2b, W2bT;  // parameters, Adam moments, grads
+    DBuf X, Xb, T, Y, idx, norm;      // feature/target staging
+    bool tscale_set = false;
+    float tscale = 1;                 // target scale (1 / std of ln h ratio)
+    float kappa = 0;                  // gather radius factor of the last step
+  } mlp;
+  // learned-h statistics of the last update
+  int64_t h_fallback = 0, h_clamped = 0;
 
   // FMM gravity: cells (internal nodes and buckets), pair lists
   struct Grav {
@@ -176,9 +201,21 @@ int permute_particles(Ctx *c, const uint32_t *perm, int64_t n_out);
 int stage_tree(Ctx *c);                // k_tree.hip
 int stage_groups(Ctx *c);              // k_tree.hip
 int stage_neighbors(Ctx *c);           // k_neighbors.hip
-// gather tiles: all b with r_ab < scale * h_a per group particle a
-int stage_gather_tiles(Ctx *c, double scale);  // k_neighbors.hip
+// gather tiles: all b with r_ab < scale * h_a per group particle a, with
+// h_a taken from hprop (float) when given; with glist only the ngl listed
+// groups are built, tile slot i belonging to group glist[i].
+int stage_gather_tiles(Ctx *c, double scale, const float *hprop = nullptr,
+                       const int32_t *glist = nullptr, int64_t ngl = -1);
 int stage_newton_h(Ctx *c, const float *h0);   // k_hsolve.hip
+int stage_learned_h(Ctx *c);                    // k_hsolve.hip
+// MLP (k_mlp.hip): X is B x 8 floats (6 features, 2 zero pads)
+int mlp_ensure(Ctx *c);
+int mlp_refresh(Ctx *c);
+int mlp_forward(Ctx *c, const float *X, int64_t B, float *Y);
+int mlp_gradient(Ctx *c, const float *X, const float *T, int64_t B);
+int mlp_adam(Ctx *c, float lr);
+int mlp_num_params();
+void free_mlp(Ctx *c);
 int stage_density(Ctx *c);             // k_sph.hip
 int stage_acceleration(Ctx *c);        // k_sph.hip
 int stage_dudt(Ctx *c);                // k_sph.hip

@@ -11,6 +11,8 @@
 // candidates in LDS each iteration, until every particle of the group has
 // converged.  Kernel sums in T with Kahan compensation; membership r < h and
 // the h arithmetic in FP64.
+#include <cmath>
+
 #include "fsph_internal.h"
 
 namespace fsph {
@@ -78,13 +80,19 @@ template <class T> struct NewtonArgs {
   const int32_t *gstart, *gcount, *cand, *ccount;
   const int64_t *moff;
   const uint32_t *mask;
+  const int32_t *glist;  // tile slot -> group (or null: slot = group)
   int64_t ccap;
   const double *x, *y, *z, *m;
   double *h;
-  const float *h0;  // proposal or null
+  const float *h0;       // proposal or null
+  int h0_bracket;        // bracket (0, kappa h0) instead of (0, kappa h)
+  const int32_t *only;   // solve only particles with only[i] != 0 (or null)
+  int32_t *flag;         // out: fallback flag per particle (or null)
+  int32_t *gflag;        // out (with flag): any particle of the slot flagged
+  int32_t *cnt;          // out (with flag): [0] += flagged, [1] += clamped
   int32_t *hngb, *hiter;
   int kernel, maxit;
-  double kappa, c3, tol;
+  double kappa, c3, tol, ntol;
 };
 
 template <class T>
@@ -93,10 +101,11 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   __shared__ T sm[kChunk];
   __shared__ int32_t sid[kChunk];
   __shared__ uint32_t smask[kChunkW * kB];
-  const int64_t g = blockIdx.x;
+  const int64_t gi = blockIdx.x;
+  const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
-  const int gc = a.gcount[g], cc = a.ccount[g];
-  const int64_t mo = a.moff[g], cb = g * a.ccap;
+  const int gc = a.gcount[g], cc = a.ccount[gi];
+  const int64_t mo = a.moff[gi], cb = gi * a.ccap;
   const int LP = lanes_per_particle(gc);
   const int k = threadIdx.x / LP, r = threadIdx.x & (LP - 1);
   const bool active = k < gc;
@@ -106,14 +115,21 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
     px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
     ma = a.m[ia];
     H = a.kappa * a.h[ia];
-    hi = H;
     h = a.h[ia];
     if (a.h0) {
       const double p = (double)a.h0[ia];
-      if (p > 0 && p < H) h = p;
+      if (a.h0_bracket) {
+        H = a.kappa * p;
+        h = p;
+      } else if (p > 0 && p < H) {
+        h = p;
+      }
     }
+    hi = H;
   }
-  bool done = !active;
+  const bool solve = active && !(a.only && !a.only[ia]);
+  bool done = !solve, conv = false;
+  double f0 = 0;
   int it = 0, nb = 0;
   // The last pass (after every particle has converged) only counts
   // neighbours at the final h.
@@ -165,6 +181,7 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
       const double h3 = h * h * h;
       const double f = (double)rho * h3 - ma * a.c3;
       const double fp = 3.0 * h * h * (double)rho + h3 * (double)drho;
+      if (it == 0) f0 = f;
       if (f < 0)
         lo = h;
       else
@@ -174,34 +191,54 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
       ++it;
       const double dh = fabs(hn - h);
       h = hn;
-      if (dh < a.tol * h || it >= a.maxit) done = true;
+      if (dh < a.tol * h) {
+        done = true;
+        conv = true;
+      } else if (it >= a.maxit) {
+        done = true;
+      }
     }
     if (!__syncthreads_or(!done)) final_pass = true;
   }
-  if (active && r == 0) {
+  if (a.flag) {
+    // A root at the bracket's upper end lies beyond the gather radius; a
+    // proposal whose weighted count (4pi/3) h^3 rho~ / m + 1 is off the
+    // target by more than ntol is not trusted either.
+    const bool clamped = solve && (!conv || h >= H * (1.0 - 2.0 * a.tol));
+    const bool far = solve && (4.0 * M_PI / 3.0) * fabs(f0) / ma > a.ntol;
+    const bool fl = clamped || far;
+    const int nf = __syncthreads_count(fl && r == 0);
+    const int ncl = __syncthreads_count(clamped && r == 0);
+    if (threadIdx.x == 0) {
+      a.gflag[gi] = nf > 0;
+      if (nf) atomicAdd(&a.cnt[0], nf);
+      if (ncl) atomicAdd(&a.cnt[1], ncl);
+    }
+    if (active && r == 0) a.flag[ia] = fl;
+  }
+  if (solve && r == 0) {
     a.h[ia] = h;
     a.hngb[ia] = nb;
-    a.hiter[ia] = it;
+    a.hiter[ia] = a.only ? a.hiter[ia] + it : it;
   }
 }
 
-template <class T> int newton_impl(Ctx *c, const float *h0) {
-  const double kappa = c->p.h_search > 1 ? c->p.h_search : 1.3;
-  FSPH_TRY(stage_gather_tiles(c, kappa));
-  NewtonArgs<T> a;
+template <class T>
+NewtonArgs<T> newton_args(Ctx *c, double kappa, const int32_t *glist) {
+  NewtonArgs<T> a{};
   a.gstart = c->gstart.as<int32_t>();
   a.gcount = c->gcount.as<int32_t>();
   a.cand = c->gcand.as<int32_t>();
   a.ccount = c->gccount.as<int32_t>();
   a.moff = c->gmoff.as<int64_t>();
   a.mask = c->gmask.as<uint32_t>();
+  a.glist = glist;
   a.ccap = c->gccap;
   a.x = c->f[PF_X].as<double>();
   a.y = c->f[PF_Y].as<double>();
   a.z = c->f[PF_Z].as<double>();
   a.m = c->f[PF_M].as<double>();
   a.h = c->f[PF_H].as<double>();
-  a.h0 = h0;
   a.hngb = c->f[PF_HNGB].as<int32_t>();
   a.hiter = c->f[PF_HITER].as<int32_t>();
   a.kernel = c->p.kernel;
@@ -209,17 +246,390 @@ template <class T> int newton_impl(Ctx *c, const float *h0) {
   a.kappa = kappa;
   a.c3 = 3.0 / (4.0 * M_PI) * (c->p.n_target - 1);
   a.tol = c->p.newton_tol > 0 ? c->p.newton_tol : 1e-4;
+  a.ntol = c->p.n_tol > 0 ? c->p.n_tol : 5;
+  return a;
+}
+
+template <class T>
+int launch_newton(Ctx *c, const NewtonArgs<T> &a, int64_t nslots) {
+  if (nslots <= 0) return 0;
   {
     PhaseTimer tm(c, FSPH_T_H);
-    k_newton<T><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
+    k_newton<T><<<(unsigned)nslots, kB, 0, c->stream>>>(a);
   }
   return hip_check(c, hipGetLastError(), "k_newton");
+}
+
+double search_factor(const Ctx *c) {
+  return c->p.h_search > 1 ? c->p.h_search : 1.3;
+}
+
+template <class T> int newton_impl(Ctx *c, const float *h0) {
+  const double kappa = search_factor(c);
+  FSPH_TRY(stage_gather_tiles(c, kappa));
+  NewtonArgs<T> a = newton_args<T>(c, kappa, nullptr);
+  a.h0 = h0;
+  return launch_newton(c, a, c->ng);
+}
+
+// ---------------------------------------------------------------------------
+// Learned smoothing length (new in this build).  Per update:
+//  1. features of every particle (and of a strided training sample) from
+//     the fields of the current step, normalised with statistics frozen at
+//     the first update;
+//  2. during the first learn_steps updates: plain Newton-h, whose results
+//     ln(h_new / h_old) are the training targets;
+//     afterwards: the MLP proposes h_p = h exp(y), the gather tiles shrink to
+//     radius kappa_L h_p (kappa_L from the predictor's training error), and
+//     Newton starts at h_p inside (0, kappa_L h_p); particles whose root lies
+//     at the bracket end, that do not converge, or whose proposal is off the
+//     target count by more than n_tol are reset and solved by plain Newton-h
+//     over the groups that hold them;
+//  3. a few Adam steps on the sample's (features, target) pairs: training
+//     continues online in both phases.
+// ---------------------------------------------------------------------------
+constexpr int64_t kBatch = 32768;
+constexpr int kFeat = 6, kFeatP = 8;
+// floors of the per-feature standard deviations used for normalisation
+const float kStdFloor[kFeat] = {0.02f, 0.02f, 1e-3f, 0.02f, 1e-3f, 1e-3f};
+
+template <class T> struct FeatArgs {
+  const double *h, *m;
+  const T *rho, *divv, *vsig, *ax, *ay, *az, *dlh;
+  const int32_t *nngb;
+  const int32_t *idx;  // sample -> particle (or null: sample = particle)
+  const float *norm;   // mean[6], inv_std[6]
+  int64_t B;
+  double dt, cN, invN;
+  float *X;
+};
+
+// Features: ln((4pi/3) rho h^3 / (m (N_t - 1))), (N_ngb - N_t) / N_t,
+// dt div v, dt vsig / h, ln(h / h_old) of the previous update,
+// |a| dt^2 / h.
+template <class T>
+__global__ __launch_bounds__(256) void k_features(FeatArgs<T> a) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.B) return;
+  const int64_t i = a.idx ? a.idx[j] : j;
+  const double h = a.h[i];
+  const double ih = 1.0 / h;
+  const double rho = (double)a.rho[i];
+  const double ax = a.ax[i], ay = a.ay[i], az = a.az[i];
+  float f[kFeat];
+  f[0] = (float)log(a.cN * rho * h * h * h / a.m[i]);
+  f[1] = (float)(a.nngb[i] * a.invN - 1.0);
+  f[2] = (float)(a.dt * (double)a.divv[i]);
+  f[3] = (float)(a.dt * (double)a.vsig[i] * ih);
+  f[4] = (float)a.dlh[i];
+  f[5] = (float)(sqrt(ax * ax + ay * ay + az * az) * a.dt * a.dt * ih);
+  float *o = a.X + j * kFeatP;
+  for (int q = 0; q < kFeat; ++q) {
+    float v = (f[q] - a.norm[q]) * a.norm[kFeat + q];
+    v = v != v ? 0.f : fminf(8.f, fmaxf(-8.f, v));
+    o[q] = v;
+  }
+  o[6] = 0.f;
+  o[7] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_sample_idx(int32_t *idx, int64_t B,
+                                                    int64_t n, int64_t stride,
+                                                    int64_t off) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < B) idx[j] = (int32_t)((off + j * stride) % n);
+}
+
+__global__ __launch_bounds__(256) void k_targets(const int32_t *idx,
+                                                 const double *h,
+                                                 const double *h_old,
+                                                 int64_t B, float scale,
+                                                 float *T) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < B) {
+    const int32_t i = idx[j];
+    T[j] = (float)(scale * log(h[i] / h_old[i]));
+  }
+}
+
+// h_prop = h exp(y); predicted changes below `snap` (the proposal's own
+// error) are not significant and leave h_prop = h.
+__global__ __launch_bounds__(256) void k_propose(const float *Y,
+                                                 const double *h, int64_t n,
+                                                 float inv_scale, float snap,
+                                                 float *hprop) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    float y = Y ? Y[i] * inv_scale : 0.f;
+    y = y != y ? 0.f : fminf(0.25f, fmaxf(-0.25f, y));
+    if (fabsf(y) < snap) y = 0.f;
+    hprop[i] = (float)(h[i] * exp((double)y));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_restore(const int32_t *flag,
+                                                 const double *h_old,
+                                                 int64_t n, double *h) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && flag[i]) h[i] = h_old[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_dlh(const double *h,
+                                             const double *h_old, int64_t n,
+                                             T *dlh) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dlh[i] = (T)log(h[i] / h_old[i]);
+}
+
+__global__ __launch_bounds__(256) void k_group_list(const int32_t *gflag,
+                                                    const int64_t *gscan,
+                                                    int64_t ng,
+                                                    int32_t *glist) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g < ng && gflag[g]) glist[gscan[g]] = (int32_t)g;
+}
+
+inline unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+template <class T> FeatArgs<T> feat_args(Ctx *c) {
+  FeatArgs<T> a{};
+  a.h = c->f[PF_H].as<double>();
+  a.m = c->f[PF_M].as<double>();
+  a.rho = c->f[PF_RHO].as<T>();
+  a.divv = c->f[PF_DIVV].as<T>();
+  a.vsig = c->f[PF_VSIG].as<T>();
+  a.ax = c->f[PF_AX].as<T>();
+  a.ay = c->f[PF_AY].as<T>();
+  a.az = c->f[PF_AZ].as<T>();
+  a.dlh = c->f[PF_DLH].as<T>();
+  a.nngb = c->f[PF_NNGB].as<int32_t>();
+  a.norm = c->mlp.norm.as<float>();
+  a.dt = c->dt;
+  const double nt = c->p.n_target > 1 ? c->p.n_target : 2;
+  a.cN = 4.0 * M_PI / 3.0 / (nt - 1.0);
+  a.invN = 1.0 / nt;
+  return a;
+}
+
+// Mean and floored inverse standard deviation of the sample's raw features.
+int set_normalisation(Ctx *c, int64_t B) {
+  auto &M = c->mlp;
+  std::vector<float> X((size_t)B * kFeatP);
+  FSPH_HIP(c, hipMemcpyAsync(X.data(), M.Xb.p, X.size() * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  float norm[16] = {0};
+  for (int q = 0; q < kFeat; ++q) {
+    double s = 0, s2 = 0;
+    for (int64_t j = 0; j < B; ++j) s += X[(size_t)j * kFeatP + q];
+    const double mu = s / (double)B;
+    for (int64_t j = 0; j < B; ++j) {
+      const double d = X[(size_t)j * kFeatP + q] - mu;
+      s2 += d * d;
+    }
+    const double sd = std::sqrt(s2 / (double)B);
+    norm[q] = (float)mu;
+    norm[kFeat + q] = (float)(1.0 / std::max(sd, (double)kStdFloor[q]));
+  }
+  FSPH_HIP(c, hipMemcpyAsync(M.norm.p, norm, sizeof(norm),
+                             hipMemcpyHostToDevice, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < kFeat; ++q) {
+    M.mean[q] = norm[q];
+    M.inv_std[q] = norm[kFeat + q];
+  }
+  M.norm_set = true;
+  return 0;
+}
+
+int set_target_scale(Ctx *c, int64_t B) {
+  auto &M = c->mlp;
+  std::vector<float> t((size_t)B);
+  FSPH_HIP(c, hipMemcpyAsync(t.data(), M.T.p, t.size() * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  double s = 0, s2 = 0;
+  for (float v : t) s += v;
+  const double mu = s / (double)B;
+  for (float v : t) s2 += (v - mu) * (v - mu);
+  const double sd = std::sqrt(s2 / (double)B);
+  M.tscale = (float)(1.0 / std::max(sd, 1e-4));
+  M.tscale_set = true;
+  return 0;
+}
+
+template <class T> int learned_impl(Ctx *c) {
+  auto &M = c->mlp;
+  FSPH_TRY(mlp_ensure(c));
+  const int64_t n = c->n, nr = c->n_real;
+  if (n <= 0) return 0;
+  const double kappa = search_factor(c);
+  FSPH_TRY(ensure(c, c->h_prev, (size_t)n * 8));
+  FSPH_HIP(c, hipMemcpyAsync(c->h_prev.p, c->f[PF_H].p, (size_t)n * 8,
+                             hipMemcpyDeviceToDevice, c->stream));
+  const int64_t B = std::min<int64_t>(nr, kBatch);
+  FSPH_TRY(ensure(c, M.Xb, (size_t)B * kFeatP * 4));
+  FSPH_TRY(ensure(c, M.T, (size_t)B * 4));
+  FSPH_TRY(ensure(c, M.idx, (size_t)B * 4));
+  FSPH_TRY(ensure(c, M.norm, 16 * 4));
+  const int64_t stride = std::max<int64_t>(1, nr / B);
+  const int64_t off = (c->iteration * 7919) % nr;
+  FeatArgs<T> fa = feat_args<T>(c);
+  // The update of the initial iteration relaxes the input's smoothing
+  // lengths to the solver's root: a transient unlike later updates, so it
+  // is solved but not learned from.
+  if (c->iteration == 0 && !M.tscale_set) {
+    c->h_fallback = c->h_clamped = 0;
+    FSPH_TRY(newton_impl<T>(c, nullptr));
+    k_dlh<T><<<blocks256(n), 256, 0, c->stream>>>(
+        c->f[PF_H].as<double>(), c->h_prev.as<double>(), n,
+        c->f[PF_DLH].as<T>());
+    return hip_check(c, hipGetLastError(), "k_dlh");
+  }
+  {
+    PhaseTimer tm(c, FSPH_T_H);
+    k_sample_idx<<<blocks256(B), 256, 0, c->stream>>>(M.idx.as<int32_t>(), B,
+                                                       nr, stride, off);
+    FSPH_HIP(c, hipGetLastError());
+    fa.idx = M.idx.as<int32_t>();
+    fa.B = B;
+    fa.X = M.Xb.as<float>();
+    if (!M.norm_set) {
+      float ident[16] = {0};
+      for (int q = 0; q < kFeat; ++q) ident[kFeat + q] = 1.f;
+      FSPH_HIP(c, hipMemcpyAsync(M.norm.p, ident, sizeof(ident),
+                                 hipMemcpyHostToDevice, c->stream));
+      k_features<T><<<blocks256(B), 256, 0, c->stream>>>(fa);
+      FSPH_HIP(c, hipGetLastError());
+      FSPH_TRY(set_normalisation(c, B));
+    }
+    k_features<T><<<blocks256(B), 256, 0, c->stream>>>(fa);
+    FSPH_HIP(c, hipGetLastError());
+  }
+  const bool learning =
+      !M.tscale_set || M.trained_steps < std::max(1, (int)c->p.learn_steps);
+  c->h_fallback = c->h_clamped = 0;
+  if (learning) {
+    FSPH_TRY(newton_impl<T>(c, nullptr));
+  } else {
+    FSPH_TRY(ensure(c, M.X, (size_t)n * kFeatP * 4));
+    FSPH_TRY(ensure(c, M.Y, (size_t)n * 4));
+    FSPH_TRY(ensure(c, c->h_prop, (size_t)n * 4));
+    FSPH_TRY(ensure(c, c->hflag, (size_t)n * 4));
+    FSPH_TRY(ensure(c, c->hgflag, (size_t)c->ng * 4));
+    FSPH_TRY(ensure(c, c->hgscan, (size_t)(c->ng + 1) * 8));
+    FSPH_TRY(ensure(c, c->glist, (size_t)c->ng * 4));
+    FSPH_TRY(ensure(c, c->hcnt, 2 * 4));
+    {
+      PhaseTimer tm(c, FSPH_T_H);
+      if (M.use_model) {
+        fa.idx = nullptr;
+        fa.B = n;
+        fa.X = M.X.as<float>();
+        k_features<T><<<blocks256(n), 256, 0, c->stream>>>(fa);
+        FSPH_HIP(c, hipGetLastError());
+        FSPH_TRY(mlp_forward(c, M.X.as<float>(), n, M.Y.as<float>()));
+      }
+      k_propose<<<blocks256(n), 256, 0, c->stream>>>(
+          M.use_model ? M.Y.as<float>() : nullptr, c->f[PF_H].as<double>(),
+          n, 1.f / M.tscale,
+          2.f * std::sqrt(std::max(0.f, M.last_loss)) / M.tscale,
+          c->h_prop.as<float>());
+      FSPH_HIP(c, hipGetLastError());
+    }
+    // gather radius from the proposal's error on the last training batch
+    // (in ln h): six standard deviations, at least 3% and at most the plain
+    // search radius.  Without the model the proposal is h itself and its
+    // error the spread of the recent changes.
+    const double rms =
+        std::sqrt(std::max(0.f, M.use_model ? M.last_loss : M.zero_loss)) /
+        M.tscale;
+    const double kl = std::exp(std::min(std::log(kappa),
+                                        std::max(0.03, 6.0 * rms)));
+    M.kappa = (float)kl;
+    FSPH_TRY(stage_gather_tiles(c, kl, c->h_prop.as<float>()));
+    FSPH_HIP(c, hipMemsetAsync(c->hcnt.p, 0, 8, c->stream));
+    NewtonArgs<T> a = newton_args<T>(c, kl, nullptr);
+    a.h0 = c->h_prop.as<float>();
+    a.h0_bracket = 1;
+    a.flag = c->hflag.as<int32_t>();
+    a.gflag = c->hgflag.as<int32_t>();
+    a.cnt = c->hcnt.as<int32_t>();
+    FSPH_TRY(launch_newton(c, a, c->ng));
+    int32_t cnt[2] = {0, 0};
+    FSPH_HIP(c, hipMemcpyAsync(cnt, c->hcnt.p, 8, hipMemcpyDeviceToHost,
+                               c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    c->h_fallback = cnt[0];
+    c->h_clamped = cnt[1];
+    if (cnt[0] > 0) {
+      FSPH_TRY(scan_i32_to_i64(c, c->hgflag.as<int32_t>(),
+                               c->hgscan.as<int64_t>(), c->ng));
+      int64_t ngl = 0;
+      FSPH_HIP(c, hipMemcpyAsync(&ngl, c->hgscan.as<int64_t>() + c->ng, 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+      k_group_list<<<blocks256(c->ng), 256, 0, c->stream>>>(
+          c->hgflag.as<int32_t>(), c->hgscan.as<int64_t>(), c->ng,
+          c->glist.as<int32_t>());
+      FSPH_HIP(c, hipGetLastError());
+      k_restore<<<blocks256(n), 256, 0, c->stream>>>(
+          c->hflag.as<int32_t>(), c->h_prev.as<double>(), n,
+          c->f[PF_H].as<double>());
+      FSPH_HIP(c, hipGetLastError());
+      FSPH_HIP(c, hipStreamSynchronize(c->stream));
+      FSPH_TRY(stage_gather_tiles(c, kappa, nullptr, c->glist.as<int32_t>(),
+                                  ngl));
+      NewtonArgs<T> b = newton_args<T>(c, kappa, c->glist.as<int32_t>());
+      b.only = c->hflag.as<int32_t>();
+      FSPH_TRY(launch_newton(c, b, ngl));
+    }
+  }
+  {
+    PhaseTimer tm(c, FSPH_T_H);
+    const bool first = !M.tscale_set;
+    k_targets<<<blocks256(B), 256, 0, c->stream>>>(
+        M.idx.as<int32_t>(), c->f[PF_H].as<double>(),
+        c->h_prev.as<double>(), B, first ? 1.f : M.tscale, M.T.as<float>());
+    FSPH_HIP(c, hipGetLastError());
+    if (first) {
+      FSPH_TRY(set_target_scale(c, B));
+      k_targets<<<blocks256(B), 256, 0, c->stream>>>(
+          M.idx.as<int32_t>(), c->f[PF_H].as<double>(),
+          c->h_prev.as<double>(), B, M.tscale, M.T.as<float>());
+      FSPH_HIP(c, hipGetLastError());
+    }
+    const int iters = learning ? 8 : 2;
+    const float lr = learning ? 3e-3f : 1e-3f;
+    for (int q = 0; q < iters; ++q) {
+      FSPH_TRY(mlp_gradient(c, M.Xb.as<float>(), M.T.as<float>(), B));
+      FSPH_TRY(mlp_adam(c, lr));
+    }
+    k_dlh<T><<<blocks256(n), 256, 0, c->stream>>>(
+        c->f[PF_H].as<double>(), c->h_prev.as<double>(), n,
+        c->f[PF_DLH].as<T>());
+    FSPH_HIP(c, hipGetLastError());
+  }
+  float loss[2] = {0, 0};
+  FSPH_HIP(c, hipMemcpyAsync(loss, M.g.as<float>() + mlp_num_params(), 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  M.last_loss = loss[0];
+  M.zero_loss = loss[1];
+  // propose with the model only where it halves the no-change error
+  M.use_model = loss[0] < 0.5f * loss[1];
+  if (learning) M.trained_steps++;
+  return 0;
 }
 
 }  // namespace
 
 int stage_newton_h(Ctx *c, const float *h0) {
   return c->rsize == 4 ? newton_impl<float>(c, h0) : newton_impl<double>(c, h0);
+}
+
+int stage_learned_h(Ctx *c) {
+  return c->rsize == 4 ? learned_impl<float>(c) : learned_impl<double>(c);
 }
 
 }  // namespace fsph

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kT) void k_permute(PermArgs a,
 // recomputing it.
 const int kPersistent[] = {PF_X,  PF_Y,  PF_Z,  PF_VX,  PF_VY, PF_VZ,
                            PF_VHX, PF_VHY, PF_VHZ, PF_U, PF_H, PF_M,
-                           PF_ID, PF_RHO, PF_P,  PF_C};
+                           PF_ID, PF_RHO, PF_P,  PF_C, PF_DLH};
 constexpr int kNPersistent = sizeof(kPersistent) / sizeof(int);
 
 // ---------------------------------------------------------------------------

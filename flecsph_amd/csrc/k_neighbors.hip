@@ -79,6 +79,8 @@ struct TravArgs {
   // no group-box test; the SPH mode uses rscale = 1 with the box test.
   double rscale;
   int gather;
+  const float *hprop;    // gather radius base instead of h (or null)
+  const int32_t *glist;  // tile slot -> group (or null: slot = group)
 };
 
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
@@ -87,7 +89,8 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
       sh[kMaxGroup];
   __shared__ int32_t stk[kStack];
   const int lane = threadIdx.x;
-  const int64_t g = blockIdx.x;
+  const int64_t gi = blockIdx.x;
+  const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
   const int gn = a.gather ? -1 : a.gnode[g];
@@ -95,7 +98,8 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     sx[k] = a.x[gs + k];
     sy[k] = a.y[gs + k];
     sz[k] = a.z[gs + k];
-    sh[k] = a.gather ? a.rscale * a.h[gs + k] : a.h[gs + k];
+    sh[k] = !a.gather ? a.h[gs + k]
+            : a.rscale * (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
   }
   double gb[6];
   if (gn >= 0)
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
   __syncthreads();
   int sp = 1;
   int64_t ncand = 0;
-  const int64_t cbase = g * a.ccap;
+  const int64_t cbase = gi * a.ccap;
   bool overflow = false;
   while (sp > 0) {
     const int nb = sp < kW ? sp : kW;
@@ -159,14 +163,18 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     ncand += tl;
     __syncthreads();
   }
-  if (lane == 0) a.ccount[g] = overflow ? 0 : (int32_t)ncand;
+  if (lane == 0) a.ccount[gi] = overflow ? 0 : (int32_t)ncand;
 }
 
 __global__ __launch_bounds__(256) void k_mask_words(
     const int32_t *__restrict__ ccount, const int32_t *__restrict__ gcount,
-    int64_t ng, int64_t *__restrict__ words) {
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g < ng) words[g] = (int64_t)gcount[g] * ((ccount[g] + 31) / 32);
+    const int32_t *__restrict__ glist, int64_t ng,
+    int64_t *__restrict__ words) {
+  const int64_t gi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (gi < ng) {
+    const int64_t g = glist ? glist[gi] : gi;
+    words[gi] = (int64_t)gcount[g] * ((ccount[gi] + 31) / 32);
+  }
 }
 
 struct MaskArgs {
@@ -181,6 +189,8 @@ struct MaskArgs {
   // gather mode: bit set iff r_kb < rscale * h_k (strict, own radius only)
   double rscale;
   int gather;
+  const float *hprop;    // gather radius base instead of h (or null)
+  const int32_t *glist;  // tile slot -> group (or null)
 };
 
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
@@ -188,13 +198,14 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
   __shared__ double cx[32], cy[32], cz[32], chh[32];
   __shared__ int cok[32];
   const int lane = threadIdx.x;
-  const int64_t g = blockIdx.x;
+  const int64_t gi = blockIdx.x;
+  const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
   const int gn = a.gather ? -1 : a.gnode[g];
-  const int cc = a.ccount[g];
-  const int64_t mo = a.moff[g];
-  const int64_t cbase = g * a.ccap;
+  const int cc = a.ccount[gi];
+  const int64_t mo = a.moff[gi];
+  const int64_t cbase = gi * a.ccap;
   double px[2], py[2], pz[2], ph[2];
   for (int s = 0; s < 2; ++s) {
     const int k = lane + kW * s;
@@ -202,7 +213,10 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       px[s] = a.x[gs + k];
       py[s] = a.y[gs + k];
       pz[s] = a.z[gs + k];
-      ph[s] = a.gather ? a.rscale * a.h[gs + k] : a.h[gs + k];
+      ph[s] = !a.gather
+                  ? a.h[gs + k]
+                  : a.rscale *
+                        (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
     }
   }
   double gcom[3] = {0, 0, 0}, gext_lap = 0, grad = 0;
@@ -272,8 +286,10 @@ struct TileBufs {
 };
 
 int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
-                int32_t *nngb) {
-  const int64_t ng = c->ng;
+                int32_t *nngb, const float *hprop, const int32_t *glist,
+                int64_t ngl) {
+  const int64_t ng = glist ? ngl : c->ng;
+  if (ng <= 0) return 0;
   FSPH_TRY(ensure(c, tb.ccount, (size_t)ng * 4));
   FSPH_TRY(ensure(c, tb.moff, (size_t)(ng + 1) * 8));
   for (int attempt = 0;; ++attempt) {
@@ -286,7 +302,8 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                c->gnode.as<int32_t>(),  tb.cand.as<int32_t>(),
                tb.ccount.as<int32_t>(), tb.ccap,
                c->flags.as<int>(),      scale,
-               gather ? 1 : 0};
+               gather ? 1 : 0,          hprop,
+               glist};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
       k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
@@ -306,7 +323,7 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
     break;
   }
   k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
-      tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), ng,
+      tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), glist, ng,
       tb.moff.as<int64_t>());
   FSPH_HIP(c, hipGetLastError());
   FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), ng));
@@ -324,7 +341,8 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
              tb.cand.as<int32_t>(),   tb.ccount.as<int32_t>(),
              tb.moff.as<int64_t>(),   tb.ccap,
              tb.mask.as<uint32_t>(),  nngb,
-             scale,                   gather ? 1 : 0};
+             scale,                   gather ? 1 : 0,
+             hprop,                   glist};
   {
     PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
     k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
@@ -336,13 +354,15 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
 
 int stage_neighbors(Ctx *c) {
   TileBufs tb{c->cand, c->ccount, c->moff, c->mask, c->ccap, c->mask_words};
-  return build_tiles(c, tb, false, 1.0, c->f[PF_NNGB].as<int32_t>());
+  return build_tiles(c, tb, false, 1.0, c->f[PF_NNGB].as<int32_t>(), nullptr,
+                     nullptr, -1);
 }
 
-int stage_gather_tiles(Ctx *c, double scale) {
+int stage_gather_tiles(Ctx *c, double scale, const float *hprop,
+                       const int32_t *glist, int64_t ngl) {
   TileBufs tb{c->gcand, c->gccount, c->gmoff, c->gmask, c->gccap,
               c->gmask_words};
-  return build_tiles(c, tb, true, scale, nullptr);
+  return build_tiles(c, tb, true, scale, nullptr, hprop, glist, ngl);
 }
 
 }  // namespace fsph

@@ -240,26 +240,30 @@ __global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <class T> struct VsigArgs {
   Tiles t;
-  const double *x, *y, *z, *vx, *vy, *vz;
-  const T *c, *c_old;
+  const double *x, *y, *z, *vx, *vy, *vz, *h, *m;
+  const T *c, *c_old, *rho;
   T *vsig;
+  T *divv;  // with DIV: div v_a = -(1/rho_a) sum_b m_b v_ab . grad_a W_ab
+  int kernel;
 };
 
-template <class T>
+template <class T, bool DIV>
 __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
   __shared__ T svx[kChunk], svy[kChunk], svz[kChunk], scn[kChunk],
       sco[kChunk];
+  __shared__ T sh[DIV ? kChunk : 1], sm[DIV ? kChunk : 1];
   __shared__ int32_t sid[kChunk];
   __shared__ uint32_t smask[kChunkW * kB];
   const TileCtx c = tile_ctx(a.t);
   double px = 0, py = 0, pz = 0;
-  T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0;
+  T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0, ph = 1, dv = 0;
   const int32_t ia = (int32_t)(c.gs + c.k);
   if (c.active) {
     px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
     pvx = (T)a.vx[ia]; pvy = (T)a.vy[ia]; pvz = (T)a.vz[ia];
     pc = a.c[ia];
+    if (DIV) ph = (T)a.h[ia];
   }
   for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
     const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
@@ -269,6 +273,10 @@ __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
       svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
       scn[j] = a.c[b]; sco[j] = a.c_old[b];
       sid[j] = b;
+      if (DIV) {
+        sh[j] = (T)a.h[b];
+        sm[j] = (T)a.m[b];
+      }
     }
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
@@ -276,19 +284,28 @@ __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
       const double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
       if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
       const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-      const T ir = rsq(dx * dx + dy * dy + dz * dz);
-      const T vdn =
-          ((pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz) *
-          ir;
+      const T r2 = dx * dx + dy * dy + dz * dz;
+      const T ir = rsq(r2);
+      const T vdr =
+          (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
+      const T vdn = vdr * ir;
       const T cb = sid[j] >= ia ? scn[j] : sco[j];
       const T cab = pc < cb ? cb : pc;
       const T v = cab - (vdn < T(0) ? vdn : T(0));
       vs = vs < v ? v : vs;
+      if (DIV) {
+        const T ih = rcp(T(0.5) * (ph + sh[j]));
+        dv += sm[j] * vdr * kernel_F<T>(a.kernel, r2 * ir, ir, ih);
+      }
     });
     __syncthreads();
   }
   vs = xor_max(vs, c.LP);
-  if (c.active && c.r == 0) a.vsig[ia] = vs;
+  if (DIV) dv = xor_sum(dv, c.LP);
+  if (c.active && c.r == 0) {
+    a.vsig[ia] = vs;
+    if (DIV) a.divv[ia] = -dv / a.rho[ia];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -451,9 +468,16 @@ template <class T> int density_impl(Ctx *c) {
   v.vz = c->f[PF_VZ].as<double>();
   v.c = c->f[PF_C].as<T>(); v.c_old = c->f[PF_COLD].as<T>();
   v.vsig = c->f[PF_VSIG].as<T>();
+  v.h = a.h; v.m = a.m;
+  v.rho = a.rho;
+  v.divv = c->f[PF_DIVV].as<T>();
+  v.kernel = c->p.kernel;
   {
     PhaseTimer tm(c, FSPH_T_VSIG);
-    k_vsig<T><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
+    if (c->p.h_mode == FSPH_H_LEARNED)
+      k_vsig<T, true><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
+    else
+      k_vsig<T, false><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
   }
   return hip_check(c, hipGetLastError(), "k_vsig");
 }

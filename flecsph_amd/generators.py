@@ -155,13 +155,32 @@ def polytrope(n=1_000_000, seed=SEED_BASE + 3):
                  rho_floor, P, c), params
 
 
-def collapse(n=4_000_000, seed=SEED_BASE + 5):
-    """Cold collapse of a uniform sphere: n seeded uniform points in the unit
-    ball, rho = 1, u = 0.05, gamma = 5/3, G = 1, theta = 0.5, 100
-    neighbours."""
+def uniform_sphere(n=1_000_000, seed=SEED_BASE + 5):
+    """n seeded uniform random points in the unit ball, rho = 1,
+    u = 0.05, gamma = 5/3, Wendland C6 with 100 neighbours (Poisson
+    positions: a test input for the smoothing-length solvers)."""
     rng = np.random.default_rng(seed)
     dirs, r = _uniform_ball(n, rng)
-    pos = dirs * r[:, None]
+    return _cold_sphere(dirs * r[:, None])
+
+
+def _fcc_ball(n, rng):
+    """Points of a face-centred cubic lattice inside the unit ball, with
+    about n points: lattice constant a from 4 / a^3 = n / (4 pi / 3), the
+    lattice origin shifted by a seeded offset in the unit cell."""
+    a = (4.0 * (4.0 * math.pi / 3.0) / n) ** (1.0 / 3.0)
+    m = int(math.ceil(1.0 / a)) + 1
+    g = np.arange(-m, m + 1) * a
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    corner = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    basis = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a
+    shift = (rng.random(3) - 0.5) * a
+    pts = (corner[:, None, :] + basis[None, :, :]).reshape(-1, 3) + shift
+    return pts[np.einsum("ij,ij->i", pts, pts) < 1.0]
+
+
+def _cold_sphere(pos):
+    n = len(pos)
     m = np.full(n, (4.0 * math.pi / 3.0) / n)
     rho = np.ones(n)
     eta = _eta_for(100)
@@ -178,6 +197,16 @@ def collapse(n=4_000_000, seed=SEED_BASE + 5):
                   theta=0.5, leaf_size=32, n_target=100)
     return _pack(pos[:, 0], pos[:, 1], pos[:, 2], zero, zero, zero, u, h, m,
                  rho, P, c), params
+
+
+def collapse(n=4_000_000, seed=SEED_BASE + 5):
+    """Cold collapse of a uniform sphere: about n points of a seeded FCC
+    lattice in the unit ball (the lattice of data/collapse_nx10.par; random
+    points would leave close pairs that unsoftened gravity merges within a
+    few steps), rho = 1, u = 0.05, gamma = 5/3, G = 1, theta = 0.5,
+    Wendland C6 with 100 neighbours."""
+    rng = np.random.default_rng(seed)
+    return _cold_sphere(_fcc_ball(n, rng))
 
 
 def random_box(n=2000, seed=1, h_spread=0.5, n_ngb=40, v_scale=0.1,

@@ -24,6 +24,10 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
 class _System:
     """Common driver over bound ABI functions `fns` and an opaque handle."""
 
@@ -222,3 +226,49 @@ class BodySystem(_System):
         if rc:
             raise FsphError("fsph_phase_times", rc)
         return dict(zip(_abi.TIMERS, list(out)))
+
+    # -- learned smoothing-length predictor (libfsph.so only) ------------
+    def mlp_num_params(self):
+        self._lib.fsph_mlp_num_params.restype = C.c_int
+        self._lib.fsph_mlp_num_params.argtypes = []
+        return int(self._lib.fsph_mlp_num_params())
+
+    def mlp_params(self):
+        p = np.zeros(self.mlp_num_params(), np.float32)
+        self._call("mlp_get_params", _fptr(p), p.size)
+        return p
+
+    def set_mlp_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._call("mlp_set_params", _fptr(p), p.size)
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        y = np.zeros(len(x), np.float32)
+        self._call("mlp_forward", _fptr(x), len(x), _fptr(y))
+        return y
+
+    def mlp_gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        g = np.zeros(self.mlp_num_params(), np.float32)
+        loss = C.c_float()
+        self._call("mlp_gradient", _fptr(x), _fptr(t), len(x), _fptr(g),
+                   C.byref(loss))
+        return g, float(loss.value)
+
+    def mlp_train(self, x, t, iters, lr):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        loss = C.c_float()
+        self._call("mlp_train", _fptr(x), _fptr(t), len(x), int(iters),
+                   float(lr), C.byref(loss))
+        return float(loss.value)
+
+    def h_stats(self):
+        fb, cl = C.c_int64(), C.c_int64()
+        ts, loss = C.c_int32(), C.c_float()
+        self._call("h_stats", C.byref(fb), C.byref(cl), C.byref(ts),
+                   C.byref(loss))
+        return {"fallback": fb.value, "clamped": cl.value,
+                "trained_steps": ts.value, "last_loss": loss.value}

@@ -95,6 +95,11 @@ typedef struct fsph_params {
   double h_search;     /* gather radius of the Newton solve, in units of
                           the current h (1.3); the solution is clamped to
                           it and counted as unconverged                     */
+  int32_t learn_steps; /* FSPH_H_LEARNED: steps of plain Newton-h whose
+                          histories train the predictor online (50)         */
+  int32_t n_tol;       /* FSPH_H_LEARNED: a proposal whose kernel-weighted
+                          neighbour count is off the target by more than
+                          this falls back to plain Newton-h (5)             */
 } fsph_params;
 
 #define FSPH_H_CLOSED 0  /* h = eta kw (m/rho)^(1/3), default_physics.h:826 */
@@ -259,6 +264,39 @@ int fsph_get_tree(fsph_ctx *ctx, int64_t *n_nodes, uint64_t *node_key,
    (returned in off[n_total]).  cap is the length of ids. */
 int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
                         int64_t cap);
+
+/* ---- learned smoothing-length predictor (new; no reference counterpart:
+   the reference updates h in closed form, default_physics.h:826-848) ---- */
+
+/* The predictor is a 6 -> 64 -> 64 -> 1 tanh MLP (BF16 weights on MFMA,
+   FP32 accumulation and master weights): y = w3 . tanh(W2^T tanh(W1^T x +
+   b1) + b2) + b3.  In FSPH_H_LEARNED mode its inputs are the features
+   [ln((4 pi / 3) rho h^3 / (m (N_target - 1))), (N_ngb - N_target) /
+   N_target, dt div(v), dt vsig / h, ln(h / h_old) of the previous update,
+   |a| dt^2 / h], normalised with the mean and standard deviation of the
+   first update's sample, and its output is ln(h_new / h) times a scale
+   fixed at the first update.  Parameters are a flat FP32 vector of
+   fsph_mlp_num_params() entries: W1 [8][64] (input rows 6 and 7 unused and
+   kept zero), b1 [64], W2 [64][64], b2 [64], w3 [64], b3.  Setting the
+   parameters also resets the Adam moments. */
+int fsph_mlp_num_params(void);
+int fsph_mlp_get_params(fsph_ctx *ctx, float *p, int64_t n);
+int fsph_mlp_set_params(fsph_ctx *ctx, const float *p, int64_t n);
+/* y[B] = MLP(x[B][6]) for host arrays (features already normalised). */
+int fsph_mlp_forward(fsph_ctx *ctx, const float *x, int64_t B, float *y);
+/* Gradient of L = mean_b (MLP(x_b) - t_b)^2 with respect to the flat
+   parameters (grad has fsph_mlp_num_params() entries) and L itself. */
+int fsph_mlp_gradient(fsph_ctx *ctx, const float *x, const float *t,
+                      int64_t B, float *grad, float *loss);
+/* `iters` Adam steps (learning rate lr) on one host batch; final loss. */
+int fsph_mlp_train(fsph_ctx *ctx, const float *x, const float *t, int64_t B,
+                   int iters, float lr, float *loss);
+/* Statistics of the last smoothing-length update: particles that fell
+   back from the learned proposal to plain Newton-h, and particles whose
+   root lay beyond the gather radius (clamped); training steps done and the
+   last training loss. */
+int fsph_h_stats(fsph_ctx *ctx, int64_t *fallback, int64_t *clamped,
+                 int32_t *trained_steps, float *last_loss);
 
 /* Milliseconds spent in each phase of the last fsph_step, measured with
    HIP events on the context's stream around the phase's kernels (summed

@@ -56,7 +56,7 @@ def test_newton_h_counts_at_target_uniform_sphere():
     """1M uniform sphere, N_target 100: Newton solves the kernel-weighted
     count, so on Poisson-random positions the interior hard counts scatter
     around the target (a few percent high in the median)."""
-    parts, prm = generators.collapse(n=1_000_000)
+    parts, prm = generators.uniform_sphere(n=1_000_000)
     prm.update(h_mode=_abi.H_NEWTON, n_target=100)
     g = BodySystem(_abi.default_params(**prm))
     g.upload(parts)
