@@ -2,6 +2,7 @@
 FleCSPH behind the C ABI of include/fsph.h (libfsph.so)."""
 from . import _abi
 from ._abi import default_params
-from .system import BodySystem, FsphError
+from .system import BatchSystem, BodySystem, FsphError, autotune
 
-__all__ = ["BodySystem", "FsphError", "default_params", "_abi"]
+__all__ = ["BatchSystem", "BodySystem", "FsphError", "autotune",
+           "default_params", "_abi"]

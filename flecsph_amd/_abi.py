@@ -47,7 +47,8 @@ FIELDS = ["x", "y", "z", "vx", "vy", "vz", "vhx", "vhy", "vhz", "u", "h", "m",
 FIELD_INDEX = {name: i for i, name in enumerate(FIELDS)}
 
 TIMERS = ["integrate", "boundary", "range_keys", "sort", "tree", "traverse",
-          "masks", "density", "vsig", "accel", "gravity", "dudt", "h", "dt"]
+          "masks", "density", "vsig", "accel", "gravity", "dudt", "h", "dt",
+          "graph"]
 
 
 class FsphParams(C.Structure):
@@ -155,6 +156,12 @@ PROTOTYPES = {
     "mlp_train": (C.c_int, [_P, _PF, _PF, C.c_int64, C.c_int, C.c_float,
                             _PF]),
     "h_stats": (C.c_int, [_P, _PI64, _PI64, _PI32, _PF]),
+    "upload_batch": (C.c_int, [_P, C.POINTER(FsphParticles), C.c_int]),
+    "instance_count": (C.c_int, [_P, C.c_int, _PI64, _PI64]),
+    "instance_download": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "instance_time": (C.c_int, [_P, C.c_int, _PI64, _PD, _PD]),
+    "autotune": (C.c_int, [_P, C.c_int, _PI32, _PI32]),
+    "enable_graph": (C.c_int, [_P, C.c_int]),
 }
 
 

@@ -48,7 +48,7 @@ int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
 
 size_t field_size(const Ctx *c, int f) {
   if (f < PF_NDBL || f == PF_ID) return 8;
-  if (f == PF_NNGB || f == PF_HNGB || f == PF_HITER) return 4;
+  if (f == PF_NNGB || f == PF_HNGB || f == PF_HITER || f == PF_INST) return 4;
   return c->rsize;
 }
 
@@ -109,8 +109,34 @@ int need_tree(Ctx *c) {
   return 0;
 }
 
+// set_adaptive_timestep's rule (default_physics.h:762-823, no output
+// matching) for one simulation's (dt, totaltime)
+void dt_rule(double dtmin, double t, double *dt, double *tnext) {
+  if (dtmin < *dt) *dt = (*dt / 2.0 < dtmin) ? *dt / 2.0 : dtmin;
+  if (dtmin > 2.0 * *dt) *dt = *dt * 2.0;
+  *tnext = t + *dt;
+  volatile double tn = *tnext;
+  *dt = tn - t;
+}
+
 int do_timestep(Ctx *c) {
   if (!c->p.adaptive_dt) return 0;
+  if (c->batch.n_inst) {
+    auto &B = c->batch;
+    const int ni = B.n_inst;
+    if (!B.dt_ready) FSPH_TRY(batch_reduce(c, 1));
+    B.dt_ready = false;
+    std::vector<double> bd((size_t)ni * 8);
+    FSPH_HIP(c, hipMemcpyAsync(bd.data(), B.bounds.p, bd.size() * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < ni; ++i)
+      dt_rule(bd[(size_t)i * 8 + 7], B.t_h[i], &B.dt_h[i], &B.tnext_h[i]);
+    FSPH_HIP(c, hipMemcpyAsync(B.dt.p, B.dt_h.data(), (size_t)ni * 8,
+                               hipMemcpyHostToDevice, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+  }
   double dtmin = 0;
   FSPH_TRY(stage_dt_min(c, &dtmin));
   // set_adaptive_timestep, default_physics.h:762-823 (no output matching)
@@ -124,6 +150,12 @@ int do_timestep(Ctx *c) {
 
 int do_advance(Ctx *c) {
   c->iteration++;
+  if (c->batch.n_inst) {
+    auto &B = c->batch;
+    for (int i = 0; i < B.n_inst; ++i)
+      B.t_h[i] = c->p.adaptive_dt ? B.tnext_h[i] : B.t_h[i] + B.dt_h[i];
+    return 0;
+  }
   if (c->p.adaptive_dt)
     c->totaltime = c->totaltime_next;
   else
@@ -177,6 +209,17 @@ int do_step(Ctx *c) {
           c, FSPH_KICK_U | FSPH_KICK_V | FSPH_SAVE_VHALF | FSPH_DRIFT));
     }
     FSPH_TRY(do_update_iteration(c));
+    if (c->graph.on && c->batch.n_inst && c->p.h_mode == FSPH_H_CLOSED &&
+        !c->p.gravity) {
+      // the rest of the step up to the dt minimum as one graph replay
+      FSPH_TRY(graph_step_tail(c));
+      c->batch.dt_ready = c->p.adaptive_dt != 0;
+      {
+        PhaseTimer t(c, FSPH_T_DT);
+        FSPH_TRY(do_timestep(c));
+      }
+      return do_advance(c);
+    }
     FSPH_TRY(stage_density(c));
     FSPH_TRY(stage_acceleration(c));
     if (c->p.gravity) {
@@ -288,6 +331,13 @@ void fsph_destroy(fsph_ctx *ctx) {
     if (b->p) hipFree(b->p);
   fsph::free_gravity(c);
   fsph::free_mlp(c);
+  fsph::free_graph(c);
+  fsph::DBuf *more[] = {&c->graph.slots, &c->graph.sizes, &c->graph.counters,
+                        &c->batch.nreal, &c->batch.segr, &c->batch.seg,
+                        &c->batch.bounds, &c->batch.halo, &c->batch.root,
+                        &c->batch.dt};
+  for (auto *b : more)
+    if (b->p) hipFree(b->p);
   for (int f = 0; f < fsph::PF_COUNT; ++f) {
     if (c->f[f].p) hipFree(c->f[f].p);
     if (c->g[f].p) hipFree(c->g[f].p);
@@ -307,9 +357,29 @@ const char *fsph_last_error(const fsph_ctx *ctx) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);              \
   if (!c) return FSPH_E_ARG;                          \
   if (hipSetDevice(c->device) != hipSuccess) return FSPH_E_HIP
+// entry points whose meaning is one simulation's
+#define NOT_BATCH                                                      \
+  if (c->batch.n_inst)                                                 \
+  return fsph::fail(c, FSPH_E_ARG,                                     \
+                    "one-simulation call on a batched context; use the " \
+                    "fsph_instance_* calls")
+
+}  // extern "C"
+namespace fsph {
+int upload_particles(Ctx *c, const fsph_particles *in);
+}
+extern "C" {
 
 int fsph_upload(fsph_ctx *ctx, const fsph_particles *in) {
   CTX_OR_FAIL;
+  if (c->batch.n_inst)
+    return fsph::fail(c, FSPH_E_ARG, "batched context: use fsph_upload_batch");
+  return fsph::upload_particles(c, in);
+}
+
+}  // extern "C"
+
+int fsph::upload_particles(Ctx *c, const fsph_particles *in) {
   if (!in || in->n < 2 || !in->x || !in->y || !in->z || !in->vx || !in->vy ||
       !in->vz || !in->u || !in->h || !in->m || !in->id)
     return fsph::fail(c, FSPH_E_ARG, "upload needs n >= 2 and every field");
@@ -364,6 +434,8 @@ int fsph_upload(fsph_ctx *ctx, const fsph_particles *in) {
   c->initial_done = false;
   return FSPH_OK;
 }
+
+extern "C" {
 
 int fsph_count(const fsph_ctx *ctx, int64_t *n_total, int64_t *n_real) {
   const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
@@ -447,6 +519,7 @@ int fsph_acceleration(fsph_ctx *ctx) {
 
 int fsph_gravity(fsph_ctx *ctx) {
   CTX_OR_FAIL;
+  NOT_BATCH;
   FSPH_TRY(fsph::need_tree(c));
   FSPH_TRY(fsph::do_gravity(c));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
@@ -478,6 +551,7 @@ int fsph_update_h(fsph_ctx *ctx) {
 
 int fsph_timestep(fsph_ctx *ctx, double *dt) {
   CTX_OR_FAIL;
+  NOT_BATCH;
   FSPH_TRY(fsph::do_timestep(c));
   if (dt) *dt = c->dt;
   return FSPH_OK;
@@ -506,7 +580,7 @@ int fsph_step(fsph_ctx *ctx) {
 int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,
               double *totaltime) {
   const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
-  if (!c) return FSPH_E_ARG;
+  if (!c || c->batch.n_inst) return FSPH_E_ARG;
   if (iteration) *iteration = c->iteration;
   if (dt) *dt = c->dt;
   if (totaltime) *totaltime = c->totaltime;
@@ -522,12 +596,15 @@ int fsph_set_dt(fsph_ctx *ctx, double dt) {
 
 int fsph_reductions(fsph_ctx *ctx, double out[10]) {
   CTX_OR_FAIL;
+  NOT_BATCH;
+  NOT_BATCH;
   if (!out) return FSPH_E_ARG;
   return fsph::stage_reductions(c, out);
 }
 
 int fsph_get_range(fsph_ctx *ctx, double lo[3], double hi[3]) {
   CTX_OR_FAIL;
+  NOT_BATCH;
   for (int d = 0; d < 3; ++d) {
     lo[d] = c->lo[d];
     hi[d] = c->hi[d];
@@ -626,6 +703,17 @@ int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c || !ms) return FSPH_E_ARG;
   for (int i = 0; i < FSPH_NTIMERS; ++i) ms[i] = c->phase_ms[i];
+  return FSPH_OK;
+}
+
+int fsph_enable_graph(fsph_ctx *ctx, int on) {
+  CTX_OR_FAIL;
+  if (on && !c->batch.n_inst)
+    return fsph::fail(c, FSPH_E_ARG, "graph mode needs a batched context");
+  if (on && (c->p.h_mode != FSPH_H_CLOSED || c->p.gravity))
+    return fsph::fail(c, FSPH_E_ARG,
+                      "graph mode needs the closed-form h and no gravity");
+  c->graph.on = on != 0;
   return FSPH_OK;
 }
 

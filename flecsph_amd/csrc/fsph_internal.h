@@ -42,6 +42,11 @@ enum : int {
 
 struct Ctx;
 
+// graph argument slots (bytes each) and their order
+constexpr size_t kGraphSlot = 1024;
+enum : int { kSlotDens = 0, kSlotVsig, kSlotAcc, kSlotDudt, kSlotStep,
+             kNSlots };
+
 // Persistent per-particle fields.  FP64 fields are double; derived fields
 // (density ... phi) are float or double by precision.
 enum PField : int {
@@ -54,6 +59,7 @@ enum PField : int {
   PF_HNGB, PF_HITER,    // int32: smoothing-length solver outputs
   PF_DIVV,              // real: velocity divergence (learned-h feature)
   PF_DLH,               // real: ln(h_new / h_old) of the last h update
+  PF_INST,              // int32: instance of the particle (batched mode)
   PF_COUNT
 };
 
@@ -93,6 +99,7 @@ struct Ctx {
   int64_t ng = 0;
   DBuf gstart, gcount, gnode, gflag, gscan;
   int64_t ccap = 2048;       // candidate capacity per group
+  int chunk = 128;           // candidates per LDS round of the pair kernels
   DBuf cand, ccount, moff;   // cand: ng*ccap int32; ccount int32; moff int64
   DBuf mask;                 // uint32 words
   int64_t mask_words = 0;
@@ -125,6 +132,37 @@ struct Ctx {
   } mlp;
   // learned-h statistics of the last update
   int64_t h_fallback = 0, h_clamped = 0;
+
+  // Batched mode: n_inst independent simulations in one concatenated
+  // state, each instance's particles contiguous in (instance, key) order.
+  struct Batch {
+    int n_inst = 0;                   // 0: one simulation (no instance table)
+    std::vector<int64_t> n_real;      // real particles per instance
+    DBuf nreal;                       // int64 [n_inst]
+    DBuf segr;                        // int64 [n_inst + 1] offsets of reals
+    DBuf seg;                         // int64 [n_inst + 1] offsets, all
+    DBuf bounds;                      // double [n_inst * 8] reductions
+    DBuf halo;                        // double [n_inst] copy halo
+    DBuf root;                        // int32 [n_inst] root node
+    DBuf dt;                          // double [n_inst]
+    std::vector<double> dt_h, t_h, tnext_h;
+    std::vector<int64_t> seg_h;       // host copy of seg after the last sort
+    bool dt_ready = false;            // bounds[.][7] already reduced
+  } batch;
+
+  // Graph mode (batched contexts, closed-form h, no gravity): the sync-free
+  // tail of a step (pair passes, kicks, h, per-instance dt minimum) is
+  // captured once as a hipGraph of persistent kernels that read their
+  // arguments and sizes from device memory, and replayed every step.
+  struct Graph {
+    bool on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int key[4] = {0, 0, 0, 0};   // chunk, rsize, adaptive_dt, n_inst
+    DBuf slots, sizes, counters; // argument slots, {ng, n}, work counters
+    std::vector<char> host;      // host staging of slots and sizes
+    int grid_groups = 0, grid_elems = 0;
+  } graph;
 
   // FMM gravity: cells (internal nodes and buckets), pair lists
   struct Grav {
@@ -194,6 +232,9 @@ int scan_i64(Ctx *c, const int64_t *in, int64_t *out, int64_t n);
 int scan_i32_to_i64(Ctx *c, const int32_t *in, int64_t *out, int64_t n);
 
 // ---- pipeline stages ----------------------------------------------------
+// fields kept across a reordering (the reference's body_u state, plus the
+// instance index in batched mode); returns their count
+int persistent_fields(const Ctx *c, int *out);
 int stage_boundary(Ctx *c);            // k_keys.hip
 int stage_range_keys(Ctx *c);          // k_keys.hip
 int stage_sort(Ctx *c);                // k_keys.hip
@@ -228,5 +269,22 @@ int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
 int stage_max_h(Ctx *c, double *hmax);       // k_step.hip
 int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
                       int64_t n);              // k_step.hip
+// Batched mode (k_batch.hip): per-instance reductions over the particles
+// with wave-aggregated atomics; what: 0 range of x +- h (6 values) and
+// max h (1), 1 minimum particle time step (1).  Results in batch.bounds,
+// 8 doubles per instance.
+int batch_reduce(Ctx *c, int what);
+// seg[] from the sorted instance indices; root[] from the tree
+int batch_segments(Ctx *c);
+int batch_roots(Ctx *c);
+// graph mode: host staging of the argument slots (k_sph.hip: the four pair
+// passes; k_step.hip: kicks, h, dt) and the persistent launches recorded
+// into the graph (pass = slot index; what: 0 kick v, 1 kick u, 2 h, 3 dt)
+void sph_graph_args(Ctx *c, char *slots);
+int sph_graph_launch(Ctx *c, int pass);
+void step_graph_args(Ctx *c, char *slots);
+int step_graph_launch(Ctx *c, int what);
+int graph_step_tail(Ctx *c);          // fsph_graph.hip
+void free_graph(Ctx *c);
 
 }  // namespace fsph

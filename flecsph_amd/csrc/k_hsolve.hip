@@ -128,16 +128,25 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
     hi = H;
   }
   const bool solve = active && !(a.only && !a.only[ia]);
-  bool done = !solve, conv = false;
+  bool done = !solve, conv = false, need_count = false;
   double f0 = 0;
   int it = 0, nb = 0;
-  // The last pass (after every particle has converged) only counts
-  // neighbours at the final h.
+  // The hard count #{b : r_ab < h} is wanted at the final h, one Newton
+  // update after the last evaluation.  Each evaluation therefore also keeps
+  // (per lane, up to two) the r^2 of candidates within a relative band
+  // 2 tol around the evaluated h, which covers a converged update; the
+  // count at the new h then follows exactly from the count at the old one.
+  // Only particles whose band overflowed get a final counting pass.
+  const double band = 2.0 * a.tol;
   for (bool final_pass = false;;) {
     KSum<T> s0, s1;
-    int cnt = 0;
+    int cnt = 0, nbd = 0;
+    double bnd0 = 0, bnd1 = 0;
     const double h2 = h * h;
+    const double lo2 = h2 * (1.0 - band) * (1.0 - band);
+    const double hi2 = h2 * (1.0 + band) * (1.0 + band);
     const T th = (T)h, ih = T(1) / th;
+    const bool eval = active && (final_pass ? need_count : !done);
     for (int c0 = 0; c0 < cc; c0 += kChunk) {
       const int nch = min(kChunk, cc - c0), nw = (nch + 31) >> 5;
       for (int j = threadIdx.x; j < nch; j += kB) {
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
       const uint32_t *src = a.mask + mo + (int64_t)(c0 >> 5) * gc;
       for (int q = threadIdx.x; q < nw * gc; q += kB) smask[q] = src[q];
       __syncthreads();
-      if (active && (!done || final_pass)) {
+      if (eval) {
         for (int w = r; w < nw; w += LP) {
           uint32_t bits = smask[w * gc + k];
           while (bits) {
@@ -157,6 +166,13 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
             bits &= bits - 1u;
             const double d0 = px - sx[j], d1 = py - sy[j], d2 = pz - sz[j];
             const double rr2 = d0 * d0 + d1 * d1 + d2 * d2;
+            if (!final_pass && rr2 >= lo2 && rr2 <= hi2) {
+              if (nbd == 0)
+                bnd0 = rr2;
+              else if (nbd == 1)
+                bnd1 = rr2;
+              ++nbd;
+            }
             if (!(rr2 < h2)) continue;
             ++cnt;
             if (final_pass || sid[j] == (int32_t)ia) continue;
@@ -174,9 +190,12 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
     const T drho = xsum(s1.s, LP) - xsum(s1.c, LP);
     for (int o = LP >> 1; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (final_pass) {
-      nb = cnt;
+      if (need_count) nb = cnt;
       break;
     }
+    double hn = h;
+    bool fin = false, cv = false;
+    int adj = 0, ovf = 0;
     if (!done) {
       const double h3 = h * h * h;
       const double f = (double)rho * h3 - ma * a.c3;
@@ -186,19 +205,37 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
         lo = h;
       else
         hi = h;
-      double hn = fp > 0 ? h - f / fp : -1.0;
+      hn = fp > 0 ? h - f / fp : -1.0;
       if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
-      ++it;
-      const double dh = fabs(hn - h);
-      h = hn;
-      if (dh < a.tol * h) {
-        done = true;
-        conv = true;
-      } else if (it >= a.maxit) {
-        done = true;
+      cv = fabs(hn - h) < a.tol * hn;
+      fin = cv || it + 1 >= a.maxit;
+      if (fin) {
+        const double hn2 = hn * hn;
+        if (nbd > 0) adj += (bnd0 < hn2) - (bnd0 < h2);
+        if (nbd > 1) adj += (bnd1 < hn2) - (bnd1 < h2);
+        ovf = nbd > 2 || !(fabs(hn - h) <= 0.99 * band * h);
       }
     }
-    if (!__syncthreads_or(!done)) final_pass = true;
+    for (int o = LP >> 1; o > 0; o >>= 1) {
+      adj += __shfl_xor(adj, o, 64);
+      ovf |= __shfl_xor(ovf, o, 64);
+    }
+    if (!done) {
+      ++it;
+      h = hn;
+      if (fin) {
+        done = true;
+        conv = cv;
+        if (ovf)
+          need_count = true;
+        else
+          nb = cnt + adj;
+      }
+    }
+    if (!__syncthreads_or(!done)) {
+      if (!__syncthreads_or(need_count)) break;
+      final_pass = true;
+    }
   }
   if (a.flag) {
     // A root at the bracket's upper end lies beyond the gather radius; a

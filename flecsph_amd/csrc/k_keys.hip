@@ -50,11 +50,13 @@ constexpr int kNPersistent = sizeof(kPersistent) / sizeof(int);
 // ---------------------------------------------------------------------------
 // Periodic boundaries, include/physics/boundary.h
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kT) void k_keep_real(const int64_t *__restrict__ id,
-                                                  int64_t n, int64_t n_real,
-                                                  int32_t *__restrict__ keep) {
+// Real particles have ids below n_real (of their instance in batched mode).
+__global__ __launch_bounds__(kT) void k_keep_real(
+    const int64_t *__restrict__ id, int64_t n, int64_t n_real,
+    const int32_t *__restrict__ inst, const int64_t *__restrict__ nreal,
+    int32_t *__restrict__ keep) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n) keep[i] = id[i] < n_real;
+  if (i < n) keep[i] = id[i] < (inst ? nreal[inst[i]] : n_real);
 }
 
 __global__ __launch_bounds__(kT) void k_compact_index(
@@ -102,14 +104,24 @@ __device__ inline int copies_of(int on) {
   return a + bb + cc + (a & bb) + (a & cc) + (cc & bb) + (a & bb & cc);
 }
 
+// halo of the particle's instance (batched mode) or the one halo
+struct Halo {
+  double h;
+  const double *per_inst;
+  const int32_t *inst;
+  __device__ double of(int64_t i) const {
+    return inst ? per_inst[inst[i]] : h;
+  }
+};
+
 __global__ __launch_bounds__(kT) void k_count_copies(
     const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, int64_t n, Box b, double halo,
+    const double *__restrict__ z, int64_t n, Box b, Halo hl,
     int32_t *__restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {x[i], y[i], z[i]};
-  cnt[i] = copies_of(on_edge_mask(p, b, halo));
+  cnt[i] = copies_of(on_edge_mask(p, b, hl.of(i)));
 }
 
 // add_corner_XX/XXX (boundary.h:72-119): mirrored coordinate of a corner
@@ -128,18 +140,28 @@ struct GenArgs {
 };
 
 // Appends the WALL copies of particle i at n_real + off[i] + j in the order
-// of boundary.h:170-215: faces x, y, z, then corners xy, xz, zy, xyz.
+// of boundary.h:170-215: faces x, y, z, then corners xy, xz, zy, xyz.  In
+// batched mode ids count within the instance (n_real of the instance plus
+// the creation index among its copies; segr: offsets of the instances'
+// real particles).
 __global__ __launch_bounds__(kT) void k_gen_copies(
     GenArgs a, double *x, double *y, double *z, int64_t *id,
     const int64_t *__restrict__ off, int64_t n, int64_t n_real, Box b,
-    double halo) {
+    Halo hl, const int64_t *__restrict__ nreal,
+    const int64_t *__restrict__ segr) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {x[i], y[i], z[i]};
+  const double halo = hl.of(i);
   const int on = on_edge_mask(p, b, halo);
   if (!on) return;
   int64_t w = n_real + off[i];
+  int64_t id_base = 0;  // id of copy dst is dst - id_base
+  if (hl.inst) {
+    const int ii = hl.inst[i];
+    id_base = n_real + off[segr[ii]] - nreal[ii];
+  }
   double *X[3] = {x, y, z};
   auto clone = [&](int64_t dst) {
     for (int f = 0; f < a.nf; ++f) {
@@ -149,7 +171,8 @@ __global__ __launch_bounds__(kT) void k_gen_copies(
         static_cast<uint32_t *>(a.f[f])[dst] = static_cast<uint32_t *>(a.f[f])[i];
     }
     X[0][dst] = p[0]; X[1][dst] = p[1]; X[2][dst] = p[2];
-    id[dst] = dst;  // WALL ids: n_real + creation index (boundary.h:235-244)
+    // WALL ids: n_real + creation index (boundary.h:235-244)
+    id[dst] = dst - id_base;
   };
   for (int d = 0; d < 3; ++d) {
     if (!(on >> d & 1)) continue;
@@ -237,13 +260,16 @@ __global__ __launch_bounds__(kT) void k_range_final(double *__restrict__ part,
   if (threadIdx.x < 6) part[nb * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// range: lo[3], hi[3] (one simulation) or 8 doubles per instance
 __global__ __launch_bounds__(kT) void k_keys(
     const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, int64_t n, const double *__restrict__ range,
-    uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
+    const int32_t *__restrict__ inst, uint64_t *__restrict__ key,
+    uint32_t *__restrict__ val) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
+  if (inst) range += 8 * (int64_t)inst[i];
   const uint64_t max_val = (1ull << 21) - 1;
   const double p[3] = {x[i], y[i], z[i]};
   uint64_t k = 1ull << 63;
@@ -272,8 +298,16 @@ __global__ __launch_bounds__(kT) void k_keys(
 constexpr int kRsItems = 16;
 constexpr int kRsTile = kT * kRsItems;
 
+// Digit of a pass: 8 key bits at `shift`, or (inst given) the instance of
+// the particle the entry points to.
+__device__ inline uint32_t rs_digit(uint64_t k, uint32_t v, int shift,
+                                    const int32_t *__restrict__ inst) {
+  return inst ? (uint32_t)inst[v] : (uint32_t)((k >> shift) & 255u);
+}
+
 __global__ __launch_bounds__(kT) void k_radix_hist(
-    const uint64_t *__restrict__ key, int64_t n, int shift,
+    const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
+    int64_t n, int shift, const int32_t *__restrict__ inst,
     int32_t *__restrict__ hist, int64_t nblocks) {
   __shared__ int32_t hs[256];
   hs[threadIdx.x] = 0;
@@ -281,7 +315,7 @@ __global__ __launch_bounds__(kT) void k_radix_hist(
   const int64_t base = (int64_t)blockIdx.x * kRsTile;
   for (int r = 0; r < kRsItems; ++r) {
     const int64_t i = base + r * kT + threadIdx.x;
-    if (i < n) atomicAdd(&hs[(key[i] >> shift) & 255u], 1);
+    if (i < n) atomicAdd(&hs[rs_digit(key[i], val[i], shift, inst)], 1);
   }
   __syncthreads();
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = hs[threadIdx.x];
@@ -290,7 +324,8 @@ __global__ __launch_bounds__(kT) void k_radix_hist(
 __global__ __launch_bounds__(kT) void k_radix_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
-    int shift, const int64_t *__restrict__ hoff, int64_t nblocks) {
+    int shift, const int32_t *__restrict__ inst,
+    const int64_t *__restrict__ hoff, int64_t nblocks) {
   __shared__ int32_t wcnt[kT / 64][256];
   __shared__ int32_t wbase[kT / 64][256];
   __shared__ int32_t run[256];
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(
     const bool valid = i < n;
     const uint64_t k = valid ? kin[i] : 0ull;
     const uint32_t v = valid ? vin[i] : 0u;
-    const uint32_t d = valid ? (uint32_t)((k >> shift) & 255u) : 0u;
+    const uint32_t d = valid ? rs_digit(k, v, shift, inst) : 0u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -339,27 +374,38 @@ __global__ __launch_bounds__(kT) void k_radix_scatter(
   }
 }
 
-__global__ __launch_bounds__(kT) void k_check_dup(const uint64_t *__restrict__ key,
-                                                  int64_t n, int *flags) {
+__global__ __launch_bounds__(kT) void k_check_dup(
+    const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
+    const int32_t *__restrict__ inst, int64_t n, int *flags) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x + 1;
-  if (i < n && key[i] == key[i - 1]) atomicOr(flags, kFlagDupKey);
+  if (i < n && key[i] == key[i - 1] &&
+      (!inst || inst[val[i]] == inst[val[i - 1]]))
+    atomicOr(flags, kFlagDupKey);
 }
 
 }  // namespace
 
+int persistent_fields(const Ctx *c, int *out) {
+  int nf = 0;
+  for (int q = 0; q < kNPersistent; ++q) out[nf++] = kPersistent[q];
+  if (c->batch.n_inst) out[nf++] = PF_INST;
+  return nf;
+}
+
 int permute_particles(Ctx *c, const uint32_t *perm, int64_t n_out) {
   if (n_out <= 0) return 0;
+  int pf[kMaxPermFields];
+  const int nf = persistent_fields(c, pf);
   PermArgs a{};
-  a.nf = kNPersistent;
-  for (int q = 0; q < kNPersistent; ++q) {
-    const int f = kPersistent[q];
-    a.src[q] = c->f[f].p;
-    a.dst[q] = c->g[f].p;
-    a.esize[q] = (int)field_size(c, f);
+  a.nf = nf;
+  for (int q = 0; q < nf; ++q) {
+    a.src[q] = c->f[pf[q]].p;
+    a.dst[q] = c->g[pf[q]].p;
+    a.esize[q] = (int)field_size(c, pf[q]);
   }
   k_permute<<<blocks_for(n_out), kT, 0, c->stream>>>(a, perm, n_out);
   FSPH_HIP(c, hipGetLastError());
-  for (int q = 0; q < kNPersistent; ++q) std::swap(c->f[kPersistent[q]], c->g[kPersistent[q]]);
+  for (int q = 0; q < nf; ++q) std::swap(c->f[pf[q]], c->g[pf[q]]);
   return 0;
 }
 
@@ -373,7 +419,9 @@ int stage_boundary(Ctx *c) {
     FSPH_TRY(ensure(c, c->pair_off, (size_t)(c->n + 1) * sizeof(int64_t)));
     int64_t *off = c->pair_off.as<int64_t>();
     k_keep_real<<<blocks_for(c->n), kT, 0, c->stream>>>(
-        c->f[PF_ID].as<int64_t>(), c->n, c->n_real, keep);
+        c->f[PF_ID].as<int64_t>(), c->n, c->n_real,
+        c->batch.n_inst ? c->f[PF_INST].as<int32_t>() : nullptr,
+        c->batch.nreal.as<int64_t>(), keep);
     FSPH_TRY(scan_i32_to_i64(c, keep, off, c->n));
     k_compact_index<<<blocks_for(c->n), kT, 0, c->stream>>>(
         keep, off, c->n, c->perm.as<uint32_t>());
@@ -381,10 +429,26 @@ int stage_boundary(Ctx *c) {
     FSPH_TRY(permute_particles(c, c->perm.as<uint32_t>(), c->n_real));
     c->n = c->n_real;
   }
-  // getSmoothinglength (bodies_system.h:112-124): halo = 2.5 h_max.
-  double hmax = 0;
-  FSPH_TRY(stage_max_h(c, &hmax));
-  const double halo = 2.5 * hmax;
+  // getSmoothinglength (bodies_system.h:112-124): halo = 2.5 h_max (of
+  // each instance in batched mode).
+  Halo hl{0.0, nullptr, nullptr};
+  if (c->batch.n_inst) {
+    FSPH_TRY(batch_reduce(c, 0));
+    const int ni = c->batch.n_inst;
+    std::vector<double> bd((size_t)ni * 8), halo((size_t)ni);
+    FSPH_HIP(c, hipMemcpyAsync(bd.data(), c->batch.bounds.p, bd.size() * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < ni; ++i) halo[i] = 2.5 * bd[(size_t)i * 8 + 6];
+    FSPH_HIP(c, hipMemcpyAsync(c->batch.halo.p, halo.data(), halo.size() * 8,
+                               hipMemcpyHostToDevice, c->stream));
+    hl.per_inst = c->batch.halo.as<double>();
+    hl.inst = c->f[PF_INST].as<int32_t>();
+  } else {
+    double hmax = 0;
+    FSPH_TRY(stage_max_h(c, &hmax));
+    hl.h = 2.5 * hmax;
+  }
   Box b{};
   for (int d = 0; d < 3; ++d) {
     b.lo[d] = -0.5 * c->p.box[d];
@@ -398,7 +462,7 @@ int stage_boundary(Ctx *c) {
   int32_t *cnt = c->f[PF_NNGB].as<int32_t>();
   k_count_copies<<<blocks_for(n), kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), n, b, halo, cnt);
+      c->f[PF_Z].as<double>(), n, b, hl, cnt);
   FSPH_HIP(c, hipGetLastError());
   FSPH_TRY(ensure(c, c->pair_off, (size_t)(n + 1) * sizeof(int64_t)));
   int64_t *off = c->pair_off.as<int64_t>();
@@ -412,16 +476,19 @@ int stage_boundary(Ctx *c) {
     // does not touch.
     FSPH_TRY(ensure_particles(c, (int64_t)((n + extra) * 1.25) + 1024, true));
   }
+  if (hl.inst) hl.inst = c->f[PF_INST].as<int32_t>();  // may have moved
+  int pf[kMaxPermFields];
+  const int nf = persistent_fields(c, pf);
   GenArgs a{};
-  a.nf = kNPersistent;
-  for (int q = 0; q < kNPersistent; ++q) {
-    a.f[q] = c->f[kPersistent[q]].p;
-    a.esize[q] = (int)field_size(c, kPersistent[q]);
+  a.nf = nf;
+  for (int q = 0; q < nf; ++q) {
+    a.f[q] = c->f[pf[q]].p;
+    a.esize[q] = (int)field_size(c, pf[q]);
   }
   k_gen_copies<<<blocks_for(n), kT, 0, c->stream>>>(
       a, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
       c->f[PF_Z].as<double>(), c->f[PF_ID].as<int64_t>(), off, n, c->n_real,
-      b, halo);
+      b, hl, c->batch.nreal.as<int64_t>(), c->batch.segr.as<int64_t>());
   FSPH_HIP(c, hipGetLastError());
   c->n = n + extra;
   return 0;
@@ -429,6 +496,18 @@ int stage_boundary(Ctx *c) {
 
 int stage_range_keys(Ctx *c) {
   const int64_t n = c->n;
+  if (c->batch.n_inst) {
+    // per-instance range (bodies_system.h:231-268 for each instance)
+    FSPH_TRY(batch_reduce(c, 0));
+    FSPH_TRY(ensure(c, c->key, (size_t)c->pcap * 8));
+    FSPH_TRY(ensure(c, c->val, (size_t)c->pcap * 4));
+    k_keys<<<blocks_for(n), kT, 0, c->stream>>>(
+        c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+        c->f[PF_Z].as<double>(), n, c->batch.bounds.as<double>(),
+        c->f[PF_INST].as<int32_t>(), c->key.as<uint64_t>(),
+        c->val.as<uint32_t>());
+    return hip_check(c, hipGetLastError(), "k_keys");
+  }
   const int nb = (int)std::min<int64_t>(1024, (n + kT - 1) / kT);
   FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 6 * sizeof(double)));
   double *part = c->red_tmp.as<double>();
@@ -440,8 +519,8 @@ int stage_range_keys(Ctx *c) {
   FSPH_TRY(ensure(c, c->val, (size_t)c->pcap * 4));
   k_keys<<<blocks_for(n), kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), n, part + nb * 6, c->key.as<uint64_t>(),
-      c->val.as<uint32_t>());
+      c->f[PF_Z].as<double>(), n, part + nb * 6, nullptr,
+      c->key.as<uint64_t>(), c->val.as<uint32_t>());
   FSPH_HIP(c, hipGetLastError());
   double r[6];
   FSPH_HIP(c, hipMemcpyAsync(r, part + nb * 6, sizeof(r), hipMemcpyDeviceToHost,
@@ -466,20 +545,32 @@ int stage_sort(Ctx *c) {
       c->hist.as<char>() + (((size_t)256 * nb * sizeof(int32_t) + 7) & ~(size_t)7));
   uint64_t *k0 = c->key.as<uint64_t>(), *k1 = c->key2.as<uint64_t>();
   uint32_t *v0 = c->val.as<uint32_t>(), *v1 = c->val2.as<uint32_t>();
-  for (int pass = 0; pass < 8; ++pass) {
+  // 8 passes over the key bytes; in batched mode a ninth, stable pass on
+  // the instance gives (instance, key, id) order.
+  const int32_t *inst = c->batch.n_inst ? c->f[PF_INST].as<int32_t>() : nullptr;
+  const int passes = inst ? 9 : 8;
+  for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
-    k_radix_hist<<<(unsigned)nb, kT, 0, c->stream>>>(k0, n, shift, hist, nb);
+    const int32_t *pi = pass == 8 ? inst : nullptr;
+    k_radix_hist<<<(unsigned)nb, kT, 0, c->stream>>>(k0, v0, n, shift, pi,
+                                                     hist, nb);
     FSPH_TRY(scan_i32_to_i64(c, hist, hoff, 256 * nb));
     k_radix_scatter<<<(unsigned)nb, kT, 0, c->stream>>>(k0, v0, k1, v1, n,
-                                                        shift, hoff, nb);
+                                                        shift, pi, hoff, nb);
     FSPH_HIP(c, hipGetLastError());
     std::swap(k0, k1);
     std::swap(v0, v1);
   }
-  // 8 passes: the sorted data is back in key/val.
-  k_check_dup<<<blocks_for(n), kT, 0, c->stream>>>(k0, n, c->flags.as<int>());
+  if (passes & 1) {
+    // keep the sorted keys in c->key for the tree
+    std::swap(c->key, c->key2);
+    std::swap(c->val, c->val2);
+  }
+  k_check_dup<<<blocks_for(n), kT, 0, c->stream>>>(k0, v0, inst, n,
+                                                   c->flags.as<int>());
   FSPH_HIP(c, hipGetLastError());
-  return permute_particles(c, v0, n);
+  FSPH_TRY(permute_particles(c, v0, n));
+  return c->batch.n_inst ? batch_segments(c) : 0;
 }
 
 }  // namespace fsph

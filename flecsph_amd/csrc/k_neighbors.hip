@@ -13,6 +13,8 @@
 // neighbour predicate and stores it as a bitmask row per particle:
 // mask[moff[g] + w*gcount + k] bit j <=> candidate 32w+j neighbours k.
 // The SPH passes stage candidates in LDS and walk these bits.
+#include <cfloat>
+
 #include "fsph_internal.h"
 
 namespace fsph {
@@ -81,6 +83,8 @@ struct TravArgs {
   int gather;
   const float *hprop;    // gather radius base instead of h (or null)
   const int32_t *glist;  // tile slot -> group (or null: slot = group)
+  // batched mode: the walk starts at the root of the group's instance
+  const int32_t *inst, *root;
 };
 
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
@@ -102,9 +106,31 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
             : a.rscale * (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
   }
   double gb[6];
-  if (gn >= 0)
+  if (gn >= 0) {
     for (int d = 0; d < 6; ++d) gb[d] = a.box[6 * (int64_t)gn + d];
-  if (lane == 0) stk[0] = 0;  // root
+  } else {
+    // Without a group node (gather mode, single-particle groups): the box
+    // hull of the group's spheres, slightly widened.  A node that meets a
+    // sphere meets the hull, so testing it first only saves sphere tests.
+    for (int d = 0; d < 3; ++d) {
+      gb[d] = DBL_MAX;
+      gb[d + 3] = -DBL_MAX;
+    }
+    for (int k = lane; k < gc; k += kW) {
+      const double r = sh[k] * (1.0 + 1e-9);
+      const double c[3] = {sx[k], sy[k], sz[k]};
+      for (int d = 0; d < 3; ++d) {
+        gb[d] = fmin(gb[d], c[d] - r);
+        gb[d + 3] = fmax(gb[d + 3], c[d] + r);
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      for (int d = 0; d < 3; ++d) {
+        gb[d] = fmin(gb[d], __shfl_xor(gb[d], o, 64));
+        gb[d + 3] = fmax(gb[d + 3], __shfl_xor(gb[d + 3], o, 64));
+      }
+  }
+  if (lane == 0) stk[0] = a.root ? a.root[a.inst[gs]] : 0;  // root
   __syncthreads();
   int sp = 1;
   int64_t ncand = 0;
@@ -120,7 +146,7 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     if (node >= 0) {
       double bx[6];
       for (int d = 0; d < 6; ++d) bx[d] = a.box[6 * (int64_t)node + d];
-      if (gn < 0 || box_box(bx, gb)) {
+      if (box_box(bx, gb)) {
         for (int k = 0; k < gc; ++k)
           if (sphere_box(bx, sx[k], sy[k], sz[k], sh[k])) {
             acc = true;
@@ -303,7 +329,9 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                tb.ccount.as<int32_t>(), tb.ccap,
                c->flags.as<int>(),      scale,
                gather ? 1 : 0,          hprop,
-               glist};
+               glist,
+               c->batch.n_inst ? c->f[PF_INST].as<int32_t>() : nullptr,
+               c->batch.n_inst ? c->batch.root.as<int32_t>() : nullptr};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
       k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);

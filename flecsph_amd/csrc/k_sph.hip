@@ -5,12 +5,13 @@
 // particle: lane r of particle k walks mask words r, r+LP, ... of the
 // particle's row, and the LP partial results are combined with xor
 // shuffles.  Candidates and the chunk's mask words are staged in LDS in
-// chunks of kChunk candidates (SoA, coalesced loads); each lane walks its set
+// chunks of CH candidates (SoA, coalesced loads); each lane walks its set
 // bits in one flattened loop, so the loop count is the lane's own neighbour
 // count rather than a per-word maximum.  Pair arithmetic runs in T (float by
 // default, double for FSPH_PREC_FP64) with compensated (Kahan) sums;
 // position differences are taken in FP64 first.
 #include <cfloat>
+#include <cstring>
 
 #include "fsph_internal.h"
 
@@ -19,8 +20,8 @@ namespace fsph {
 namespace {
 
 constexpr int kB = 128;      // threads per workgroup
-constexpr int kChunk = 128;  // candidates staged per round
-constexpr int kChunkW = kChunk / 32;
+// Candidates staged per LDS round: a template parameter CH (64, 128 or 256)
+// of the pair kernels, chosen by the autotuner.
 
 template <class T> struct KSum {  // Kahan compensated sum
   T s = 0, c = 0;
@@ -136,9 +137,9 @@ struct TileCtx {
   bool active;
 };
 
-__device__ inline TileCtx tile_ctx(const Tiles &t) {
+__device__ inline TileCtx tile_ctx(const Tiles &t, int64_t g) {
   TileCtx c;
-  c.g = blockIdx.x;
+  c.g = g;
   c.gs = t.gstart[c.g];
   c.gc = t.gcount[c.g];
   c.cc = t.ccount[c.g];
@@ -188,12 +189,12 @@ template <class T> struct DensArgs {
   Phys ph;
 };
 
-template <class T>
-__global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
-  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
-  __shared__ T sh[kChunk], sm[kChunk];
-  __shared__ uint32_t smask[kChunkW * kB];
-  const TileCtx c = tile_ctx(a.t);
+template <class T, int CH>
+__device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
+  __shared__ double sx[CH], sy[CH], sz[CH];
+  __shared__ T sh[CH], sm[CH];
+  __shared__ uint32_t smask[(CH / 32) * kB];
+  const TileCtx c = tile_ctx(a.t, g);
   double px = 0, py = 0, pz = 0;
   T ph = 1;
   if (c.active) {
@@ -202,8 +203,8 @@ __global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
     ph = (T)a.h[i];
   }
   KSum<T> rho;
-  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
-    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+  for (int c0 = 0; c0 < c.cc; c0 += CH) {
+    const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
@@ -247,15 +248,15 @@ template <class T> struct VsigArgs {
   int kernel;
 };
 
-template <class T, bool DIV>
-__global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
-  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
-  __shared__ T svx[kChunk], svy[kChunk], svz[kChunk], scn[kChunk],
-      sco[kChunk];
-  __shared__ T sh[DIV ? kChunk : 1], sm[DIV ? kChunk : 1];
-  __shared__ int32_t sid[kChunk];
-  __shared__ uint32_t smask[kChunkW * kB];
-  const TileCtx c = tile_ctx(a.t);
+template <class T, bool DIV, int CH>
+__device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
+  __shared__ double sx[CH], sy[CH], sz[CH];
+  __shared__ T svx[CH], svy[CH], svz[CH], scn[CH],
+      sco[CH];
+  __shared__ T sh[DIV ? CH : 1], sm[DIV ? CH : 1];
+  __shared__ int32_t sid[CH];
+  __shared__ uint32_t smask[(CH / 32) * kB];
+  const TileCtx c = tile_ctx(a.t, g);
   double px = 0, py = 0, pz = 0;
   T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0, ph = 1, dv = 0;
   const int32_t ia = (int32_t)(c.gs + c.k);
@@ -265,8 +266,8 @@ __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
     pc = a.c[ia];
     if (DIV) ph = (T)a.h[ia];
   }
-  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
-    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+  for (int c0 = 0; c0 < c.cc; c0 += CH) {
+    const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int32_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
@@ -320,19 +321,21 @@ template <class T, bool DUDT> struct ForceArgs {
   T *ax, *ay, *az, *gx, *gy, *gz, *phi;  // acceleration pass
   T *dudt, *P_out, *c_out;               // du/dt pass
   double dt;
+  const double *dtv;    // batched mode: dt of the particle's instance
+  const int32_t *inst;
   Phys ph;
 };
 
-template <class T, bool DUDT>
-__global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
-  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
-  __shared__ T svhx[kChunk], svhy[kChunk], svhz[kChunk];
-  __shared__ T svx[DUDT ? kChunk : 1], svy[DUDT ? kChunk : 1],
-      svz[DUDT ? kChunk : 1];
-  __shared__ T sh[kChunk], sm[kChunk], srho[kChunk], sprho2[kChunk],
-      sc[kChunk];
-  __shared__ uint32_t smask[kChunkW * kB];
-  const TileCtx c = tile_ctx(a.t);
+template <class T, bool DUDT, int CH>
+__device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
+  __shared__ double sx[CH], sy[CH], sz[CH];
+  __shared__ T svhx[CH], svhy[CH], svhz[CH];
+  __shared__ T svx[DUDT ? CH : 1], svy[DUDT ? CH : 1],
+      svz[DUDT ? CH : 1];
+  __shared__ T sh[CH], sm[CH], srho[CH], sprho2[CH],
+      sc[CH];
+  __shared__ uint32_t smask[(CH / 32) * kB];
+  const TileCtx c = tile_ctx(a.t, g);
   double px = 0, py = 0, pz = 0;
   T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
   T ph = 1, prho = 1, pprho2 = 0, pc = 0;
@@ -350,8 +353,8 @@ __global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
     pc = a.c[i];
   }
   KSum<T> s0, s1, s2;
-  for (int c0 = 0; c0 < c.cc; c0 += kChunk) {
-    const int nch = min(kChunk, c.cc - c0), nw = (nch + 31) >> 5;
+  for (int c0 = 0; c0 < c.cc; c0 += CH) {
+    const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
@@ -414,12 +417,67 @@ __global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
     const T du = pprho2 * t0 + T(0.5) * t1;
     a.dudt[i] = du;
     // recompute_pressure_soundspeed (default_physics.h:154-161)
-    const double uh = a.u[i] + 0.5 * a.dt * (double)du;
+    const double dti = a.dtv ? a.dtv[a.inst[i]] : a.dt;
+    const double uh = a.u[i] + 0.5 * dti * (double)du;
     double P, cs;
     eos_pc(a.ph, (double)prho, uh, &P, &cs);
     a.P_out[i] = (T)P;
     a.c_out[i] = (T)cs;
   }
+}
+
+// One workgroup per group (grid = number of groups).
+template <class T, int CH>
+__global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
+  density_group<T, CH>(a, blockIdx.x);
+}
+template <class T, bool DIV, int CH>
+__global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
+  vsig_group<T, DIV, CH>(a, blockIdx.x);
+}
+template <class T, bool DUDT, int CH>
+__global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
+  force_group<T, DUDT, CH>(a, blockIdx.x);
+}
+
+// Persistent form (for graph replay): a fixed grid whose workgroups take
+// groups from a work counter until *ng are done; the arguments are read
+// from device memory, so one captured launch serves every step.
+struct Work {
+  const int64_t *ng;
+  unsigned long long *counter;  // zeroed before the launch
+};
+
+template <class F> __device__ inline void persistent_groups(const Work &w, F &&f) {
+  __shared__ int64_t s_g;
+  const int64_t ng = *w.ng;
+  for (;;) {
+    if (threadIdx.x == 0) s_g = (int64_t)atomicAdd(w.counter, 1ull);
+    __syncthreads();
+    const int64_t g = s_g;
+    __syncthreads();
+    if (g >= ng) break;
+    f(g);
+    __syncthreads();
+  }
+}
+
+template <class T, int CH>
+__global__ __launch_bounds__(kB) void k_density_p(const DensArgs<T> *pa,
+                                                  Work w) {
+  const DensArgs<T> a = *pa;
+  persistent_groups(w, [&](int64_t g) { density_group<T, CH>(a, g); });
+}
+template <class T, int CH>
+__global__ __launch_bounds__(kB) void k_vsig_p(const VsigArgs<T> *pa, Work w) {
+  const VsigArgs<T> a = *pa;
+  persistent_groups(w, [&](int64_t g) { vsig_group<T, false, CH>(a, g); });
+}
+template <class T, bool DUDT, int CH>
+__global__ __launch_bounds__(kB) void k_force_p(const ForceArgs<T, DUDT> *pa,
+                                                Work w) {
+  const ForceArgs<T, DUDT> a = *pa;
+  persistent_groups(w, [&](int64_t g) { force_group<T, DUDT, CH>(a, g); });
 }
 
 Phys phys_of(const Ctx *c) {
@@ -446,8 +504,15 @@ Tiles tiles_of(const Ctx *c) {
   return t;
 }
 
-template <class T> int density_impl(Ctx *c) {
-  std::swap(c->f[PF_C], c->f[PF_COLD]);  // keep the previous sound speed
+template <class T, bool DIV> void vsig_launch(Ctx *c, const VsigArgs<T> &v) {
+  switch (c->chunk) {
+    case 64: k_vsig<T, DIV, 64><<<(unsigned)c->ng, kB, 0, c->stream>>>(v); break;
+    case 256: k_vsig<T, DIV, 256><<<(unsigned)c->ng, kB, 0, c->stream>>>(v); break;
+    default: k_vsig<T, DIV, 128><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
+  }
+}
+
+template <class T> DensArgs<T> dens_args(Ctx *c) {
   DensArgs<T> a;
   a.t = tiles_of(c);
   a.x = c->f[PF_X].as<double>(); a.y = c->f[PF_Y].as<double>();
@@ -456,33 +521,26 @@ template <class T> int density_impl(Ctx *c) {
   a.rho = c->f[PF_RHO].as<T>(); a.P = c->f[PF_P].as<T>();
   a.c = c->f[PF_C].as<T>();
   a.ph = phys_of(c);
-  {
-    PhaseTimer tm(c, FSPH_T_DENSITY);
-    k_density<T><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
-  }
-  FSPH_HIP(c, hipGetLastError());
+  return a;
+}
+
+template <class T> VsigArgs<T> vsig_args(Ctx *c) {
   VsigArgs<T> v;
-  v.t = a.t;
-  v.x = a.x; v.y = a.y; v.z = a.z;
+  v.t = tiles_of(c);
+  v.x = c->f[PF_X].as<double>(); v.y = c->f[PF_Y].as<double>();
+  v.z = c->f[PF_Z].as<double>();
   v.vx = c->f[PF_VX].as<double>(); v.vy = c->f[PF_VY].as<double>();
   v.vz = c->f[PF_VZ].as<double>();
   v.c = c->f[PF_C].as<T>(); v.c_old = c->f[PF_COLD].as<T>();
   v.vsig = c->f[PF_VSIG].as<T>();
-  v.h = a.h; v.m = a.m;
-  v.rho = a.rho;
+  v.h = c->f[PF_H].as<double>(); v.m = c->f[PF_M].as<double>();
+  v.rho = c->f[PF_RHO].as<T>();
   v.divv = c->f[PF_DIVV].as<T>();
   v.kernel = c->p.kernel;
-  {
-    PhaseTimer tm(c, FSPH_T_VSIG);
-    if (c->p.h_mode == FSPH_H_LEARNED)
-      k_vsig<T, true><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
-    else
-      k_vsig<T, false><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
-  }
-  return hip_check(c, hipGetLastError(), "k_vsig");
+  return v;
 }
 
-template <class T, bool DUDT> int force_impl(Ctx *c) {
+template <class T, bool DUDT> ForceArgs<T, DUDT> force_args(Ctx *c) {
   ForceArgs<T, DUDT> a{};
   a.t = tiles_of(c);
   a.x = c->f[PF_X].as<double>(); a.y = c->f[PF_Y].as<double>();
@@ -505,10 +563,47 @@ template <class T, bool DUDT> int force_impl(Ctx *c) {
   a.P_out = c->g[PF_P].as<T>();
   a.c_out = c->g[PF_C].as<T>();
   a.dt = c->dt;
+  if (c->batch.n_inst) {
+    a.dtv = c->batch.dt.as<double>();
+    a.inst = c->f[PF_INST].as<int32_t>();
+  }
   a.ph = phys_of(c);
+  return a;
+}
+
+template <class T> int density_impl(Ctx *c) {
+  std::swap(c->f[PF_C], c->f[PF_COLD]);  // keep the previous sound speed
+  const DensArgs<T> a = dens_args<T>(c);
+  {
+    PhaseTimer tm(c, FSPH_T_DENSITY);
+    switch (c->chunk) {
+      case 64: k_density<T, 64><<<(unsigned)c->ng, kB, 0, c->stream>>>(a); break;
+      case 256: k_density<T, 256><<<(unsigned)c->ng, kB, 0, c->stream>>>(a); break;
+      default: k_density<T, 128><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
+    }
+  }
+  FSPH_HIP(c, hipGetLastError());
+  const VsigArgs<T> v = vsig_args<T>(c);
+  {
+    PhaseTimer tm(c, FSPH_T_VSIG);
+    if (c->p.h_mode == FSPH_H_LEARNED)
+      vsig_launch<T, true>(c, v);
+    else
+      vsig_launch<T, false>(c, v);
+  }
+  return hip_check(c, hipGetLastError(), "k_vsig");
+}
+
+template <class T, bool DUDT> int force_impl(Ctx *c) {
+  const ForceArgs<T, DUDT> a = force_args<T, DUDT>(c);
   {
     PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
-    k_force<T, DUDT><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
+    const unsigned ng = (unsigned)c->ng;
+    switch (c->chunk) {
+      case 64: k_force<T, DUDT, 64><<<ng, kB, 0, c->stream>>>(a); break;
+      case 256: k_force<T, DUDT, 256><<<ng, kB, 0, c->stream>>>(a); break;
+      default: k_force<T, DUDT, 128><<<ng, kB, 0, c->stream>>>(a);
+    }
   }
   FSPH_HIP(c, hipGetLastError());
   if (DUDT) {
@@ -518,7 +613,80 @@ template <class T, bool DUDT> int force_impl(Ctx *c) {
   return 0;
 }
 
+// ---- graph mode -------------------------------------------------------------
+// Host side of the four pair passes of one step, in the eager order (with
+// the same buffer rotations), written into the host staging of the graph's
+// argument slots.
+template <class T> void sph_graph_args_t(Ctx *c, char *slots) {
+  static_assert(sizeof(DensArgs<T>) <= kGraphSlot, "slot");
+  static_assert(sizeof(ForceArgs<T, true>) <= kGraphSlot, "slot");
+  std::swap(c->f[PF_C], c->f[PF_COLD]);
+  const DensArgs<T> d = dens_args<T>(c);
+  const VsigArgs<T> v = vsig_args<T>(c);
+  const ForceArgs<T, false> fa = force_args<T, false>(c);
+  const ForceArgs<T, true> fd = force_args<T, true>(c);
+  std::memcpy(slots + kSlotDens * kGraphSlot, &d, sizeof(d));
+  std::memcpy(slots + kSlotVsig * kGraphSlot, &v, sizeof(v));
+  std::memcpy(slots + kSlotAcc * kGraphSlot, &fa, sizeof(fa));
+  std::memcpy(slots + kSlotDudt * kGraphSlot, &fd, sizeof(fd));
+  std::swap(c->f[PF_P], c->g[PF_P]);
+  std::swap(c->f[PF_C], c->g[PF_C]);
+}
+
+template <class T, int CH> void sph_graph_launch_t(Ctx *c, int pass) {
+  const char *slots = c->graph.slots.as<char>();
+  unsigned long long *ctr = c->graph.counters.as<unsigned long long>();
+  const int64_t *ng = c->graph.sizes.as<int64_t>();
+  const unsigned grid = (unsigned)c->graph.grid_groups;
+  switch (pass) {
+    case kSlotDens:
+      k_density_p<T, CH><<<grid, kB, 0, c->stream>>>(
+          reinterpret_cast<const DensArgs<T> *>(slots + kSlotDens * kGraphSlot),
+          Work{ng, ctr + 0});
+      break;
+    case kSlotVsig:
+      k_vsig_p<T, CH><<<grid, kB, 0, c->stream>>>(
+          reinterpret_cast<const VsigArgs<T> *>(slots + kSlotVsig * kGraphSlot),
+          Work{ng, ctr + 1});
+      break;
+    case kSlotAcc:
+      k_force_p<T, false, CH><<<grid, kB, 0, c->stream>>>(
+          reinterpret_cast<const ForceArgs<T, false> *>(slots +
+                                                        kSlotAcc * kGraphSlot),
+          Work{ng, ctr + 2});
+      break;
+    default:
+      k_force_p<T, true, CH><<<grid, kB, 0, c->stream>>>(
+          reinterpret_cast<const ForceArgs<T, true> *>(slots +
+                                                       kSlotDudt * kGraphSlot),
+          Work{ng, ctr + 3});
+  }
+}
+
+template <class T> void sph_graph_launch_c(Ctx *c, int pass) {
+  switch (c->chunk) {
+    case 64: sph_graph_launch_t<T, 64>(c, pass); break;
+    case 256: sph_graph_launch_t<T, 256>(c, pass); break;
+    default: sph_graph_launch_t<T, 128>(c, pass);
+  }
+}
+
 }  // namespace
+
+void sph_graph_args(Ctx *c, char *slots) {
+  if (c->rsize == 4)
+    sph_graph_args_t<float>(c, slots);
+  else
+    sph_graph_args_t<double>(c, slots);
+}
+
+int sph_graph_launch(Ctx *c, int pass) {
+  if (c->rsize == 4)
+    sph_graph_launch_c<float>(c, pass);
+  else
+    sph_graph_launch_c<double>(c, pass);
+  return hip_check(c, hipGetLastError(), "pair pass (persistent)");
+}
 
 int stage_density(Ctx *c) {
   return c->rsize == 4 ? density_impl<float>(c) : density_impl<double>(c);

@@ -5,6 +5,7 @@
 // (analysis.h:57-219).  FP64 throughout, no FMA contraction, so these match
 // the reference's arithmetic operation by operation.
 #include <cfloat>
+#include <cstring>
 
 #include "fsph_internal.h"
 
@@ -21,11 +22,9 @@ template <class T> struct IntArgs {
 };
 
 template <class T>
-__global__ __launch_bounds__(kT) void k_integrate(IntArgs<T> a, int64_t n,
-                                                  uint32_t what, double dt) {
+__device__ inline void integrate_one(const IntArgs<T> &a, int64_t i,
+                                     uint32_t what, double dt) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
   if (what & FSPH_KICK_U) a.u[i] = a.u[i] + 0.5 * dt * (double)a.dudt[i];
   double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
   if (what & FSPH_KICK_V) {
@@ -43,6 +42,17 @@ __global__ __launch_bounds__(kT) void k_integrate(IntArgs<T> a, int64_t n,
     a.y[i] = a.y[i] + vy * dt;
     a.z[i] = a.z[i] + vz * dt;
   }
+}
+
+// dtv/inst (batched mode): the time step of the particle's instance
+template <class T>
+__global__ __launch_bounds__(kT) void k_integrate(IntArgs<T> a, int64_t n,
+                                                  uint32_t what, double dt,
+                                                  const double *dtv,
+                                                  const int32_t *inst) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  integrate_one(a, i, what, dtv ? dtv[inst[i]] : dt);
 }
 
 template <class T>
@@ -64,35 +74,39 @@ template <class T> struct DtArgs {
 
 __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 
+// compute_dt for one particle (default_physics.h:603-695)
+template <class T>
+__device__ inline double particle_dt(const DtArgs<T> &a, int64_t i) {
+#pragma clang fp contract(off)
+  const double tiny = 1e-24, mc = 0.6;
+  const double dx = a.h[i] / a.eta_kw;
+  double sv = 0;
+  sv += a.vx[i] * a.vx[i];
+  sv += a.vy[i] * a.vy[i];
+  sv += a.vz[i] * a.vz[i];
+  const double vn = sqrt(sv);
+  const double dt_v = dx / (vn + tiny);
+  const double a0 = (double)a.ax[i] + (double)a.gx[i];
+  const double a1 = (double)a.ay[i] + (double)a.gy[i];
+  const double a2 = (double)a.az[i] + (double)a.gz[i];
+  double sa = 0;
+  sa += a0 * a0;
+  sa += a1 * a1;
+  sa += a2 * a2;
+  const double acc = sqrt(sa);
+  const double dt_a = sqrt(dx / (acc + tiny));
+  const double dt_c = dx / (tiny + (double)a.vsig[i] * (1 + mc * a.alpha));
+  return a.cfl * smin(smin(dt_v, dt_a), dt_c);
+}
+
 template <class T>
 __global__ __launch_bounds__(kT) void k_dt_partial(DtArgs<T> a, int64_t n,
                                                    double *part) {
-#pragma clang fp contract(off)
   __shared__ double s[kT];
-  const double tiny = 1e-24, mc = 0.6;
   double dmin = 1e24;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kT) {
-    const double dx = a.h[i] / a.eta_kw;
-    double sv = 0;
-    sv += a.vx[i] * a.vx[i];
-    sv += a.vy[i] * a.vy[i];
-    sv += a.vz[i] * a.vz[i];
-    const double vn = sqrt(sv);
-    const double dt_v = dx / (vn + tiny);
-    const double a0 = (double)a.ax[i] + (double)a.gx[i];
-    const double a1 = (double)a.ay[i] + (double)a.gy[i];
-    const double a2 = (double)a.az[i] + (double)a.gz[i];
-    double sa = 0;
-    sa += a0 * a0;
-    sa += a1 * a1;
-    sa += a2 * a2;
-    const double acc = sqrt(sa);
-    const double dt_a = sqrt(dx / (acc + tiny));
-    const double dt_c = dx / (tiny + (double)a.vsig[i] * (1 + mc * a.alpha));
-    const double dti = a.cfl * smin(smin(dt_v, dt_a), dt_c);
-    dmin = smin(dmin, dti);
-  }
+       i += (int64_t)gridDim.x * kT)
+    dmin = smin(dmin, particle_dt(a, i));
   s[threadIdx.x] = dmin;
   __syncthreads();
   for (int st = kT / 2; st > 0; st >>= 1) {
@@ -198,6 +212,195 @@ __global__ __launch_bounds__(kT) void k_to_double(const T *src, double *dst,
   if (i < n) dst[i] = (double)src[i];
 }
 
+// ---- batched mode: per-instance reductions -------------------------------
+// Min/max of doubles by compare-and-swap on the bit pattern (vector memory
+// atomics); exact, so the order of the updates does not matter.
+__device__ inline void atomic_min_f64(double *a, double v) {
+  unsigned long long *p = reinterpret_cast<unsigned long long *>(a);
+  unsigned long long old = *p;
+  while (v < __longlong_as_double((long long)old)) {
+    const unsigned long long prev =
+        atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+__device__ inline void atomic_max_f64(double *a, double v) {
+  unsigned long long *p = reinterpret_cast<unsigned long long *>(a);
+  unsigned long long old = *p;
+  while (v > __longlong_as_double((long long)old)) {
+    const unsigned long long prev =
+        atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// Reduces NV values per particle into out[8 * inst + off + q] (min for the
+// first NMIN, max for the rest).  Lanes are served one instance at a time:
+// a wavefront inside one instance (the common case) does one reduction and
+// one atomic per value.
+template <int NV, int NMIN>
+__device__ inline void seg_reduce(bool valid, int inst, const double (&v)[NV],
+                                  double *out, int off) {
+  const int lane = threadIdx.x & 63;
+  uint64_t todo = __ballot(valid);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int li = __shfl(inst, leader, 64);
+    const bool mine = valid && inst == li && ((todo >> lane) & 1ull);
+    double r[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      r[q] = mine ? v[q] : (q < NMIN ? DBL_MAX : -DBL_MAX);
+      for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(r[q], o, 64);
+        r[q] = q < NMIN ? (w < r[q] ? w : r[q]) : (w > r[q] ? w : r[q]);
+      }
+    }
+    if (lane == leader)
+      for (int q = 0; q < NV; ++q) {
+        if (q < NMIN)
+          atomic_min_f64(out + 8 * (int64_t)li + off + q, r[q]);
+        else
+          atomic_max_f64(out + 8 * (int64_t)li + off + q, r[q]);
+      }
+    todo &= ~__ballot(mine);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_seg_init(double *out, int ni,
+                                                 int what) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= ni) return;
+  if (what == 0) {
+    for (int d = 0; d < 3; ++d) {
+      out[8 * i + d] = DBL_MAX;
+      out[8 * i + 3 + d] = -DBL_MAX;
+    }
+    out[8 * i + 6] = 0.0;
+  } else {
+    out[8 * i + 7] = 1e24;
+  }
+}
+
+// range of x +- h (bodies_system.h:231-268) and max h per instance
+__global__ __launch_bounds__(kT) void k_seg_range(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ h,
+    const int32_t *__restrict__ inst, int64_t n, double *out) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const bool valid = i < n;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  int ii = 0;
+  if (valid) {
+    const double hh = h[i];
+    v[0] = x[i] - hh; v[1] = y[i] - hh; v[2] = z[i] - hh;
+    v[3] = x[i] + hh; v[4] = y[i] + hh; v[5] = z[i] + hh;
+    v[6] = hh;
+    ii = inst[i];
+  }
+  // min of 3 lower ends, max of 3 upper ends and of h
+  double vmin[3] = {v[0], v[1], v[2]};
+  double vmax[4] = {v[3], v[4], v[5], v[6]};
+  seg_reduce<3, 3>(valid, ii, vmin, out, 0);
+  seg_reduce<4, 0>(valid, ii, vmax, out, 3);
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_seg_dt(DtArgs<T> a,
+                                               const int32_t *__restrict__ inst,
+                                               int64_t n, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const bool valid = i < n;
+  double v[1] = {1e24};
+  int ii = 0;
+  if (valid) {
+    v[0] = particle_dt(a, i);
+    ii = inst[i];
+  }
+  seg_reduce<1, 1>(valid, ii, v, out, 7);
+}
+
+__global__ __launch_bounds__(kT) void k_seg_bounds(
+    const int32_t *__restrict__ inst, int64_t n, int ni, int64_t *seg) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  if (i == 0 || inst[i] != inst[i - 1]) seg[inst[i]] = i;
+  if (i == n - 1) seg[ni] = n;
+}
+
+__global__ __launch_bounds__(kT) void k_seg_roots(
+    const int64_t *__restrict__ seg, const int64_t *__restrict__ off,
+    int ni, int32_t *__restrict__ root) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i < ni) root[i] = (int32_t)off[seg[i]];  // the depth-0 node of pair seg[i]
+}
+
+// ---- graph mode: persistent (grid-stride) forms reading device arguments --
+template <class T> struct StepSlot {
+  IntArgs<T> ia;
+  DtArgs<T> da;
+  double *h;
+  const double *m;
+  const T *rho;
+  double eta, kw;
+  const int32_t *inst;
+  const double *dtv;
+  double *bounds;
+};
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_integrate_p(const StepSlot<T> *ps,
+                                                    const int64_t *pn,
+                                                    uint32_t what) {
+  const StepSlot<T> &s = *ps;
+  const int64_t n = pn[1];
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT)
+    integrate_one(s.ia, i, what, s.dtv[s.inst[i]]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_update_h_p(const StepSlot<T> *ps,
+                                                   const int64_t *pn) {
+#pragma clang fp contract(off)
+  const StepSlot<T> &s = *ps;
+  const int64_t n = pn[1];
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT)
+    s.h[i] = cbrt(s.m[i] / (double)s.rho[i]) * s.eta * s.kw;
+}
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_seg_dt_p(const StepSlot<T> *ps,
+                                                 const int64_t *pn) {
+  const StepSlot<T> &s = *ps;
+  const int64_t n = pn[1];
+  for (int64_t b = (int64_t)blockIdx.x * kT; b < n;
+       b += (int64_t)gridDim.x * kT) {
+    const int64_t i = b + threadIdx.x;
+    const bool valid = i < n;
+    double v[1] = {1e24};
+    int ii = 0;
+    if (valid) {
+      v[0] = particle_dt(s.da, i);
+      ii = s.inst[i];
+    }
+    seg_reduce<1, 1>(valid, ii, v, s.bounds, 7);
+  }
+}
+
+template <class T> DtArgs<T> dt_args(Ctx *c) {
+  return DtArgs<T>{c->f[PF_H].as<double>(), c->f[PF_VX].as<double>(),
+                   c->f[PF_VY].as<double>(), c->f[PF_VZ].as<double>(),
+                   c->f[PF_AX].as<T>(), c->f[PF_AY].as<T>(), c->f[PF_AZ].as<T>(),
+                   c->f[PF_GX].as<T>(), c->f[PF_GY].as<T>(), c->f[PF_GZ].as<T>(),
+                   c->f[PF_VSIG].as<T>(), c->p.eta * 2.0, c->p.cfl,
+                   c->p.visc_alpha};
+}
+
 template <class T> int integrate_impl(Ctx *c, uint32_t what) {
   IntArgs<T> a{c->f[PF_X].as<double>(),   c->f[PF_Y].as<double>(),
                c->f[PF_Z].as<double>(),   c->f[PF_VX].as<double>(),
@@ -208,7 +411,10 @@ template <class T> int integrate_impl(Ctx *c, uint32_t what) {
                c->f[PF_AZ].as<T>(),       c->f[PF_GX].as<T>(),
                c->f[PF_GY].as<T>(),       c->f[PF_GZ].as<T>(),
                c->f[PF_DUDT].as<T>()};
-  k_integrate<T><<<blocks_for(c->n), kT, 0, c->stream>>>(a, c->n, what, c->dt);
+  const bool b = c->batch.n_inst > 0;
+  k_integrate<T><<<blocks_for(c->n), kT, 0, c->stream>>>(
+      a, c->n, what, c->dt, b ? c->batch.dt.as<double>() : nullptr,
+      b ? c->f[PF_INST].as<int32_t>() : nullptr);
   return hip_check(c, hipGetLastError(), "k_integrate");
 }
 
@@ -268,6 +474,107 @@ int stage_update_h(Ctx *c) {
         c->f[PF_H].as<double>(), c->f[PF_M].as<double>(),
         c->f[PF_RHO].as<double>(), c->n, c->p.eta, kw);
   return hip_check(c, hipGetLastError(), "k_update_h");
+}
+
+template <class T> void step_graph_args_t(Ctx *c, char *slot) {
+  static_assert(sizeof(StepSlot<T>) <= kGraphSlot, "slot");
+  StepSlot<T> s{};
+  s.ia = IntArgs<T>{c->f[PF_X].as<double>(),   c->f[PF_Y].as<double>(),
+                    c->f[PF_Z].as<double>(),   c->f[PF_VX].as<double>(),
+                    c->f[PF_VY].as<double>(),  c->f[PF_VZ].as<double>(),
+                    c->f[PF_VHX].as<double>(), c->f[PF_VHY].as<double>(),
+                    c->f[PF_VHZ].as<double>(), c->f[PF_U].as<double>(),
+                    c->f[PF_AX].as<T>(),       c->f[PF_AY].as<T>(),
+                    c->f[PF_AZ].as<T>(),       c->f[PF_GX].as<T>(),
+                    c->f[PF_GY].as<T>(),       c->f[PF_GZ].as<T>(),
+                    c->f[PF_DUDT].as<T>()};
+  s.da = dt_args<T>(c);
+  s.h = c->f[PF_H].as<double>();
+  s.m = c->f[PF_M].as<double>();
+  s.rho = c->f[PF_RHO].as<T>();
+  s.eta = c->p.eta;
+  s.kw = 2.0;  // kernels.h:753-756
+  s.inst = c->f[PF_INST].as<int32_t>();
+  s.dtv = c->batch.dt.as<double>();
+  s.bounds = c->batch.bounds.as<double>();
+  std::memcpy(slot, &s, sizeof(s));
+}
+
+template <class T> void step_graph_launch_t(Ctx *c, int what) {
+  const StepSlot<T> *ps = reinterpret_cast<const StepSlot<T> *>(
+      c->graph.slots.as<char>() + kSlotStep * kGraphSlot);
+  const int64_t *pn = c->graph.sizes.as<int64_t>();
+  const unsigned grid = (unsigned)c->graph.grid_elems;
+  switch (what) {
+    case 0: k_integrate_p<T><<<grid, kT, 0, c->stream>>>(ps, pn, FSPH_KICK_V); break;
+    case 1: k_integrate_p<T><<<grid, kT, 0, c->stream>>>(ps, pn, FSPH_KICK_U); break;
+    case 2: k_update_h_p<T><<<grid, kT, 0, c->stream>>>(ps, pn); break;
+    default: {
+      const int ni = c->batch.n_inst;
+      k_seg_init<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(
+          c->batch.bounds.as<double>(), ni, 1);
+      k_seg_dt_p<T><<<grid, kT, 0, c->stream>>>(ps, pn);
+    }
+  }
+}
+
+void step_graph_args(Ctx *c, char *slots) {
+  char *slot = slots + kSlotStep * kGraphSlot;
+  if (c->rsize == 4)
+    step_graph_args_t<float>(c, slot);
+  else
+    step_graph_args_t<double>(c, slot);
+}
+
+int step_graph_launch(Ctx *c, int what) {
+  if (c->rsize == 4)
+    step_graph_launch_t<float>(c, what);
+  else
+    step_graph_launch_t<double>(c, what);
+  return hip_check(c, hipGetLastError(), "step kernels (persistent)");
+}
+
+int batch_reduce(Ctx *c, int what) {
+  const int ni = c->batch.n_inst;
+  double *out = c->batch.bounds.as<double>();
+  k_seg_init<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(out, ni, what);
+  const int32_t *inst = c->f[PF_INST].as<int32_t>();
+  if (what == 0) {
+    k_seg_range<<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+        c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(), inst, c->n, out);
+  } else if (c->rsize == 4) {
+    k_seg_dt<float><<<blocks_for(c->n), kT, 0, c->stream>>>(dt_args<float>(c),
+                                                            inst, c->n, out);
+  } else {
+    k_seg_dt<double><<<blocks_for(c->n), kT, 0, c->stream>>>(
+        dt_args<double>(c), inst, c->n, out);
+  }
+  return hip_check(c, hipGetLastError(), "batch_reduce");
+}
+
+int batch_segments(Ctx *c) {
+  const int ni = c->batch.n_inst;
+  k_seg_bounds<<<blocks_for(c->n), kT, 0, c->stream>>>(
+      c->f[PF_INST].as<int32_t>(), c->n, ni, c->batch.seg.as<int64_t>());
+  FSPH_HIP(c, hipGetLastError());
+  c->batch.seg_h.resize(ni + 1);
+  FSPH_HIP(c, hipMemcpyAsync(c->batch.seg_h.data(), c->batch.seg.p,
+                             (size_t)(ni + 1) * 8, hipMemcpyDeviceToHost,
+                             c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ni; ++i)
+    if (c->batch.seg_h[i + 1] - c->batch.seg_h[i] < 2)
+      return fail(c, FSPH_E_ARG, "instance %d holds fewer than 2 particles", i);
+  return 0;
+}
+
+int batch_roots(Ctx *c) {
+  const int ni = c->batch.n_inst;
+  k_seg_roots<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(
+      c->batch.seg.as<int64_t>(), c->pair_off.as<int64_t>(), ni,
+      c->batch.root.as<int32_t>());
+  return hip_check(c, hipGetLastError(), "k_seg_roots");
 }
 
 int stage_dt_min(Ctx *c, double *dtmin) {

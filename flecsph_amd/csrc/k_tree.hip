@@ -29,10 +29,23 @@ __device__ inline int32_t Lm1(const int32_t *L, int64_t i) {
   return i > 0 ? L[i - 1] : -1;
 }
 
-// First sorted index whose key is >= v.
-__device__ inline int64_t lower_bound(const uint64_t *__restrict__ k, int64_t n,
-                                      uint64_t v) {
-  int64_t lo = 0, hi = n;
+// Sorted index range of the simulation that particle i belongs to: all of
+// [0, n), or its instance's segment in batched mode (the keys of each
+// instance are sorted; the tree is built per instance).
+struct Seg {
+  const int32_t *inst;
+  const int64_t *seg;
+  int64_t n;
+  __device__ int64_t lo(int64_t i) const { return inst ? seg[inst[i]] : 0; }
+  __device__ int64_t hi(int64_t i) const {
+    return inst ? seg[inst[i] + 1] : n;
+  }
+};
+
+// First sorted index in [lo, n) whose key is >= v.
+__device__ inline int64_t lower_bound(const uint64_t *__restrict__ k,
+                                      int64_t lo, int64_t n, uint64_t v) {
+  int64_t hi = n;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
     if (k[mid] < v)
@@ -52,20 +65,24 @@ __device__ inline int64_t node_id(const int64_t *__restrict__ off,
 
 // Node index of the depth-D node containing sorted particle i.
 __device__ inline int64_t containing_node(const uint64_t *__restrict__ k,
-                                          int64_t n,
+                                          const Seg &sg,
                                           const int64_t *__restrict__ off,
                                           const int32_t *__restrict__ L,
                                           int64_t i, int D) {
   const int sh = 3 * (kMaxDepth - D);
   const uint64_t first = (D == 0) ? (1ull << 63) : ((k[i] >> sh) << sh);
-  return node_id(off, L, lower_bound(k, n, first), D);
+  return node_id(off, L, lower_bound(k, sg.lo(i), sg.hi(i), first), D);
 }
 
+// Common-prefix depth of sorted keys i and i+1; -1 at the end of the array
+// (or of an instance, which makes every instance a tree of its own).
 __global__ __launch_bounds__(kT) void k_prefix(const uint64_t *__restrict__ k,
-                                               int64_t n, int32_t *__restrict__ L) {
+                                               int64_t n,
+                                               const int32_t *__restrict__ inst,
+                                               int32_t *__restrict__ L) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
-  if (i == n - 1) {
+  if (i == n - 1 || (inst && inst[i] != inst[i + 1])) {
     L[i] = -1;
     return;
   }
@@ -90,12 +107,12 @@ struct NodeOut {
 };
 
 __global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
-                                              int64_t n,
+                                              Seg sg,
                                               const int32_t *__restrict__ L,
                                               const int64_t *__restrict__ off,
                                               NodeOut o) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
+  if (i >= sg.n) return;
   const int d0 = Lm1(L, i) + 1, d1 = L[i];
   for (int d = d0; d <= d1; ++d) {
     const int64_t id = off[i] + (d - d0);
@@ -107,7 +124,7 @@ __global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
     // last index with the same prefix
     const uint64_t last_key =
         (d == 0) ? ~0ull : ((prefix << sh) | ((1ull << sh) - 1ull));
-    int64_t lo = i, hi = n;  // first index with key > last_key
+    int64_t lo = i, hi = sg.hi(i);  // first index with key > last_key
     while (lo < hi) {
       const int64_t mid = (lo + hi) >> 1;
       if (k[mid] <= last_key)
@@ -118,7 +135,7 @@ __global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
     o.count[id] = (int32_t)(lo - i);
     int64_t par = -1;
     if (d > 0) {
-      par = (d - 1 >= d0) ? id - 1 : containing_node(k, n, off, L, i, d - 1);
+      par = (d - 1 >= d0) ? id - 1 : containing_node(k, sg, off, L, i, d - 1);
       o.child[par * 8 + digit_at(k[i], d)] = (int32_t)id;
     }
     o.parent[id] = (int32_t)par;
@@ -126,17 +143,17 @@ __global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
 }
 
 __global__ __launch_bounds__(kT) void k_leaves(const uint64_t *__restrict__ k,
-                                               int64_t n,
+                                               Seg sg,
                                                const int32_t *__restrict__ L,
                                                const int64_t *__restrict__ off,
                                                int32_t *__restrict__ child,
                                                int32_t *__restrict__ leaf_parent) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
+  if (i >= sg.n) return;
   const int a = Lm1(L, i), b = L[i];
   const int D = a > b ? a : b;
   const int64_t par = (b > a) ? off[i] + b - (a + 1)
-                              : containing_node(k, n, off, L, i, D);
+                              : containing_node(k, sg, off, L, i, D);
   child[par * 8 + digit_at(k[i], D + 1)] = leaf_code(i);
   leaf_parent[i] = (int32_t)par;
 }
@@ -315,7 +332,9 @@ int stage_tree(Ctx *c) {
   FSPH_TRY(ensure(c, c->pair_off, (size_t)(c->pcap + 1) * 8));
   FSPH_TRY(ensure(c, c->leaf_parent, (size_t)c->pcap * 4));
   int32_t *L = c->L.as<int32_t>();
-  k_prefix<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L);
+  const Seg sg{c->batch.n_inst ? c->f[PF_INST].as<int32_t>() : nullptr,
+               c->batch.seg.as<int64_t>(), n};
+  k_prefix<<<blocks_for(n), kT, 0, c->stream>>>(k, n, sg.inst, L);
   k_pair_count<<<blocks_for(n), kT, 0, c->stream>>>(L, n,
                                                     c->pair_cnt.as<int32_t>());
   FSPH_HIP(c, hipGetLastError());
@@ -345,8 +364,8 @@ int stage_tree(Ctx *c) {
   NodeOut o{c->nkey.as<uint64_t>(), c->ndepth.as<int32_t>(),
             c->nstart.as<int32_t>(), c->ncount.as<int32_t>(),
             c->nparent.as<int32_t>(), c->nchild.as<int32_t>()};
-  k_nodes<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L, off, o);
-  k_leaves<<<blocks_for(n), kT, 0, c->stream>>>(k, n, L, off,
+  k_nodes<<<blocks_for(n), kT, 0, c->stream>>>(k, sg, L, off, o);
+  k_leaves<<<blocks_for(n), kT, 0, c->stream>>>(k, sg, L, off,
                                                 c->nchild.as<int32_t>(),
                                                 c->leaf_parent.as<int32_t>());
   FSPH_HIP(c, hipGetLastError());
@@ -387,7 +406,8 @@ int stage_tree(Ctx *c) {
     k_cofm<<<blocks_for(cntd), kT, 0, c->stream>>>(
         a, c->nlevel.as<int32_t>() + c->level_off[d], cntd);
   }
-  return hip_check(c, hipGetLastError(), "k_cofm");
+  FSPH_HIP(c, hipGetLastError());
+  return c->batch.n_inst ? batch_roots(c) : 0;
 }
 
 int stage_groups(Ctx *c) {

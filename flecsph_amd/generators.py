@@ -106,6 +106,21 @@ def sedov(n_side=64, seed=SEED_BASE):
     return _pack(x, y, z, zero, zero, zero, u, h, m, rho, P, c), params
 
 
+def sedov_batch(n_inst=64, lo=8_000, hi=200_000, seed=SEED_BASE + 7):
+    """Batched workload: n_inst Sedov blasts whose particle counts are drawn
+    log-uniform in [lo, hi] (seeded) and rounded to the nearest cube n_side^3.
+    Returns (list of particle sets, common parameters)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n_inst))
+    sides = np.clip(np.rint(np.cbrt(counts)).astype(int),
+                    int(np.ceil(np.cbrt(lo))), int(np.floor(np.cbrt(hi))))
+    sets, prm = [], None
+    for ns in sides:
+        parts, prm = sedov(n_side=int(ns))
+        sets.append(parts)
+    return sets, prm
+
+
 def _polytrope_mass_inverse(q):
     """r with M(r) = q for M(r) = [sin(pi r) - pi r cos(pi r)] / pi."""
     lo = np.zeros_like(q)

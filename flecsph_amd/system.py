@@ -272,3 +272,83 @@ class BodySystem(_System):
                    C.byref(loss))
         return {"fallback": fb.value, "clamped": cl.value,
                 "trained_steps": ts.value, "last_loss": loss.value}
+
+
+class BatchSystem(BodySystem):
+    """n_inst independent simulations of one parameter set in one context
+    (fsph_create_batch): one launch per kernel per step for all of them."""
+
+    def __init__(self, params, n_inst, device=0):
+        from . import _lib
+        lib = _lib.load()
+        fns = _abi.bind(lib, "fsph_")
+        h = C.c_void_p()
+        lib.fsph_create_batch.restype = C.c_int
+        lib.fsph_create_batch.argtypes = [C.POINTER(_abi.FsphParams), C.c_int,
+                                          C.c_int, C.POINTER(C.c_void_p)]
+        rc = lib.fsph_create_batch(C.byref(params), int(device), int(n_inst),
+                                   C.byref(h))
+        if rc != 0:
+            raise FsphError("fsph_create_batch", rc)
+        lib.fsph_last_error.restype = C.c_char_p
+        lib.fsph_last_error.argtypes = [C.c_void_p]
+        _System.__init__(self, fns, h, params, lib.fsph_last_error)
+        self._lib = lib
+        self.n_inst = int(n_inst)
+
+    def upload(self, parts_list):
+        """parts_list: one particle dict per instance (ids in [0, n_i))."""
+        if len(parts_list) != self.n_inst:
+            raise ValueError("need one particle set per instance")
+        arr = (_abi.FsphParticles * self.n_inst)()
+        keep = []
+        for i, parts in enumerate(parts_list):
+            n = len(parts["x"])
+            d = {}
+            for k in ("x", "y", "z", "vx", "vy", "vz", "u", "h", "m", "rho",
+                      "P", "c"):
+                if k in parts and parts[k] is not None:
+                    d[k] = np.ascontiguousarray(parts[k], dtype=np.float64)
+            d["id"] = np.ascontiguousarray(parts.get("id", np.arange(n)),
+                                           dtype=np.int64)
+            keep.append(d)
+            arr[i].n = n
+            for k in ("x", "y", "z", "vx", "vy", "vz", "u", "h", "m", "rho",
+                      "P", "c"):
+                setattr(arr[i], k, _dptr(d.get(k)))
+            arr[i].id = d["id"].ctypes.data_as(C.POINTER(C.c_int64))
+        self._call("upload_batch", arr, self.n_inst)
+
+    def instance_count(self, i):
+        nt, nr = C.c_int64(), C.c_int64()
+        self._call("instance_count", int(i), C.byref(nt), C.byref(nr))
+        return nt.value, nr.value
+
+    def instance_get(self, i, field):
+        n, _ = self.instance_count(i)
+        if field == "id":
+            out = np.empty(n, dtype=np.int64)
+        elif field == "key":
+            out = np.empty(n, dtype=np.uint64)
+        else:
+            out = np.empty(n, dtype=np.float64)
+        self._call("instance_download", int(i), _abi.FIELD_INDEX[field],
+                   out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def enable_graph(self, on=True):
+        self._call("enable_graph", int(bool(on)))
+
+    def instance_time(self, i):
+        it, dt, t = C.c_int64(), C.c_double(), C.c_double()
+        self._call("instance_time", int(i), C.byref(it), C.byref(dt),
+                   C.byref(t))
+        return it.value, dt.value, t.value
+
+
+def autotune(system, steps=2):
+    """Runs the library's autotuner on a BodySystem or BatchSystem; returns
+    (group_max, chunk)."""
+    g, ch = C.c_int32(), C.c_int32()
+    system._call("autotune", int(steps), C.byref(g), C.byref(ch))
+    return g.value, ch.value

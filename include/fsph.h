@@ -317,11 +317,58 @@ enum fsph_timer {
   FSPH_T_DUDT,          /* du/dt + P, c at the half step (1 kernel)      */
   FSPH_T_H,             /* smoothing-length update                       */
   FSPH_T_DT,            /* time-step reduction                           */
+  FSPH_T_GRAPH,         /* graph replay of a batched step's tail (pair
+                           passes, kicks, h, dt minimum)                 */
   FSPH_NTIMERS
 };
 int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]);
 /* Turns the per-phase HIP-event timing on or off (default off). */
 int fsph_enable_timing(fsph_ctx *ctx, int on);
+
+/* ---- batched mode (new; no reference counterpart: the reference runs one
+   simulation per MPI job) ----------------------------------------------
+
+   One context advances n_inst (1..256) independent simulations that share
+   one parameter set: the instances' particles are concatenated, and every
+   kernel of a step runs once for all of them.  Each instance evolves
+   exactly as it would in a context of its own (same range, keys, tree,
+   neighbours, periodic copies and time step).  fsph_step advances every
+   instance by one iteration of its own dt; fsph_update_iteration,
+   fsph_density_eos, fsph_acceleration, fsph_dudt, fsph_integrate and
+   fsph_update_h act on all instances.  Calls with one-simulation meaning
+   (fsph_upload, fsph_timestep, fsph_time, fsph_set_dt, fsph_reductions,
+   fsph_get_range, fsph_gravity) return FSPH_E_ARG on a batched context;
+   fsph_count and fsph_download cover the concatenated arrays.  Self-gravity
+   and FSPH_H_LEARNED are not available in batched mode. */
+int fsph_create_batch(const fsph_params *p, int device, int n_inst,
+                      fsph_ctx **out);
+/* parts[i] is instance i (ids in [0, parts[i].n)); rho, P, c for all
+   instances or none. */
+int fsph_upload_batch(fsph_ctx *ctx, const fsph_particles *parts,
+                      int n_inst);
+int fsph_instance_count(const fsph_ctx *ctx, int inst, int64_t *n_total,
+                        int64_t *n_real);
+/* One field (as fsph_download) of instance inst's particles, in the
+   instance's current (sorted) order. */
+int fsph_instance_download(fsph_ctx *ctx, int inst, int field, void *dst);
+int fsph_instance_time(const fsph_ctx *ctx, int inst, int64_t *iteration,
+                       double *dt, double *totaltime);
+
+/* Graph mode of a batched context (new): the sync-free tail of every step
+   after the neighbour tiles (density, signal speed, acceleration, kick v,
+   du/dt, kick u, closed-form h, per-instance dt minimum) runs as one
+   replayed hipGraph of persistent kernels.  Needs h_mode FSPH_H_CLOSED;
+   results are those of the eager path. */
+int fsph_enable_graph(fsph_ctx *ctx, int on);
+
+/* Autotuner (new): times `steps` calls of fsph_step for each candidate
+   leaf-group width (group_max 32, 64, 128) and candidate tile chunk of the
+   pair kernels (64, 128, 256 candidates staged in LDS per round), keeps the
+   fastest and returns it.  The timed steps are real steps of the
+   simulation(s); the physics does not depend on the choice (neighbour sets
+   are the same; sums are only re-associated). */
+int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
+                  int32_t *chunk);
 
 #ifdef __cplusplus
 }
