@@ -7,6 +7,16 @@ copies, range/keys/sort, octree, neighbour tiles, density/EOS/signal speed,
 acceleration, du/dt, smoothing length and time step.  Inputs are resident in
 HBM before the timed region; WALL copies are not counted in N.
 
+Two further measurements ride along in the same JSON line:
+  "learned_h": Newton-h against the learned smoothing length on the cold
+  collapse of an FCC-lattice sphere with self-gravity (default 4M particles,
+  the first --lh-learn steps train the predictor; times are averaged over
+  the steps after training): h-phase and whole-step speedups and the
+  neighbour-count error of both;
+  "batched": 64 Sedov blasts with log-uniform particle counts in [8k, 200k]
+  in one batched context (autotuned group width and LDS chunk, graph replay
+  of the step tail): aggregate particle-steps/s and p50/p99 step latency.
+
 Prints one JSON line (see the keys below).  --gpus N > 1 is not supported:
 this build runs one GPU.
 """
@@ -68,7 +78,104 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-learned-h", action="store_true")
+    ap.add_argument("--lh-n", type=int, default=4_000_000)
+    ap.add_argument("--lh-steps", type=int, default=80)
+    ap.add_argument("--lh-learn", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-inst", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def learned_h_comparison(n, steps, learn):
+    """Newton-h and learned-h runs of the same collapse; per-step h-phase
+    (HIP events) and wall times averaged over the steps after training."""
+    import numpy as np
+    from flecsph_amd import BodySystem, _abi, generators
+    parts, prm = generators.collapse(n=n)
+    n_real = len(parts["x"])
+    out = {}
+    for name, mode in (("newton", _abi.H_NEWTON), ("learned", _abi.H_LEARNED)):
+        g = BodySystem(_abi.default_params(
+            **dict(prm, h_mode=mode, learn_steps=learn)))
+        g.upload(parts)
+        g.enable_timing(True)
+        hms, sms = [], []
+        for k in range(steps):
+            t0 = time.perf_counter()
+            g.step()
+            t1 = time.perf_counter()
+            if k >= learn + 2:
+                hms.append(g.phase_times()["h"])
+                sms.append((t1 - t0) * 1e3)
+            if k % 20 == 19:
+                log(f"learned-h comparison: {name} step {k + 1}/{steps}")
+        ngb = g.get("hngb")[:]
+        ids = g.get("id")
+        real = ids < n_real
+        it = g.get("hiter")[real]
+        rec = {"h_ms": float(np.mean(hms)), "step_ms": float(np.mean(sms)),
+               "ngb_abs_err": float(np.mean(np.abs(ngb[real] - prm["n_target"]))),
+               "iterations": float(it.mean())}
+        if mode == _abi.H_LEARNED:
+            st = g.h_stats()
+            rec.update(fallback_last=st["fallback"], clamped_last=st["clamped"],
+                       train_loss=st["last_loss"])
+        out[name] = rec
+        g.close()
+    nw, le = out["newton"], out["learned"]
+    return {
+        "workload": f"collapse_fcc_sphere_{n_real}_fmm_theta0.5_quad_leaf32",
+        "steps": steps, "learn_steps": learn,
+        "timed_steps": steps - learn - 2,
+        "newton_h_ms": nw["h_ms"], "learned_h_ms": le["h_ms"],
+        "h_phase_speedup": nw["h_ms"] / le["h_ms"],
+        "newton_step_ms": nw["step_ms"], "learned_step_ms": le["step_ms"],
+        "step_speedup": nw["step_ms"] / le["step_ms"],
+        "ngb_abs_err_newton": nw["ngb_abs_err"],
+        "ngb_abs_err_learned": le["ngb_abs_err"],
+        "iterations_newton": nw["iterations"],
+        "iterations_learned": le["iterations"],
+        "fallback_last_step": le["fallback_last"],
+        "train_loss": le["train_loss"],
+    }
+
+
+def batched_run(n_inst, steps):
+    """Batched Sedov workload: autotune, graph replay, then `steps` timed
+    steps."""
+    import numpy as np
+    from flecsph_amd import BatchSystem, _abi, autotune, generators
+    sets, prm = generators.sedov_batch(n_inst)
+    N = sum(len(p["x"]) for p in sets)
+    b = BatchSystem(_abi.default_params(**prm), n_inst)
+    b.upload(sets)
+    b.step()
+    gm, ch = autotune(b, steps=1)
+    log(f"batched: autotune chose group_max {gm}, chunk {ch}")
+    b.enable_graph(True)
+    b.step()
+    lat = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        b.step()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    b.close()
+    lat = np.array(lat)
+    return {
+        "workload": f"sedov3d_x{n_inst}_loguniform_8k_200k",
+        "instances": n_inst, "particles": N, "steps": steps,
+        "value": N / (lat.mean() * 1e-3), "unit": "particle-steps/s",
+        "ms_per_step": float(lat.mean()),
+        "p50_step_ms": float(np.percentile(lat, 50)),
+        "p99_step_ms": float(np.percentile(lat, 99)),
+        "group_max": gm, "chunk": ch, "graph": True,
+    }
 
 
 def cpu_baseline(name, threads, steps):
@@ -162,7 +269,16 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in per_phase.items()},
         "sim_state": {"iteration": it, "dt": dt, "time": tt},
     }
+    g.close()
+    if not args.no_learned_h:
+        log("learned-h comparison ...")
+        out["learned_h"] = learned_h_comparison(args.lh_n, args.lh_steps,
+                                                args.lh_learn)
+    if not args.no_batch:
+        log("batched run ...")
+        out["batched"] = batched_run(args.batch_inst, args.batch_steps)
     if not args.no_cpu_baseline:
+        log("CPU baseline ...")
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.workload, thr, args.cpu_steps)
     print(json.dumps(out))

@@ -4,11 +4,12 @@
 // After the neighbour tiles are built, a step runs, with no host round trip
 // in between: density+EOS, signal speed, acceleration, kick v, du/dt, kick u,
 // the smoothing-length update and the per-instance minimum time step.  These
-// are captured once into a hipGraph as persistent kernels (fixed grids that
-// take their work from device-side counters and read their arguments and
-// the step's sizes from device memory), so every later step replays the
-// same executable graph after uploading one block of arguments.  The graph
-// is captured again when the configuration it was built for changes.
+// are captured once into a hipGraph as persistent kernels (grids of the
+// resident size that take their work from device-side counters and read
+// their arguments and the step's sizes from device memory), so every later
+// step replays the same executable graph after uploading one block of
+// arguments.  The graph is captured again when the configuration it was
+// built for changes.
 #include <cstring>
 
 #include "fsph_internal.h"
@@ -35,8 +36,8 @@ int graph_setup(Ctx *c) {
   FSPH_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                     c->device));
   if (cus < 1) cus = 1;
-  G.grid_groups = cus * 8;  // 128-thread workgroups of the pair passes
-  G.grid_elems = cus * 4;   // 256-thread blocks of the per-particle kernels
+  G.cus = cus;
+  G.grid_elems = cus * 16;  // 256-thread blocks of the per-particle kernels
   return 0;
 }
 

@@ -161,7 +161,7 @@ struct Ctx {
     int key[4] = {0, 0, 0, 0};   // chunk, rsize, adaptive_dt, n_inst
     DBuf slots, sizes, counters; // argument slots, {ng, n}, work counters
     std::vector<char> host;      // host staging of slots and sizes
-    int grid_groups = 0, grid_elems = 0;
+    int cus = 0, grid_elems = 0;
   } graph;
 
   // FMM gravity: cells (internal nodes and buckets), pair lists

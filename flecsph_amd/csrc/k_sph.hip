@@ -633,29 +633,43 @@ template <class T> void sph_graph_args_t(Ctx *c, char *slots) {
   std::swap(c->f[PF_C], c->g[PF_C]);
 }
 
+// Persistent grid: as many workgroups as fit on the device at once.
+template <class K> unsigned resident_grid(Ctx *c, K kernel, int threads) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads,
+                                                   0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 4;
+  return (unsigned)(per_cu * c->graph.cus);
+}
+
 template <class T, int CH> void sph_graph_launch_t(Ctx *c, int pass) {
   const char *slots = c->graph.slots.as<char>();
   unsigned long long *ctr = c->graph.counters.as<unsigned long long>();
   const int64_t *ng = c->graph.sizes.as<int64_t>();
-  const unsigned grid = (unsigned)c->graph.grid_groups;
+  unsigned grid = 0;
   switch (pass) {
     case kSlotDens:
+      grid = resident_grid(c, k_density_p<T, CH>, kB);
       k_density_p<T, CH><<<grid, kB, 0, c->stream>>>(
           reinterpret_cast<const DensArgs<T> *>(slots + kSlotDens * kGraphSlot),
           Work{ng, ctr + 0});
       break;
     case kSlotVsig:
+      grid = resident_grid(c, k_vsig_p<T, CH>, kB);
       k_vsig_p<T, CH><<<grid, kB, 0, c->stream>>>(
           reinterpret_cast<const VsigArgs<T> *>(slots + kSlotVsig * kGraphSlot),
           Work{ng, ctr + 1});
       break;
     case kSlotAcc:
+      grid = resident_grid(c, k_force_p<T, false, CH>, kB);
       k_force_p<T, false, CH><<<grid, kB, 0, c->stream>>>(
           reinterpret_cast<const ForceArgs<T, false> *>(slots +
                                                         kSlotAcc * kGraphSlot),
           Work{ng, ctr + 2});
       break;
     default:
+      grid = resident_grid(c, k_force_p<T, true, CH>, kB);
       k_force_p<T, true, CH><<<grid, kB, 0, c->stream>>>(
           reinterpret_cast<const ForceArgs<T, true> *>(slots +
                                                        kSlotDudt * kGraphSlot),
