@@ -74,7 +74,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sedov64",
-                    choices=["sedov64", "polytrope1m"])
+                    choices=["sedov64", "polytrope1m", "collapse4m_newton",
+                             "collapse4m_learned", "sedov_batch64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -201,12 +202,51 @@ def cpu_baseline(name, threads, steps):
                        f"oracle, {threads} OpenMP threads, {dt:.1f} s")
 
 
+def side_workload(args):
+    """The learned-h, Newton-h and batched workloads on their own: warmup
+    steps, then `steps` timed steps; one JSON line (used for profiling)."""
+    from flecsph_amd import BatchSystem, BodySystem, _abi, generators
+    name = args.workload
+    if name == "sedov_batch64":
+        sets, prm = generators.sedov_batch(64)
+        s = BatchSystem(_abi.default_params(**prm), 64)
+        s.upload(sets)
+        n = sum(len(p["x"]) for p in sets)
+        label = "sedov3d_x64_loguniform_8k_200k_graph"
+    else:
+        parts, prm = generators.collapse(n=4_000_000)
+        mode = _abi.H_LEARNED if name.endswith("learned") else _abi.H_NEWTON
+        s = BodySystem(_abi.default_params(
+            **dict(prm, h_mode=mode, learn_steps=args.lh_learn)))
+        s.upload(parts)
+        n = len(parts["x"])
+        label = f"collapse_fcc_sphere_{n}_{name.split('_')[1]}_h"
+    for k in range(args.warmup):
+        s.step()
+        if name == "sedov_batch64" and k == 0:
+            s.enable_graph(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    el = time.perf_counter() - t0
+    print(json.dumps({"metric": "particle-steps/s", "value": n * args.steps / el,
+                      "unit": "particle-steps/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": el / args.steps * 1e3,
+                      "higher_is_better": True, "data": "synthetic",
+                      "config": {"workload": label}}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "--gpus must be 1"}))
         sys.exit(2)
+    if args.workload in ("collapse4m_newton", "collapse4m_learned",
+                         "sedov_batch64"):
+        side_workload(args)
+        return
     from flecsph_amd import BodySystem, _abi
 
     parts, prm, label, step_bytes = workload(args.workload)

@@ -332,7 +332,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   fsph::free_gravity(c);
   fsph::free_mlp(c);
   fsph::free_graph(c);
-  fsph::DBuf *more[] = {&c->graph.slots, &c->graph.sizes, &c->graph.counters,
+  fsph::DBuf *more[] = {&c->graph.counters,
                         &c->batch.nreal, &c->batch.segr, &c->batch.seg,
                         &c->batch.bounds, &c->batch.halo, &c->batch.root,
                         &c->batch.dt};

@@ -24,6 +24,17 @@ __host__ __device__ inline int32_t leaf_code(int64_t p) {
 }
 __host__ __device__ inline int64_t leaf_of(int32_t c) { return -(int64_t)c - 2; }
 
+// Work item of workgroup b in a launch of G workgroups such that each XCD
+// gets a contiguous range of items.  The MI355X dispatcher hands workgroup b
+// to XCD b % 8, and each XCD has its own L2; consecutive groups (neighbours
+// in space, sharing most of their candidates) then hit the same L2.  A
+// bijection of [0, G).
+__host__ __device__ inline int64_t xcd_item(int64_t b, int64_t G) {
+  constexpr int64_t kX = 8;
+  const int64_t q = G / kX, r = G % kX, x = b % kX, i = b / kX;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // Growable device buffer.
 struct DBuf {
   void *p = nullptr;
@@ -159,9 +170,14 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int key[4] = {0, 0, 0, 0};   // chunk, rsize, adaptive_dt, n_inst
-    DBuf slots, sizes, counters; // argument slots, {ng, n}, work counters
-    std::vector<char> host;      // host staging of slots and sizes
-    int cus = 0, grid_elems = 0;
+    DBuf counters;               // work counters of the pair passes
+    std::vector<char> host;      // host staging of the kernel arguments
+    int cus = 0;
+    static constexpr int kNodes = 8;  // replayed kernels, in launch order
+    hipGraphNode_t node[kNodes] = {};
+    const void *func[kNodes] = {};
+    unsigned grid[kNodes] = {}, block[kNodes] = {};
+    int n_nodes = 0;
   } graph;
 
   // FMM gravity: cells (internal nodes and buckets), pair lists
@@ -277,13 +293,17 @@ int batch_reduce(Ctx *c, int what);
 // seg[] from the sorted instance indices; root[] from the tree
 int batch_segments(Ctx *c);
 int batch_roots(Ctx *c);
-// graph mode: host staging of the argument slots (k_sph.hip: the four pair
-// passes; k_step.hip: kicks, h, dt) and the persistent launches recorded
-// into the graph (pass = slot index; what: 0 kick v, 1 kick u, 2 h, 3 dt)
+// graph mode: host staging of the kernel arguments of one step (k_sph.hip:
+// the four pair passes, with the eager passes' buffer rotations; k_step.hip:
+// kicks, h, dt), the persistent kernels for the current configuration
+// (pass = slot index; what: 0 kick v, 1 kick u, 2 h, 3 dt minimum) and the
+// per-pass work descriptor
 void sph_graph_args(Ctx *c, char *slots);
-int sph_graph_launch(Ctx *c, int pass);
+const void *sph_graph_func(const Ctx *c, int pass);
+void sph_graph_work(Ctx *c, int pass, void *work);
 void step_graph_args(Ctx *c, char *slots);
-int step_graph_launch(Ctx *c, int what);
+const void *step_graph_func(const Ctx *c, int what);
+int step_graph_seg_init(Ctx *c);
 int graph_step_tail(Ctx *c);          // fsph_graph.hip
 void free_graph(Ctx *c);
 

@@ -373,7 +373,7 @@ template <class T> struct M2LArgs {
 
 template <class T>
 __global__ __launch_bounds__(64) void k_m2l(M2LArgs<T> a) {
-  const int64_t cell = blockIdx.x;
+  const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
   const int64_t b0 = a.off[cell], b1 = a.off[cell + 1];
   if (b0 == b1) return;
   T L[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   __shared__ T sm[kStage];
   __shared__ int32_t sidx[kStage];
   __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
-  const int64_t cell = blockIdx.x;
+  const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
   if (!a.c.bucket[cell]) return;
   const int cnt = a.c.count[cell];
   const int64_t st = a.c.start[cell];

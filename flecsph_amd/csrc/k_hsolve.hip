@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   __shared__ T sm[kChunk];
   __shared__ int32_t sid[kChunk];
   __shared__ uint32_t smask[kChunkW * kB];
-  const int64_t gi = blockIdx.x;
+  const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g], cc = a.ccount[gi];

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
       sh[kMaxGroup];
   __shared__ int32_t stk[kStack];
   const int lane = threadIdx.x;
-  const int64_t gi = blockIdx.x;
+  const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
@@ -219,16 +219,19 @@ struct MaskArgs {
   const int32_t *glist;  // tile slot -> group (or null)
 };
 
+// GATHER selects the gather-mode predicate at compile time (the SPH form
+// keeps its inner loop free of the other mode's code).
+template <bool GATHER>
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
 #pragma clang fp contract(off)
   __shared__ double cx[32], cy[32], cz[32], chh[32];
   __shared__ int cok[32];
   const int lane = threadIdx.x;
-  const int64_t gi = blockIdx.x;
+  const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g];
-  const int gn = a.gather ? -1 : a.gnode[g];
+  const int gn = GATHER ? -1 : a.gnode[g];
   const int cc = a.ccount[gi];
   const int64_t mo = a.moff[gi];
   const int64_t cbase = gi * a.ccap;
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       px[s] = a.x[gs + k];
       py[s] = a.y[gs + k];
       pz[s] = a.z[gs + k];
-      ph[s] = !a.gather
+      ph[s] = !GATHER
                   ? a.h[gs + k]
                   : a.rscale *
                         (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       for (int j = 0; j < 32; ++j) {
         if (!cok[j]) continue;
         bool in;
-        if (a.gather) {
+        if (GATHER) {
           const double d0 = px[s] - cx[j], d1 = py[s] - cy[j],
                        d2 = pz[s] - cz[j];
           in = d0 * d0 + d1 * d1 + d2 * d2 < ph[s] * ph[s];
@@ -373,7 +376,10 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
              hprop,                   glist};
   {
     PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
-    k_masks<<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    if (gather)
+      k_masks<true><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    else
+      k_masks<false><<<(unsigned)ng, kW, 0, c->stream>>>(m);
   }
   return hip_check(c, hipGetLastError(), "k_masks");
 }

@@ -429,55 +429,60 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
 // One workgroup per group (grid = number of groups).
 template <class T, int CH>
 __global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
-  density_group<T, CH>(a, blockIdx.x);
+  density_group<T, CH>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 template <class T, bool DIV, int CH>
 __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
-  vsig_group<T, DIV, CH>(a, blockIdx.x);
+  vsig_group<T, DIV, CH>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 template <class T, bool DUDT, int CH>
 __global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
-  force_group<T, DUDT, CH>(a, blockIdx.x);
+  force_group<T, DUDT, CH>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
-// Persistent form (for graph replay): a fixed grid whose workgroups take
-// groups from a work counter until *ng are done; the arguments are read
-// from device memory, so one captured launch serves every step.
+// Persistent form (for graph replay): a grid of resident size whose
+// workgroups take groups from a work counter until all ng are done.  The
+// arguments are kernel arguments (held in scalar registers, as in the
+// one-group-per-workgroup form); a replayed graph gets each step's values
+// through its kernel-node parameters.
 struct Work {
-  const int64_t *ng;
   unsigned long long *counter;  // zeroed before the launch
+  int64_t ng;
 };
 
-template <class F> __device__ inline void persistent_groups(const Work &w, F &&f) {
+// Next group of a persistent workgroup (uniform over the workgroup).
+__device__ inline int64_t next_group(const Work &w, int64_t *s_g) {
+  if (threadIdx.x == 0) *s_g = (int64_t)atomicAdd(w.counter, 1ull);
+  __syncthreads();
+  const int64_t g = *s_g;
+  __syncthreads();
+  return g;
+}
+
+template <class T, int CH>
+__global__ __launch_bounds__(kB) void k_density_p(DensArgs<T> a, Work w) {
   __shared__ int64_t s_g;
-  const int64_t ng = *w.ng;
-  for (;;) {
-    if (threadIdx.x == 0) s_g = (int64_t)atomicAdd(w.counter, 1ull);
-    __syncthreads();
-    const int64_t g = s_g;
-    __syncthreads();
-    if (g >= ng) break;
-    f(g);
+  for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
+    density_group<T, CH>(a, g);
     __syncthreads();
   }
 }
-
 template <class T, int CH>
-__global__ __launch_bounds__(kB) void k_density_p(const DensArgs<T> *pa,
-                                                  Work w) {
-  const DensArgs<T> a = *pa;
-  persistent_groups(w, [&](int64_t g) { density_group<T, CH>(a, g); });
-}
-template <class T, int CH>
-__global__ __launch_bounds__(kB) void k_vsig_p(const VsigArgs<T> *pa, Work w) {
-  const VsigArgs<T> a = *pa;
-  persistent_groups(w, [&](int64_t g) { vsig_group<T, false, CH>(a, g); });
+__global__ __launch_bounds__(kB) void k_vsig_p(VsigArgs<T> a, Work w) {
+  __shared__ int64_t s_g;
+  for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
+    vsig_group<T, false, CH>(a, g);
+    __syncthreads();
+  }
 }
 template <class T, bool DUDT, int CH>
-__global__ __launch_bounds__(kB) void k_force_p(const ForceArgs<T, DUDT> *pa,
+__global__ __launch_bounds__(kB) void k_force_p(ForceArgs<T, DUDT> a,
                                                 Work w) {
-  const ForceArgs<T, DUDT> a = *pa;
-  persistent_groups(w, [&](int64_t g) { force_group<T, DUDT, CH>(a, g); });
+  __shared__ int64_t s_g;
+  for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
+    force_group<T, DUDT, CH>(a, g);
+    __syncthreads();
+  }
 }
 
 Phys phys_of(const Ctx *c) {
@@ -633,55 +638,21 @@ template <class T> void sph_graph_args_t(Ctx *c, char *slots) {
   std::swap(c->f[PF_C], c->g[PF_C]);
 }
 
-// Persistent grid: as many workgroups as fit on the device at once.
-template <class K> unsigned resident_grid(Ctx *c, K kernel, int threads) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads,
-                                                   0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 4;
-  return (unsigned)(per_cu * c->graph.cus);
-}
-
-template <class T, int CH> void sph_graph_launch_t(Ctx *c, int pass) {
-  const char *slots = c->graph.slots.as<char>();
-  unsigned long long *ctr = c->graph.counters.as<unsigned long long>();
-  const int64_t *ng = c->graph.sizes.as<int64_t>();
-  unsigned grid = 0;
+template <class T, int CH> const void *sph_graph_func_t(int pass) {
   switch (pass) {
-    case kSlotDens:
-      grid = resident_grid(c, k_density_p<T, CH>, kB);
-      k_density_p<T, CH><<<grid, kB, 0, c->stream>>>(
-          reinterpret_cast<const DensArgs<T> *>(slots + kSlotDens * kGraphSlot),
-          Work{ng, ctr + 0});
-      break;
-    case kSlotVsig:
-      grid = resident_grid(c, k_vsig_p<T, CH>, kB);
-      k_vsig_p<T, CH><<<grid, kB, 0, c->stream>>>(
-          reinterpret_cast<const VsigArgs<T> *>(slots + kSlotVsig * kGraphSlot),
-          Work{ng, ctr + 1});
-      break;
+    case kSlotDens: return reinterpret_cast<const void *>(k_density_p<T, CH>);
+    case kSlotVsig: return reinterpret_cast<const void *>(k_vsig_p<T, CH>);
     case kSlotAcc:
-      grid = resident_grid(c, k_force_p<T, false, CH>, kB);
-      k_force_p<T, false, CH><<<grid, kB, 0, c->stream>>>(
-          reinterpret_cast<const ForceArgs<T, false> *>(slots +
-                                                        kSlotAcc * kGraphSlot),
-          Work{ng, ctr + 2});
-      break;
-    default:
-      grid = resident_grid(c, k_force_p<T, true, CH>, kB);
-      k_force_p<T, true, CH><<<grid, kB, 0, c->stream>>>(
-          reinterpret_cast<const ForceArgs<T, true> *>(slots +
-                                                       kSlotDudt * kGraphSlot),
-          Work{ng, ctr + 3});
+      return reinterpret_cast<const void *>(k_force_p<T, false, CH>);
+    default: return reinterpret_cast<const void *>(k_force_p<T, true, CH>);
   }
 }
 
-template <class T> void sph_graph_launch_c(Ctx *c, int pass) {
-  switch (c->chunk) {
-    case 64: sph_graph_launch_t<T, 64>(c, pass); break;
-    case 256: sph_graph_launch_t<T, 256>(c, pass); break;
-    default: sph_graph_launch_t<T, 128>(c, pass);
+template <class T> const void *sph_graph_func_c(int chunk, int pass) {
+  switch (chunk) {
+    case 64: return sph_graph_func_t<T, 64>(pass);
+    case 256: return sph_graph_func_t<T, 256>(pass);
+    default: return sph_graph_func_t<T, 128>(pass);
   }
 }
 
@@ -694,12 +665,14 @@ void sph_graph_args(Ctx *c, char *slots) {
     sph_graph_args_t<double>(c, slots);
 }
 
-int sph_graph_launch(Ctx *c, int pass) {
-  if (c->rsize == 4)
-    sph_graph_launch_c<float>(c, pass);
-  else
-    sph_graph_launch_c<double>(c, pass);
-  return hip_check(c, hipGetLastError(), "pair pass (persistent)");
+const void *sph_graph_func(const Ctx *c, int pass) {
+  return c->rsize == 4 ? sph_graph_func_c<float>(c->chunk, pass)
+                       : sph_graph_func_c<double>(c->chunk, pass);
+}
+
+void sph_graph_work(Ctx *c, int pass, void *work) {
+  const Work w{c->graph.counters.as<unsigned long long>() + pass, c->ng};
+  std::memcpy(work, &w, sizeof(w));
 }
 
 int stage_density(Ctx *c) {

@@ -213,27 +213,13 @@ __global__ __launch_bounds__(kT) void k_to_double(const T *src, double *dst,
 }
 
 // ---- batched mode: per-instance reductions -------------------------------
-// Min/max of doubles by compare-and-swap on the bit pattern (vector memory
-// atomics); exact, so the order of the updates does not matter.
+// Min/max of doubles with the native FP64 atomics (global_atomic_min_f64 /
+// max_f64); exact, so the order of the updates does not matter.
 __device__ inline void atomic_min_f64(double *a, double v) {
-  unsigned long long *p = reinterpret_cast<unsigned long long *>(a);
-  unsigned long long old = *p;
-  while (v < __longlong_as_double((long long)old)) {
-    const unsigned long long prev =
-        atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
-    if (prev == old) break;
-    old = prev;
-  }
+  __hip_atomic_fetch_min(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void atomic_max_f64(double *a, double v) {
-  unsigned long long *p = reinterpret_cast<unsigned long long *>(a);
-  unsigned long long old = *p;
-  while (v > __longlong_as_double((long long)old)) {
-    const unsigned long long prev =
-        atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
-    if (prev == old) break;
-    old = prev;
-  }
+  __hip_atomic_fetch_max(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Reduces NV values per particle into out[8 * inst + off + q] (min for the
@@ -285,42 +271,76 @@ __global__ __launch_bounds__(kT) void k_seg_init(double *out, int ni,
 }
 
 // range of x +- h (bodies_system.h:231-268) and max h per instance
+// One wavefront reduces a tile of 64 * kSegItems consecutive particles:
+// lane l takes particles base + 64 r + l (coalesced), keeps running
+// min/max while the instance stays the same (instances are contiguous runs,
+// so a tile rarely spans two), and the wavefront then combines its lanes.
+constexpr int kSegItems = 16;
+constexpr int64_t kSegTile = 64 * kSegItems;
+
+template <int NV, int NMIN, class F>
+__device__ inline void seg_tile(int64_t tile, int64_t n,
+                                const int32_t *__restrict__ inst, double *out,
+                                int off, F &&value) {
+  const int lane = threadIdx.x & 63;
+  const int64_t base = tile * kSegTile;
+  double acc[NV];
+  auto reset = [&]() {
+    for (int q = 0; q < NV; ++q) acc[q] = q < NMIN ? DBL_MAX : -DBL_MAX;
+  };
+  reset();
+  int cur = -1;
+  for (int r = 0; r < kSegItems; ++r) {
+    const int64_t i = base + (int64_t)r * 64 + lane;
+    if (i >= n) break;
+    const int ii = inst[i];
+    if (ii != cur) {
+      if (cur >= 0)
+        for (int q = 0; q < NV; ++q) {
+          if (q < NMIN)
+            atomic_min_f64(out + 8 * (int64_t)cur + off + q, acc[q]);
+          else
+            atomic_max_f64(out + 8 * (int64_t)cur + off + q, acc[q]);
+        }
+      cur = ii;
+      reset();
+    }
+    double v[NV];
+    value(i, v);
+    for (int q = 0; q < NV; ++q)
+      acc[q] = q < NMIN ? (v[q] < acc[q] ? v[q] : acc[q])
+                        : (v[q] > acc[q] ? v[q] : acc[q]);
+  }
+  seg_reduce<NV, NMIN>(cur >= 0, cur < 0 ? 0 : cur, acc, out, off);
+}
+
+// range of x +- h (bodies_system.h:231-268) and max h per instance
 __global__ __launch_bounds__(kT) void k_seg_range(
     const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ h,
     const int32_t *__restrict__ inst, int64_t n, double *out) {
+  const int64_t tile = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+  seg_tile<7, 3>(tile, n, inst, out, 0, [&](int64_t i, double *v) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const bool valid = i < n;
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  int ii = 0;
-  if (valid) {
     const double hh = h[i];
     v[0] = x[i] - hh; v[1] = y[i] - hh; v[2] = z[i] - hh;
     v[3] = x[i] + hh; v[4] = y[i] + hh; v[5] = z[i] + hh;
     v[6] = hh;
-    ii = inst[i];
-  }
-  // min of 3 lower ends, max of 3 upper ends and of h
-  double vmin[3] = {v[0], v[1], v[2]};
-  double vmax[4] = {v[3], v[4], v[5], v[6]};
-  seg_reduce<3, 3>(valid, ii, vmin, out, 0);
-  seg_reduce<4, 0>(valid, ii, vmax, out, 3);
+  });
 }
 
 template <class T>
 __global__ __launch_bounds__(kT) void k_seg_dt(DtArgs<T> a,
                                                const int32_t *__restrict__ inst,
                                                int64_t n, double *out) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const bool valid = i < n;
-  double v[1] = {1e24};
-  int ii = 0;
-  if (valid) {
-    v[0] = particle_dt(a, i);
-    ii = inst[i];
-  }
-  seg_reduce<1, 1>(valid, ii, v, out, 7);
+  const int64_t tile = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+  seg_tile<1, 1>(tile, n, inst, out, 7,
+                 [&](int64_t i, double *v) { v[0] = particle_dt(a, i); });
+}
+
+inline unsigned seg_blocks(int64_t n) {
+  const int64_t tiles = (n + kSegTile - 1) / kSegTile;
+  return (unsigned)((tiles * 64 + kT - 1) / kT);
 }
 
 __global__ __launch_bounds__(kT) void k_seg_bounds(
@@ -351,45 +371,30 @@ template <class T> struct StepSlot {
   double *bounds;
 };
 
-template <class T>
-__global__ __launch_bounds__(kT) void k_integrate_p(const StepSlot<T> *ps,
-                                                    const int64_t *pn,
-                                                    uint32_t what) {
-  const StepSlot<T> &s = *ps;
-  const int64_t n = pn[1];
+template <class T, uint32_t WHAT>
+__global__ __launch_bounds__(kT) void k_integrate_p(StepSlot<T> s, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kT)
-    integrate_one(s.ia, i, what, s.dtv[s.inst[i]]);
+    integrate_one(s.ia, i, WHAT, s.dtv[s.inst[i]]);
 }
 
 template <class T>
-__global__ __launch_bounds__(kT) void k_update_h_p(const StepSlot<T> *ps,
-                                                   const int64_t *pn) {
+__global__ __launch_bounds__(kT) void k_update_h_p(StepSlot<T> s, int64_t n) {
 #pragma clang fp contract(off)
-  const StepSlot<T> &s = *ps;
-  const int64_t n = pn[1];
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kT)
     s.h[i] = cbrt(s.m[i] / (double)s.rho[i]) * s.eta * s.kw;
 }
 
 template <class T>
-__global__ __launch_bounds__(kT) void k_seg_dt_p(const StepSlot<T> *ps,
-                                                 const int64_t *pn) {
-  const StepSlot<T> &s = *ps;
-  const int64_t n = pn[1];
-  for (int64_t b = (int64_t)blockIdx.x * kT; b < n;
-       b += (int64_t)gridDim.x * kT) {
-    const int64_t i = b + threadIdx.x;
-    const bool valid = i < n;
-    double v[1] = {1e24};
-    int ii = 0;
-    if (valid) {
+__global__ __launch_bounds__(kT) void k_seg_dt_p(StepSlot<T> s, int64_t n) {
+  const int64_t tiles = (n + kSegTile - 1) / kSegTile;
+  const int64_t waves = (int64_t)gridDim.x * (kT / 64);
+  for (int64_t t = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6; t < tiles;
+       t += waves)
+    seg_tile<1, 1>(t, n, s.inst, s.bounds, 7, [&](int64_t i, double *v) {
       v[0] = particle_dt(s.da, i);
-      ii = s.inst[i];
-    }
-    seg_reduce<1, 1>(valid, ii, v, s.bounds, 7);
-  }
+    });
 }
 
 template <class T> DtArgs<T> dt_args(Ctx *c) {
@@ -500,21 +505,12 @@ template <class T> void step_graph_args_t(Ctx *c, char *slot) {
   std::memcpy(slot, &s, sizeof(s));
 }
 
-template <class T> void step_graph_launch_t(Ctx *c, int what) {
-  const StepSlot<T> *ps = reinterpret_cast<const StepSlot<T> *>(
-      c->graph.slots.as<char>() + kSlotStep * kGraphSlot);
-  const int64_t *pn = c->graph.sizes.as<int64_t>();
-  const unsigned grid = (unsigned)c->graph.grid_elems;
+template <class T> const void *step_graph_func_t(int what) {
   switch (what) {
-    case 0: k_integrate_p<T><<<grid, kT, 0, c->stream>>>(ps, pn, FSPH_KICK_V); break;
-    case 1: k_integrate_p<T><<<grid, kT, 0, c->stream>>>(ps, pn, FSPH_KICK_U); break;
-    case 2: k_update_h_p<T><<<grid, kT, 0, c->stream>>>(ps, pn); break;
-    default: {
-      const int ni = c->batch.n_inst;
-      k_seg_init<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(
-          c->batch.bounds.as<double>(), ni, 1);
-      k_seg_dt_p<T><<<grid, kT, 0, c->stream>>>(ps, pn);
-    }
+    case 0: return reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_V>);
+    case 1: return reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_U>);
+    case 2: return reinterpret_cast<const void *>(k_update_h_p<T>);
+    default: return reinterpret_cast<const void *>(k_seg_dt_p<T>);
   }
 }
 
@@ -526,12 +522,16 @@ void step_graph_args(Ctx *c, char *slots) {
     step_graph_args_t<double>(c, slot);
 }
 
-int step_graph_launch(Ctx *c, int what) {
-  if (c->rsize == 4)
-    step_graph_launch_t<float>(c, what);
-  else
-    step_graph_launch_t<double>(c, what);
-  return hip_check(c, hipGetLastError(), "step kernels (persistent)");
+const void *step_graph_func(const Ctx *c, int what) {
+  return c->rsize == 4 ? step_graph_func_t<float>(what)
+                       : step_graph_func_t<double>(what);
+}
+
+int step_graph_seg_init(Ctx *c) {
+  const int ni = c->batch.n_inst;
+  k_seg_init<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(
+      c->batch.bounds.as<double>(), ni, 1);
+  return hip_check(c, hipGetLastError(), "k_seg_init");
 }
 
 int batch_reduce(Ctx *c, int what) {
@@ -540,14 +540,14 @@ int batch_reduce(Ctx *c, int what) {
   k_seg_init<<<(ni + kT - 1) / kT, kT, 0, c->stream>>>(out, ni, what);
   const int32_t *inst = c->f[PF_INST].as<int32_t>();
   if (what == 0) {
-    k_seg_range<<<blocks_for(c->n), kT, 0, c->stream>>>(
+    k_seg_range<<<seg_blocks(c->n), kT, 0, c->stream>>>(
         c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
         c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(), inst, c->n, out);
   } else if (c->rsize == 4) {
-    k_seg_dt<float><<<blocks_for(c->n), kT, 0, c->stream>>>(dt_args<float>(c),
+    k_seg_dt<float><<<seg_blocks(c->n), kT, 0, c->stream>>>(dt_args<float>(c),
                                                             inst, c->n, out);
   } else {
-    k_seg_dt<double><<<blocks_for(c->n), kT, 0, c->stream>>>(
+    k_seg_dt<double><<<seg_blocks(c->n), kT, 0, c->stream>>>(
         dt_args<double>(c), inst, c->n, out);
   }
   return hip_check(c, hipGetLastError(), "batch_reduce");
