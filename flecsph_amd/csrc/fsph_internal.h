@@ -24,15 +24,19 @@ __host__ __device__ inline int32_t leaf_code(int64_t p) {
 }
 __host__ __device__ inline int64_t leaf_of(int32_t c) { return -(int64_t)c - 2; }
 
-// Work item of workgroup b in a launch of G workgroups such that each XCD
-// gets a contiguous range of items.  The MI355X dispatcher hands workgroup b
-// to XCD b % 8, and each XCD has its own L2; consecutive groups (neighbours
-// in space, sharing most of their candidates) then hit the same L2.  A
-// bijection of [0, G).
+// Work item of workgroup b in a launch of G workgroups.  The MI355X
+// dispatcher hands workgroup b to XCD b % 8 and each XCD has its own L2, so
+// consecutive workgroups run on different L2s.  Here the items are dealt to
+// the XCDs in runs of kXcdRun consecutive ones (neighbouring groups share
+// most of their candidates, which then stay in one L2), while every XCD
+// still sweeps the whole range (groups of uneven cost stay balanced).  A
+// bijection of [0, G): the tail beyond the last whole round is not swizzled.
+constexpr int64_t kXcdRun = 16;
 __host__ __device__ inline int64_t xcd_item(int64_t b, int64_t G) {
-  constexpr int64_t kX = 8;
-  const int64_t q = G / kX, r = G % kX, x = b % kX, i = b / kX;
-  return x * q + (x < r ? x : r) + i;
+  constexpr int64_t kX = 8, kRound = kX * kXcdRun;
+  if (b >= G / kRound * kRound) return b;
+  const int64_t x = b % kX, j = b / kX;
+  return ((j / kXcdRun) * kX + x) * kXcdRun + j % kXcdRun;
 }
 
 // Growable device buffer.
