@@ -55,6 +55,21 @@ KERNEL_BYTES = {
 B_HYDRO = 864
 B_GRAV = 968
 
+# FP32 operations per neighbour pair of the pair kernels, counted from
+# flecsph_amd/csrc/k_sph.hip (Wendland C6; FMA = 2, rcp/rsqrt = 1; the three
+# FP64 position differences and the Kahan compensation included).
+PAIR_FLOPS = {"density": 32, "vsig": 20, "accel": 76, "dudt": 69}
+VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
+
+# HBM bytes per launch of the Sedov 64^3 pair kernels, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each scaled by what the
+# counter reads for a plain 256 MiB device copy (FETCH_SIZE x 2.00,
+# WRITE_SIZE x 1.00); profiles/sedov64_hbm_traffic.md has the commands.
+SEDOV64_TRAFFIC_BYTES = {
+    "masks": 224_360_354, "traverse": 124_931_367, "density": 234_199_386,
+    "vsig": 383_764_120, "accel": 562_229_812, "dudt": 717_041_808,
+}
+
 
 def workload(name):
     """(particles, params, label, step bytes) of a named workload."""
@@ -79,9 +94,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only three 256 MiB device-to-device copies (the "
+                    "reference kernel for scaling HBM counters)")
     ap.add_argument("--no-learned-h", action="store_true")
     ap.add_argument("--lh-n", type=int, default=4_000_000)
-    ap.add_argument("--lh-steps", type=int, default=80)
+    ap.add_argument("--lh-steps", type=int, default=200)
     ap.add_argument("--lh-learn", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-inst", type=int, default=64)
@@ -111,7 +129,7 @@ def learned_h_comparison(n, steps, learn):
             t0 = time.perf_counter()
             g.step()
             t1 = time.perf_counter()
-            if k >= learn + 2:
+            if k >= learn:
                 hms.append(g.phase_times()["h"])
                 sms.append((t1 - t0) * 1e3)
             if k % 20 == 19:
@@ -133,7 +151,7 @@ def learned_h_comparison(n, steps, learn):
     return {
         "workload": f"collapse_fcc_sphere_{n_real}_fmm_theta0.5_quad_leaf32",
         "steps": steps, "learn_steps": learn,
-        "timed_steps": steps - learn - 2,
+        "timed_steps": steps - learn,
         "newton_h_ms": nw["h_ms"], "learned_h_ms": le["h_ms"],
         "h_phase_speedup": nw["h_ms"] / le["h_ms"],
         "newton_step_ms": nw["step_ms"], "learned_step_ms": le["step_ms"],
@@ -237,8 +255,32 @@ def side_workload(args):
                       "config": {"workload": label}}))
 
 
+def copy_probe():
+    """Three plain 256 MiB device-to-device copies by the HIP runtime's copy
+    kernel: the known-size reference for FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    lib = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if lib.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed")
+    if lib.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+        raise SystemExit("hipMemset failed")
+    for _ in range(3):
+        if lib.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # device->device
+            raise SystemExit("hipMemcpy failed")
+    lib.hipDeviceSynchronize()
+    for p in (a, b):
+        lib.hipFree(p)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
                           "--gpus must be 1"}))
@@ -274,6 +316,17 @@ def main():
     single = {k: per_phase[k] for k in KERNEL_BYTES}
     dom = max(single, key=single.get)
     achieved = KERNEL_BYTES[dom] * avg_ntot / (single[dom] * 1e-3) / 1e9
+    traffic = (SEDOV64_TRAFFIC_BYTES.get(dom)
+               if args.workload == "sedov64" else None)
+    # pair count of the step: the neighbour-mask bits, one per (a, b)
+    pairs = float(g.get("nngb").sum())
+    valu = None
+    if dom in PAIR_FLOPS:
+        tf = PAIR_FLOPS[dom] * pairs / (single[dom] * 1e-3) / 1e12
+        valu = {"kernel": dom, "flops_per_pair": PAIR_FLOPS[dom],
+                "pairs": pairs, "achieved": tf,
+                "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / VALU_FP32_PEAK_TFLOPS}
     it, dt, tt = g.time()
     out = {
         "metric": "particle-steps/s",
@@ -303,8 +356,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
+        "valu": valu,
         "step_hbm_frac": step_bytes * value / (HBM_PEAK_GBS * 1e9),
         "phase_ms": {k: round(v, 4) for k, v in per_phase.items()},
         "sim_state": {"iteration": it, "dt": dt, "time": tt},
