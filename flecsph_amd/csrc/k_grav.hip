@@ -303,20 +303,50 @@ __global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
 }
 
 // counting sort of a pair list by sink: counts, then scatter of sources
+// Pairs of one sink come in runs (the children of a split source cell are
+// listed together), so each wavefront updates a sink's counter once per run:
+// the first lane of a run adds the run's length.
+struct SinkRun {
+  int sink, head, len;  // head: first lane of this lane's run
+};
+
+__device__ inline SinkRun sink_run(bool valid, int sink) {
+  const int lane = threadIdx.x & 63;
+  int prev = __shfl_up(sink, 1, 64);
+  if (lane == 0) prev = -2;
+  const bool head = valid && sink != prev;
+  const uint64_t hm = __ballot(head);
+  const uint64_t vm = __ballot(valid);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  SinkRun r{sink, 63 - __clzll(hm & upto), 0};
+  if (head) {
+    const uint64_t later = lane == 63 ? 0ull : (hm & ~upto);
+    const int next = later ? __ffsll((long long)later) - 1 : 64;
+    const int end_valid = 64 - __clzll(vm);
+    r.len = (next < end_valid ? next : end_valid) - lane;
+  }
+  return r;
+}
+
 __global__ __launch_bounds__(kT) void k_count_sinks(const int2 *__restrict__ l,
                                                     int64_t n,
                                                     int32_t *__restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[l[i].x], 1);
+  const bool valid = i < n;
+  const SinkRun r = sink_run(valid, valid ? l[i].x : -1);
+  if (valid && r.len) atomicAdd(&cnt[r.sink], r.len);
 }
-
 __global__ __launch_bounds__(kT) void k_scatter_sinks(
     const int2 *__restrict__ l, int64_t n,
     unsigned long long *__restrict__ cursor, int32_t *__restrict__ src) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  const int2 pr = l[i];
-  src[atomicAdd(&cursor[pr.x], 1ull)] = pr.y;
+  const bool valid = i < n;
+  const int2 pr = valid ? l[i] : make_int2(-1, 0);
+  const SinkRun r = sink_run(valid, pr.x);
+  unsigned long long base = 0;
+  if (valid && r.len) base = atomicAdd(&cursor[r.sink], (unsigned long long)r.len);
+  base = __shfl(base, r.head, 64);
+  if (valid) src[base + (threadIdx.x & 63) - r.head] = pr.y;
 }
 
 // ---------------------------------------------------------------------------
