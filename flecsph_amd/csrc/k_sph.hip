@@ -8,10 +8,12 @@
 // chunks of CH candidates (SoA, coalesced loads); each lane walks its set
 // bits in one flattened loop, so the loop count is the lane's own neighbour
 // count rather than a per-word maximum.  Pair arithmetic runs in T (float by
-// default, double for FSPH_PREC_FP64) with compensated (Kahan) sums;
-// position differences are taken in FP64 first.
+// default, double for FSPH_PREC_FP64) with compensated (Kahan) sums; FP32
+// kernels stage positions relative to the group's first particle (rounded
+// once from an FP64 difference), FP64 kernels stage them as they are.
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 
 #include "fsph_internal.h"
 
@@ -61,6 +63,17 @@ __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double rcp(double x) { return 1.0 / x; }
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+
+// Positions in the pair passes: FP32 kernels take them relative to the
+// group's first particle (one FP64 subtraction, one rounding: the error of a
+// pair difference is below 2^-24 of the particles' distance from that
+// origin, a few smoothing lengths), FP64 kernels absolute.
+template <class T> using Pos = typename std::conditional<sizeof(T) == 4, float,
+                                                         double>::type;
+template <class T> __device__ inline double pos_origin(const double *x,
+                                                       int64_t gs) {
+  return sizeof(T) == 4 ? x[gs] : 0.0;
+}
 
 struct Phys {
   int kernel;
@@ -191,15 +204,18 @@ template <class T> struct DensArgs {
 
 template <class T, int CH>
 __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
-  __shared__ double sx[CH], sy[CH], sz[CH];
+  using P = Pos<T>;
+  __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T sh[CH], sm[CH];
   __shared__ uint32_t smask[(CH / 32) * kB];
   const TileCtx c = tile_ctx(a.t, g);
-  double px = 0, py = 0, pz = 0;
+  const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
+               oz = pos_origin<T>(a.z, c.gs);
+  P px = 0, py = 0, pz = 0;
   T ph = 1;
   if (c.active) {
     const int64_t i = c.gs + c.k;
-    px = a.x[i]; py = a.y[i]; pz = a.z[i];
+    px = (P)(a.x[i] - ox); py = (P)(a.y[i] - oy); pz = (P)(a.z[i] - oz);
     ph = (T)a.h[i];
   }
   KSum<T> rho;
@@ -207,7 +223,8 @@ __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
-      sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
+      sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
+      sz[j] = (P)(a.z[b] - oz);
       sh[j] = (T)a.h[b]; sm[j] = (T)a.m[b];
     }
     stage_masks(a.t, c, c0, nw, smask);
@@ -250,18 +267,21 @@ template <class T> struct VsigArgs {
 
 template <class T, bool DIV, int CH>
 __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
-  __shared__ double sx[CH], sy[CH], sz[CH];
+  using P = Pos<T>;
+  __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T svx[CH], svy[CH], svz[CH], scn[CH],
       sco[CH];
   __shared__ T sh[DIV ? CH : 1], sm[DIV ? CH : 1];
   __shared__ int32_t sid[CH];
   __shared__ uint32_t smask[(CH / 32) * kB];
   const TileCtx c = tile_ctx(a.t, g);
-  double px = 0, py = 0, pz = 0;
+  const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
+               oz = pos_origin<T>(a.z, c.gs);
+  P px = 0, py = 0, pz = 0;
   T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0, ph = 1, dv = 0;
   const int32_t ia = (int32_t)(c.gs + c.k);
   if (c.active) {
-    px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
+    px = (P)(a.x[ia] - ox); py = (P)(a.y[ia] - oy); pz = (P)(a.z[ia] - oz);
     pvx = (T)a.vx[ia]; pvy = (T)a.vy[ia]; pvz = (T)a.vz[ia];
     pc = a.c[ia];
     if (DIV) ph = (T)a.h[ia];
@@ -270,7 +290,8 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int32_t b = a.t.cand[c.cb + c0 + j];
-      sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
+      sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
+      sz[j] = (P)(a.z[b] - oz);
       svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
       scn[j] = a.c[b]; sco[j] = a.c_old[b];
       sid[j] = b;
@@ -282,8 +303,8 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
     walk(c, smask, nw, [&](int j) {
-      const double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-      if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
+      const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+      if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
       const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
       const T r2 = dx * dx + dy * dy + dz * dz;
       const T ir = rsq(r2);
@@ -328,7 +349,8 @@ template <class T, bool DUDT> struct ForceArgs {
 
 template <class T, bool DUDT, int CH>
 __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
-  __shared__ double sx[CH], sy[CH], sz[CH];
+  using P = Pos<T>;
+  __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T svhx[CH], svhy[CH], svhz[CH];
   __shared__ T svx[DUDT ? CH : 1], svy[DUDT ? CH : 1],
       svz[DUDT ? CH : 1];
@@ -336,13 +358,15 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
       sc[CH];
   __shared__ uint32_t smask[(CH / 32) * kB];
   const TileCtx c = tile_ctx(a.t, g);
-  double px = 0, py = 0, pz = 0;
+  const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
+               oz = pos_origin<T>(a.z, c.gs);
+  P px = 0, py = 0, pz = 0;
   T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
   T ph = 1, prho = 1, pprho2 = 0, pc = 0;
   const T alpha = (T)a.ph.alpha, beta = (T)a.ph.beta, eps = (T)a.ph.eps;
   if (c.active) {
     const int64_t i = c.gs + c.k;
-    px = a.x[i]; py = a.y[i]; pz = a.z[i];
+    px = (P)(a.x[i] - ox); py = (P)(a.y[i] - oy); pz = (P)(a.z[i] - oz);
     pvhx = (T)a.vhx[i]; pvhy = (T)a.vhy[i]; pvhz = (T)a.vhz[i];
     if (DUDT) {
       pvx = (T)a.vx[i]; pvy = (T)a.vy[i]; pvz = (T)a.vz[i];
@@ -357,7 +381,8 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
     for (int j = threadIdx.x; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
-      sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
+      sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
+      sz[j] = (P)(a.z[b] - oz);
       svhx[j] = (T)a.vhx[b]; svhy[j] = (T)a.vhy[b]; svhz[j] = (T)a.vhz[b];
       if (DUDT) {
         svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
@@ -372,8 +397,8 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
     walk(c, smask, nw, [&](int j) {
-      const double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-      if (ddx == 0.0 && ddy == 0.0 && ddz == 0.0) return;
+      const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+      if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
       const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
       const T r2 = dx * dx + dy * dy + dz * dz;
       const T ir = rsq(r2);
