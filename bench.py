@@ -40,8 +40,9 @@ KERNEL_BYTES = {
     "density": 60,
     # x,y,z,vx,vy,vz read (48); c, c_old read (8); vsig written (4)
     "vsig": 60,
-    # x,y,z,vhx,vhy,vhz,h,m read (64); rho,P,c read (12); a, g, phi (28)
-    "accel": 104,
+    # x,y,z,vhx,vhy,vhz,h,m read (64); rho,P,c read (12); a, g, phi (28);
+    # with the signal speed folded in: v (24), c_old (4) read, vsig (4)
+    "accel": 136,
     # x,y,z,vh,v,h,m,u read (88); rho,P,c read (12); dudt,P,c written (12)
     "dudt": 112,
     # x,y,z,h read (32); candidate ids written (4 B per candidate ~ not
@@ -56,9 +57,10 @@ B_HYDRO = 864
 B_GRAV = 968
 
 # FP32 operations per neighbour pair of the pair kernels, counted from
-# flecsph_amd/csrc/k_sph.hip (Wendland C6; FMA = 2, rcp/rsqrt = 1; the three
-# FP64 position differences and the Kahan compensation included).
-PAIR_FLOPS = {"density": 32, "vsig": 20, "accel": 76, "dudt": 69}
+# flecsph_amd/csrc/k_sph.hip (Wendland C6; FMA = 2, rcp/rsqrt = 1; position
+# differences and the Kahan compensation included; the driver's
+# acceleration pass includes the signal speed).
+PAIR_FLOPS = {"density": 32, "vsig": 20, "accel": 89, "dudt": 69}
 VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
 
 # HBM bytes per launch of the Sedov 64^3 pair kernels, from rocprofv3

@@ -185,12 +185,12 @@ int do_step(Ctx *c) {
   if (!c->initial_done) {
     // hydro/main_driver.cc:102-173 (iteration 0: no integration)
     FSPH_TRY(do_update_iteration(c));
-    FSPH_TRY(stage_density(c));
+    FSPH_TRY(stage_density_only(c));
     {
       PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(c, FSPH_SAVE_VHALF));
     }
-    FSPH_TRY(stage_acceleration(c));
+    FSPH_TRY(stage_acceleration_vsig(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
       FSPH_TRY(do_gravity(c));
@@ -220,8 +220,8 @@ int do_step(Ctx *c) {
       }
       return do_advance(c);
     }
-    FSPH_TRY(stage_density(c));
-    FSPH_TRY(stage_acceleration(c));
+    FSPH_TRY(stage_density_only(c));
+    FSPH_TRY(stage_acceleration_vsig(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
       FSPH_TRY(do_gravity(c));

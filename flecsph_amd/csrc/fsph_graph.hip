@@ -2,12 +2,12 @@
 // contexts with the closed-form smoothing length and no self-gravity).
 //
 // After the neighbour tiles are built, a step runs, with no host round trip
-// in between: density+EOS, signal speed, acceleration, kick v, du/dt, kick u,
-// the smoothing-length update and the per-instance minimum time step.  These
-// are captured once into a hipGraph as persistent kernels: grids of the
-// resident size whose workgroups take groups from device-side work counters
-// (pair passes) or stride over the particles (per-particle kernels), so the
-// launch shapes do not depend on the step.  Every step then updates the
+// in between: density+EOS, acceleration with the signal speed, kick v,
+// du/dt, kick u, the smoothing-length update and the per-instance minimum
+// time step.  These are captured once into a hipGraph as persistent
+// kernels: grids of the resident size whose workgroups take groups from
+// device-side work counters (pair passes) or stride over the particles
+// (per-particle kernels), so the launch shapes do not depend on the step.  Every step then updates the
 // kernel nodes' arguments (pointers after the buffer rotations, group and
 // particle counts) and replays the same executable graph.  The graph is
 // captured again when the configuration it was built for changes.
@@ -68,8 +68,7 @@ int step_specs(Ctx *c, NodeSpec *sp) {
   };
   int k = 0;
   sp[k++] = pair(kSlotDens);
-  sp[k++] = pair(kSlotVsig);
-  sp[k++] = pair(kSlotAcc);
+  sp[k++] = pair(kSlotAcc);  // with the signal speed
   sp[k++] = step(0);
   sp[k++] = pair(kSlotDudt);
   sp[k++] = step(1);
@@ -104,7 +103,8 @@ int graph_capture(Ctx *c, const NodeSpec *sp, int ns) {
       G.counters.as<unsigned long long>(), 4);
   hipError_t e = hipGetLastError();
   for (int i = 0; i < ns && e == hipSuccess; ++i) {
-    if (i == 7 && step_graph_seg_init(c)) e = hipErrorUnknown;
+    if (i == ns - 1 && c->p.adaptive_dt && step_graph_seg_init(c))
+      e = hipErrorUnknown;
     if (e == hipSuccess)
       e = hipLaunchKernel(sp[i].func, dim3(G.grid[i]), dim3(G.block[i]),
                           const_cast<void **>(sp[i].args), 0, c->stream);

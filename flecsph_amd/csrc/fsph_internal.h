@@ -279,6 +279,10 @@ int mlp_num_params();
 void free_mlp(Ctx *c);
 int stage_density(Ctx *c);             // k_sph.hip
 int stage_acceleration(Ctx *c);        // k_sph.hip
+// the driver's forms: density without the signal speed, and the
+// acceleration pass computing the signal speed (and div v) as well
+int stage_density_only(Ctx *c);        // k_sph.hip
+int stage_acceleration_vsig(Ctx *c);   // k_sph.hip
 int stage_dudt(Ctx *c);                // k_sph.hip
 int stage_gravity(Ctx *c);             // k_grav.hip
 void free_gravity(Ctx *c);             // k_grav.hip

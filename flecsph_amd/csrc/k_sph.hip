@@ -344,31 +344,41 @@ template <class T, bool DUDT> struct ForceArgs {
   double dt;
   const double *dtv;    // batched mode: dt of the particle's instance
   const int32_t *inst;
+  // signal speed (and div v) in the acceleration pass, as the separate
+  // signal-speed pass computes them: c_old, vsig, divv
+  const T *c_old;
+  T *vsig, *divv;
   Phys ph;
 };
 
-template <class T, bool DUDT, int CH>
+// VS (acceleration pass only): 0 no signal speed, 1 signal speed, 2 signal
+// speed and div v.  The pair set, pair order and arithmetic of the signal
+// speed are those of vsig_group, so the results are the same.
+template <class T, bool DUDT, int CH, int VS = 0>
 __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   using P = Pos<T>;
+  constexpr bool V = DUDT || VS > 0;  // velocities staged
   __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T svhx[CH], svhy[CH], svhz[CH];
-  __shared__ T svx[DUDT ? CH : 1], svy[DUDT ? CH : 1],
-      svz[DUDT ? CH : 1];
+  __shared__ T svx[V ? CH : 1], svy[V ? CH : 1], svz[V ? CH : 1];
   __shared__ T sh[CH], sm[CH], srho[CH], sprho2[CH],
       sc[CH];
+  __shared__ T sco[VS ? CH : 1];
+  __shared__ int32_t sid[VS ? CH : 1];
   __shared__ uint32_t smask[(CH / 32) * kB];
   const TileCtx c = tile_ctx(a.t, g);
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
                oz = pos_origin<T>(a.z, c.gs);
   P px = 0, py = 0, pz = 0;
   T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
-  T ph = 1, prho = 1, pprho2 = 0, pc = 0;
+  T ph = 1, prho = 1, pprho2 = 0, pc = 0, vs = 0, dv = 0;
   const T alpha = (T)a.ph.alpha, beta = (T)a.ph.beta, eps = (T)a.ph.eps;
+  const int32_t ia = (int32_t)(c.gs + c.k);
   if (c.active) {
     const int64_t i = c.gs + c.k;
     px = (P)(a.x[i] - ox); py = (P)(a.y[i] - oy); pz = (P)(a.z[i] - oz);
     pvhx = (T)a.vhx[i]; pvhy = (T)a.vhy[i]; pvhz = (T)a.vhz[i];
-    if (DUDT) {
+    if (V) {
       pvx = (T)a.vx[i]; pvy = (T)a.vy[i]; pvz = (T)a.vz[i];
     }
     ph = (T)a.h[i];
@@ -384,8 +394,12 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
       svhx[j] = (T)a.vhx[b]; svhy[j] = (T)a.vhy[b]; svhz[j] = (T)a.vhz[b];
-      if (DUDT) {
+      if (V) {
         svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
+      }
+      if (VS) {
+        sco[j] = a.c_old[b];
+        sid[j] = (int32_t)b;
       }
       sh[j] = (T)a.h[b];
       sm[j] = (T)a.m[b];
@@ -406,6 +420,17 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
       const T hab = T(0.5) * (ph + sh[j]);
       const T ih = rcp(hab);
       const T F = kernel_F<T>(a.ph.kernel, r, ir, ih);
+      if (VS) {
+        // default_physics.h:237-265 (see vsig_group)
+        const T vdr =
+            (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
+        const T vdn = vdr * ir;
+        const T cb = sid[j] >= ia ? sc[j] : sco[j];
+        const T cmax = pc < cb ? cb : pc;
+        const T v = cmax - (vdn < T(0) ? vdn : T(0));
+        vs = vs < v ? v : vs;
+        if (VS == 2) dv += sm[j] * vdr * F;
+      }
       if (F == T(0)) return;
       const T vhdr = (pvhx - svhx[j]) * dx + (pvhy - svhy[j]) * dy +
                      (pvhz - svhz[j]) * dz;
@@ -431,8 +456,12 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   }
   const T t0 = ksum_total(s0, c.LP), t1 = ksum_total(s1, c.LP);
   const T t2 = DUDT ? T(0) : ksum_total(s2, c.LP);
+  if (VS) vs = xor_max(vs, c.LP);
+  if (VS == 2) dv = xor_sum(dv, c.LP);
   if (!(c.active && c.r == 0)) return;
   const int64_t i = c.gs + c.k;
+  if (VS) a.vsig[i] = vs;
+  if (VS == 2) a.divv[i] = -dv / prho;
   if (!DUDT) {
     a.ax[i] = t0;
     a.ay[i] = t1;
@@ -460,9 +489,9 @@ template <class T, bool DIV, int CH>
 __global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
   vsig_group<T, DIV, CH>(a, xcd_item(blockIdx.x, gridDim.x));
 }
-template <class T, bool DUDT, int CH>
+template <class T, bool DUDT, int CH, int VS = 0>
 __global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
-  force_group<T, DUDT, CH>(a, xcd_item(blockIdx.x, gridDim.x));
+  force_group<T, DUDT, CH, VS>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
 // Persistent form (for graph replay): a grid of resident size whose
@@ -500,12 +529,12 @@ __global__ __launch_bounds__(kB) void k_vsig_p(VsigArgs<T> a, Work w) {
     __syncthreads();
   }
 }
-template <class T, bool DUDT, int CH>
+template <class T, bool DUDT, int CH, int VS = 0>
 __global__ __launch_bounds__(kB) void k_force_p(ForceArgs<T, DUDT> a,
                                                 Work w) {
   __shared__ int64_t s_g;
   for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
-    force_group<T, DUDT, CH>(a, g);
+    force_group<T, DUDT, CH, VS>(a, g);
     __syncthreads();
   }
 }
@@ -597,11 +626,15 @@ template <class T, bool DUDT> ForceArgs<T, DUDT> force_args(Ctx *c) {
     a.dtv = c->batch.dt.as<double>();
     a.inst = c->f[PF_INST].as<int32_t>();
   }
+  a.c_old = c->f[PF_COLD].as<T>();
+  a.vsig = c->f[PF_VSIG].as<T>();
+  a.divv = c->f[PF_DIVV].as<T>();
   a.ph = phys_of(c);
   return a;
 }
 
-template <class T> int density_impl(Ctx *c) {
+// with_vsig false: the signal speed is left to the acceleration pass
+template <class T> int density_impl(Ctx *c, bool with_vsig) {
   std::swap(c->f[PF_C], c->f[PF_COLD]);  // keep the previous sound speed
   const DensArgs<T> a = dens_args<T>(c);
   {
@@ -613,6 +646,7 @@ template <class T> int density_impl(Ctx *c) {
     }
   }
   FSPH_HIP(c, hipGetLastError());
+  if (!with_vsig) return 0;
   const VsigArgs<T> v = vsig_args<T>(c);
   {
     PhaseTimer tm(c, FSPH_T_VSIG);
@@ -624,15 +658,15 @@ template <class T> int density_impl(Ctx *c) {
   return hip_check(c, hipGetLastError(), "k_vsig");
 }
 
-template <class T, bool DUDT> int force_impl(Ctx *c) {
+template <class T, bool DUDT, int VS = 0> int force_impl(Ctx *c) {
   const ForceArgs<T, DUDT> a = force_args<T, DUDT>(c);
   {
     PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
     const unsigned ng = (unsigned)c->ng;
     switch (c->chunk) {
-      case 64: k_force<T, DUDT, 64><<<ng, kB, 0, c->stream>>>(a); break;
-      case 256: k_force<T, DUDT, 256><<<ng, kB, 0, c->stream>>>(a); break;
-      default: k_force<T, DUDT, 128><<<ng, kB, 0, c->stream>>>(a);
+      case 64: k_force<T, DUDT, 64, VS><<<ng, kB, 0, c->stream>>>(a); break;
+      case 256: k_force<T, DUDT, 256, VS><<<ng, kB, 0, c->stream>>>(a); break;
+      default: k_force<T, DUDT, 128, VS><<<ng, kB, 0, c->stream>>>(a);
     }
   }
   FSPH_HIP(c, hipGetLastError());
@@ -668,7 +702,7 @@ template <class T, int CH> const void *sph_graph_func_t(int pass) {
     case kSlotDens: return reinterpret_cast<const void *>(k_density_p<T, CH>);
     case kSlotVsig: return reinterpret_cast<const void *>(k_vsig_p<T, CH>);
     case kSlotAcc:
-      return reinterpret_cast<const void *>(k_force_p<T, false, CH>);
+      return reinterpret_cast<const void *>(k_force_p<T, false, CH, 1>);
     default: return reinterpret_cast<const void *>(k_force_p<T, true, CH>);
   }
 }
@@ -701,12 +735,25 @@ void sph_graph_work(Ctx *c, int pass, void *work) {
 }
 
 int stage_density(Ctx *c) {
-  return c->rsize == 4 ? density_impl<float>(c) : density_impl<double>(c);
+  return c->rsize == 4 ? density_impl<float>(c, true)
+                       : density_impl<double>(c, true);
 }
 
 int stage_acceleration(Ctx *c) {
   return c->rsize == 4 ? force_impl<float, false>(c)
                        : force_impl<double, false>(c);
+}
+
+int stage_density_only(Ctx *c) {
+  return c->rsize == 4 ? density_impl<float>(c, false)
+                       : density_impl<double>(c, false);
+}
+
+int stage_acceleration_vsig(Ctx *c) {
+  const bool div = c->p.h_mode == FSPH_H_LEARNED;
+  if (c->rsize == 4)
+    return div ? force_impl<float, false, 2>(c) : force_impl<float, false, 1>(c);
+  return div ? force_impl<double, false, 2>(c) : force_impl<double, false, 1>(c);
 }
 
 int stage_dudt(Ctx *c) {
