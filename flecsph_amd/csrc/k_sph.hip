@@ -1,14 +1,13 @@
 // k_sph.hip -- the SPH pair passes over the neighbour tiles.
 //
-// One 128-thread workgroup per group.  A group of gc particles gets
-// LP = 128/gc (rounded down to a power of two, at most 64) lanes per
-// particle: lane r of particle k walks mask words r, r+LP, ... of the
-// particle's row, and the LP partial results are combined with xor
-// shuffles.  Candidates and the chunk's mask words are staged in LDS in
-// chunks of CH candidates (SoA, coalesced loads); each lane walks its set
-// bits in one flattened loop, so the loop count is the lane's own neighbour
-// count rather than a per-word maximum.  Pair arithmetic runs in T (float by
-// default, double for FSPH_PREC_FP64) with compensated (Kahan) sums; FP32
+// One 128-thread workgroup per group.  Candidates and the chunk's mask
+// words are staged in LDS in chunks of CH candidates (SoA, coalesced
+// loads); the group's own particles are staged once.  The chunk's
+// neighbour pairs are then dealt to the threads in equal runs (the balanced
+// schedule below), so a thread's loop count is the chunk's pair count over
+// 128 whatever the particles' neighbour counts, and the partial sums are
+// added up per particle in a fixed order.  Pair arithmetic runs in T (float
+// by default, double for FSPH_PREC_FP64) with compensated (Kahan) sums; FP32
 // kernels stage positions relative to the group's first particle (rounded
 // once from an FP64 difference), FP64 kernels stage them as they are.
 #include <cfloat>
@@ -34,28 +33,6 @@ template <class T> struct KSum {  // Kahan compensated sum
     s = t;
   }
 };
-
-template <class T> __device__ inline T xor_sum(T v, int LP) {
-  for (int o = LP >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-template <class T> __device__ inline T xor_max(T v, int LP) {
-  for (int o = LP >> 1; o > 0; o >>= 1) {
-    const T u = __shfl_xor(v, o, 64);
-    v = v < u ? u : v;
-  }
-  return v;
-}
-// Combines the lanes' Kahan partials of one particle.
-template <class T> __device__ inline T ksum_total(const KSum<T> &k, int LP) {
-  return xor_sum(k.s, LP) - xor_sum(k.c, LP);
-}
-
-__device__ inline int lanes_per_particle(int gc) {
-  int lp = 1;
-  while (lp < 64 && gc * lp * 2 <= kB) lp <<= 1;
-  return lp;
-}
 
 // Fast reciprocal / reciprocal square root: hardware approximations in
 // FP32 (about 1 ulp), IEEE operations in FP64.
@@ -144,52 +121,163 @@ struct Tiles {
 };
 
 // Per-workgroup view of one group's tile.
-struct TileCtx {
-  int64_t g, gs, mo, cb;
-  int gc, cc, LP, k, r;
-  bool active;
+struct Grp {
+  int64_t gs, mo, cb;
+  int gc, cc;
 };
 
-__device__ inline TileCtx tile_ctx(const Tiles &t, int64_t g) {
-  TileCtx c;
-  c.g = g;
-  c.gs = t.gstart[c.g];
-  c.gc = t.gcount[c.g];
-  c.cc = t.ccount[c.g];
-  c.mo = t.moff[c.g];
-  c.cb = c.g * t.ccap;
-  c.LP = lanes_per_particle(c.gc);
-  c.k = threadIdx.x / c.LP;
-  c.r = threadIdx.x & (c.LP - 1);
-  c.active = c.k < c.gc;
+__device__ inline Grp grp_of(const Tiles &t, int64_t g) {
+  Grp c;
+  c.gs = t.gstart[g];
+  c.gc = t.gcount[g];
+  c.cc = t.ccount[g];
+  c.mo = t.moff[g];
+  c.cb = g * t.ccap;
   return c;
 }
 
 // Stages the chunk's mask words ([word][particle] layout, contiguous) in LDS.
-__device__ inline void stage_masks(const Tiles &t, const TileCtx &c, int c0,
+__device__ inline void stage_masks(const Tiles &t, const Grp &c, int c0,
                                    int nw, uint32_t *smask) {
   const uint32_t *src = t.mask + c.mo + (int64_t)(c0 >> 5) * c.gc;
   for (int q = threadIdx.x; q < nw * c.gc; q += kB) smask[q] = src[q];
 }
 
-// Calls f(j) for every candidate j of the chunk that neighbours this lane's
-// particle, over the lane's share of the mask words.
-template <class F>
-__device__ inline void walk(const TileCtx &c, const uint32_t *smask, int nw,
-                            F &&f) {
-  if (!c.active) return;
-  int w = c.r;
-  uint32_t bits = w < nw ? smask[w * c.gc + c.k] : 0u;
-  for (;;) {
-    while (bits == 0u) {
-      w += c.LP;
-      if (w >= nw) return;
-      bits = smask[w * c.gc + c.k];
-    }
-    const int j = __ffs(bits) - 1;
-    bits &= bits - 1u;
-    f(w * 32 + j);
+// ---------------------------------------------------------------------------
+// Balanced pair schedule of one chunk.
+//
+// Row k of the chunk's mask (nw words; candidate j is bit j%32 of word
+// j/32) marks the neighbours of group particle k among the chunk's
+// candidates.  The chunk's P pairs, in (k, j) order, are cut into kB runs
+// of q = ceil(P/kB) pairs and thread l walks run l, so every thread walks
+// the same number of pairs however unevenly they fall on the particles.
+// A run covers whole rows plus at most two partial rows: the row it starts
+// in (which may have begun in an earlier run) and the row it ends in (which
+// may go on in a later run).  A thread writes the partial result of each
+// row it covers to one of three places: own[k] when its run holds all of
+// row k, first[l] when the row began in an earlier run, last[l] when the
+// row goes on in a later run.  After the walk thread k adds up row k in run
+// order: own[k], or last[la] + first[la+1] + ... + first[lb] for the first
+// and last runs la < lb holding its pairs.  The result depends only on the
+// tile, not on timing.
+// ---------------------------------------------------------------------------
+struct Sched {
+  int off[kB + 1];  // pairs before row k; off[k] = P for k >= gc
+  int wsum[2];
+};
+
+template <class T, int NV> struct Parts {
+  T first[NV][kB], last[NV][kB], own[NV][kB];
+};
+
+enum { kOwn = 0, kFirst = 1, kLast = 2 };
+
+// Row offsets of the chunk (requires the staged masks); returns P.
+__device__ inline int schedule(Sched &S, const uint32_t *smask, int nw,
+                               int gc) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int n = 0;
+  if (t < gc)
+    for (int w = 0; w < nw; ++w) n += __popc(smask[w * gc + t]);
+  int inc = n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
   }
+  if (lane == 63) S.wsum[wv] = inc;
+  __syncthreads();
+  const int total = S.wsum[0] + S.wsum[1];
+  S.off[t] = inc - n + (wv ? S.wsum[0] : 0);
+  if (t == 0) S.off[kB] = total;
+  __syncthreads();
+  return total;
+}
+
+// x without its r lowest set bits (r < popc(x)).
+__device__ inline uint32_t drop_low(uint32_t x, int r) {
+  uint32_t y = x;
+  int pos = 0;
+#pragma unroll
+  for (int w = 16; w > 0; w >>= 1) {
+    const int n = __popc(y & ((1u << w) - 1u));
+    if (n <= r) {
+      r -= n;
+      y >>= w;
+      pos += w;
+    }
+  }
+  return x & (~0u << pos);
+}
+
+// Walks this thread's run: load(k) when the run enters row k, pair(j) for
+// every pair (k, j), flush(k, kind) when the run leaves row k.
+template <class Load, class Pair, class Flush>
+__device__ inline void walk_run(const Sched &S, const uint32_t *smask, int gc,
+                                int total, Load &&load, Pair &&pair,
+                                Flush &&flush) {
+  const int q = (total + kB - 1) / kB;
+  const int p0 = (int)threadIdx.x * q, p1 = min(p0 + q, total);
+  if (p0 >= p1) return;
+  int k = 0;
+#pragma unroll
+  for (int s = kB / 2; s > 0; s >>= 1)
+    if (S.off[k + s] <= p0) k += s;
+  int rend = S.off[k + 1];
+  int r = p0 - S.off[k];
+  bool began = r > 0;
+  int w = 0;
+  uint32_t bits = smask[k];
+  for (int n; (n = __popc(bits)) <= r;) {
+    r -= n;
+    bits = smask[(++w) * gc + k];
+  }
+  bits = drop_low(bits, r);
+  load(k);
+  for (int p = p0; p < p1; ++p) {
+    if (p == rend) {
+      flush(k, began ? kFirst : kOwn);
+      began = false;
+      ++k;
+      while (S.off[k + 1] <= p) ++k;
+      rend = S.off[k + 1];
+      w = 0;
+      bits = smask[k];
+      load(k);
+    }
+    while (bits == 0u) bits = smask[(++w) * gc + k];
+    const int j = (w << 5) + __ffs(bits) - 1;
+    bits &= bits - 1u;
+    pair(j);
+  }
+  flush(k, began ? kFirst : (rend > p1 ? kLast : kOwn));
+}
+
+template <class T, int NV>
+__device__ inline void put(Parts<T, NV> &R, int i, int k, int kind, T v) {
+  if (kind == kOwn)
+    R.own[i][k] = v;
+  else if (kind == kFirst)
+    R.first[i][threadIdx.x] = v;
+  else
+    R.last[i][threadIdx.x] = v;
+}
+
+// Row k's result of this chunk, component i (after the walk; the row must
+// hold pairs); MAX combines with max instead of +.
+template <bool MAX, class T, int NV>
+__device__ inline T row_total(const Sched &S, const Parts<T, NV> &R, int i,
+                              int k, int total) {
+  const int q = (total + kB - 1) / kB;
+  const int a = S.off[k], b = S.off[k + 1];
+  const int la = a / q, lb = (b - 1) / q;
+  if (la == lb) return R.own[i][k];
+  T v = R.last[i][la];
+  for (int l = la + 1; l <= lb; ++l) {
+    const T u = R.first[i][l];
+    v = MAX ? (v < u ? u : v) : v + u;
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -208,20 +296,23 @@ __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
   __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T sh[CH], sm[CH];
   __shared__ uint32_t smask[(CH / 32) * kB];
-  const TileCtx c = tile_ctx(a.t, g);
+  __shared__ P qx[kB], qy[kB], qz[kB];
+  __shared__ T qh[kB];
+  __shared__ Sched S;
+  __shared__ Parts<T, 1> R;
+  const Grp c = grp_of(a.t, g);
+  const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
                oz = pos_origin<T>(a.z, c.gs);
-  P px = 0, py = 0, pz = 0;
-  T ph = 1;
-  if (c.active) {
-    const int64_t i = c.gs + c.k;
-    px = (P)(a.x[i] - ox); py = (P)(a.y[i] - oy); pz = (P)(a.z[i] - oz);
-    ph = (T)a.h[i];
+  if (t < c.gc) {
+    const int64_t i = c.gs + t;
+    qx[t] = (P)(a.x[i] - ox); qy[t] = (P)(a.y[i] - oy);
+    qz[t] = (P)(a.z[i] - oz); qh[t] = (T)a.h[i];
   }
   KSum<T> rho;
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = threadIdx.x; j < nch; j += kB) {
+    for (int j = t; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
@@ -229,19 +320,32 @@ __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
     }
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    walk(c, smask, nw, [&](int j) {
-      const T dx = (T)(px - sx[j]), dy = (T)(py - sy[j]), dz = (T)(pz - sz[j]);
-      const T r2 = dx * dx + dy * dy + dz * dz;
-      const T r = r2 > T(0) ? r2 * rsq(r2) : T(0);
-      const T ih = rcp(T(0.5) * (ph + sh[j]));
-      rho.add(sm[j] * kernel_W<T>(a.ph.kernel, r, ih));
-    });
+    const int np = schedule(S, smask, nw, c.gc);
+    P px = 0, py = 0, pz = 0;
+    T ph = 1;
+    KSum<T> acc;
+    walk_run(
+        S, smask, c.gc, np,
+        [&](int k) { px = qx[k]; py = qy[k]; pz = qz[k]; ph = qh[k]; },
+        [&](int j) {
+          const T dx = (T)(px - sx[j]), dy = (T)(py - sy[j]),
+                  dz = (T)(pz - sz[j]);
+          const T r2 = dx * dx + dy * dy + dz * dz;
+          const T r = r2 > T(0) ? r2 * rsq(r2) : T(0);
+          const T ih = rcp(T(0.5) * (ph + sh[j]));
+          acc.add(sm[j] * kernel_W<T>(a.ph.kernel, r, ih));
+        },
+        [&](int k, int kind) {
+          put(R, 0, k, kind, acc.s - acc.c);
+          acc = KSum<T>();
+        });
     __syncthreads();
+    if (t < c.gc && S.off[t + 1] > S.off[t])
+      rho.add(row_total<false>(S, R, 0, t, np));
   }
-  const T tot = ksum_total(rho, c.LP);
-  if (c.active && c.r == 0) {
-    const int64_t i = c.gs + c.k;
-    const double rr = (double)tot;
+  if (t < c.gc) {
+    const int64_t i = c.gs + t;
+    const double rr = (double)(rho.s - rho.c);
     double P, cs;
     eos_pc(a.ph, rr, a.u[i], &P, &cs);
     a.rho[i] = (T)rr;
@@ -268,27 +372,33 @@ template <class T> struct VsigArgs {
 template <class T, bool DIV, int CH>
 __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
   using P = Pos<T>;
+  constexpr int NV = DIV ? 2 : 1;
   __shared__ P sx[CH], sy[CH], sz[CH];
-  __shared__ T svx[CH], svy[CH], svz[CH], scn[CH],
-      sco[CH];
+  __shared__ T svx[CH], svy[CH], svz[CH], scn[CH], sco[CH];
   __shared__ T sh[DIV ? CH : 1], sm[DIV ? CH : 1];
   __shared__ int32_t sid[CH];
   __shared__ uint32_t smask[(CH / 32) * kB];
-  const TileCtx c = tile_ctx(a.t, g);
+  __shared__ P qx[kB], qy[kB], qz[kB];
+  __shared__ T qvx[kB], qvy[kB], qvz[kB], qc[kB];
+  __shared__ T qh[DIV ? kB : 1];
+  __shared__ Sched S;
+  __shared__ Parts<T, NV> R;
+  const Grp c = grp_of(a.t, g);
+  const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
                oz = pos_origin<T>(a.z, c.gs);
-  P px = 0, py = 0, pz = 0;
-  T pvx = 0, pvy = 0, pvz = 0, pc = 0, vs = 0, ph = 1, dv = 0;
-  const int32_t ia = (int32_t)(c.gs + c.k);
-  if (c.active) {
-    px = (P)(a.x[ia] - ox); py = (P)(a.y[ia] - oy); pz = (P)(a.z[ia] - oz);
-    pvx = (T)a.vx[ia]; pvy = (T)a.vy[ia]; pvz = (T)a.vz[ia];
-    pc = a.c[ia];
-    if (DIV) ph = (T)a.h[ia];
+  if (t < c.gc) {
+    const int64_t i = c.gs + t;
+    qx[t] = (P)(a.x[i] - ox); qy[t] = (P)(a.y[i] - oy);
+    qz[t] = (P)(a.z[i] - oz);
+    qvx[t] = (T)a.vx[i]; qvy[t] = (T)a.vy[i]; qvz[t] = (T)a.vz[i];
+    qc[t] = a.c[i];
+    if (DIV) qh[t] = (T)a.h[i];
   }
+  T vs = 0, dv = 0;  // thread t: particle t's totals
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = threadIdx.x; j < nch; j += kB) {
+    for (int j = t; j < nch; j += kB) {
       const int32_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
@@ -302,31 +412,54 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
     }
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    walk(c, smask, nw, [&](int j) {
-      const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-      if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
-      const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-      const T r2 = dx * dx + dy * dy + dz * dz;
-      const T ir = rsq(r2);
-      const T vdr =
-          (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
-      const T vdn = vdr * ir;
-      const T cb = sid[j] >= ia ? scn[j] : sco[j];
-      const T cab = pc < cb ? cb : pc;
-      const T v = cab - (vdn < T(0) ? vdn : T(0));
-      vs = vs < v ? v : vs;
-      if (DIV) {
-        const T ih = rcp(T(0.5) * (ph + sh[j]));
-        dv += sm[j] * vdr * kernel_F<T>(a.kernel, r2 * ir, ir, ih);
-      }
-    });
+    const int np = schedule(S, smask, nw, c.gc);
+    P px = 0, py = 0, pz = 0;
+    T pvx = 0, pvy = 0, pvz = 0, pc = 0, ph = 1, rvs = 0, rdv = 0;
+    int32_t ia = 0;
+    walk_run(
+        S, smask, c.gc, np,
+        [&](int k) {
+          px = qx[k]; py = qy[k]; pz = qz[k];
+          pvx = qvx[k]; pvy = qvy[k]; pvz = qvz[k];
+          pc = qc[k];
+          if (DIV) ph = qh[k];
+          ia = (int32_t)(c.gs + k);
+        },
+        [&](int j) {
+          const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+          if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
+          const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
+          const T r2 = dx * dx + dy * dy + dz * dz;
+          const T ir = rsq(r2);
+          const T vdr =
+              (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
+          const T vdn = vdr * ir;
+          const T cb = sid[j] >= ia ? scn[j] : sco[j];
+          const T cab = pc < cb ? cb : pc;
+          const T v = cab - (vdn < T(0) ? vdn : T(0));
+          rvs = rvs < v ? v : rvs;
+          if (DIV) {
+            const T ih = rcp(T(0.5) * (ph + sh[j]));
+            rdv += sm[j] * vdr * kernel_F<T>(a.kernel, r2 * ir, ir, ih);
+          }
+        },
+        [&](int k, int kind) {
+          put(R, 0, k, kind, rvs);
+          if (DIV) put(R, NV - 1, k, kind, rdv);
+          rvs = 0;
+          rdv = 0;
+        });
     __syncthreads();
+    if (t < c.gc && S.off[t + 1] > S.off[t]) {
+      const T u = row_total<true>(S, R, 0, t, np);
+      vs = vs < u ? u : vs;
+      if (DIV) dv += row_total<false>(S, R, NV - 1, t, np);
+    }
   }
-  vs = xor_max(vs, c.LP);
-  if (DIV) dv = xor_sum(dv, c.LP);
-  if (c.active && c.r == 0) {
-    a.vsig[ia] = vs;
-    if (DIV) a.divv[ia] = -dv / a.rho[ia];
+  if (t < c.gc) {
+    const int64_t i = c.gs + t;
+    a.vsig[i] = vs;
+    if (DIV) a.divv[i] = -dv / a.rho[i];
   }
 }
 
@@ -352,44 +485,51 @@ template <class T, bool DUDT> struct ForceArgs {
 };
 
 // VS (acceleration pass only): 0 no signal speed, 1 signal speed, 2 signal
-// speed and div v.  The pair set, pair order and arithmetic of the signal
+// speed and div v.  The pair set and the pair arithmetic of the signal
 // speed are those of vsig_group, so the results are the same.
 template <class T, bool DUDT, int CH, int VS = 0>
 __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   using P = Pos<T>;
   constexpr bool V = DUDT || VS > 0;  // velocities staged
+  constexpr int NS = DUDT ? 2 : 3;     // compensated sums
+  constexpr int NV = NS + (VS > 0) + (VS == 2);
   __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T svhx[CH], svhy[CH], svhz[CH];
   __shared__ T svx[V ? CH : 1], svy[V ? CH : 1], svz[V ? CH : 1];
-  __shared__ T sh[CH], sm[CH], srho[CH], sprho2[CH],
-      sc[CH];
+  __shared__ T sh[CH], sm[CH], srho[CH], sprho2[CH], sc[CH];
   __shared__ T sco[VS ? CH : 1];
   __shared__ int32_t sid[VS ? CH : 1];
   __shared__ uint32_t smask[(CH / 32) * kB];
-  const TileCtx c = tile_ctx(a.t, g);
+  __shared__ P qx[kB], qy[kB], qz[kB];
+  __shared__ T qvhx[kB], qvhy[kB], qvhz[kB];
+  __shared__ T qvx[V ? kB : 1], qvy[V ? kB : 1], qvz[V ? kB : 1];
+  __shared__ T qh[kB], qrho[kB], qprho2[kB], qc[kB];
+  __shared__ Sched S;
+  __shared__ Parts<T, NV> R;
+  const Grp c = grp_of(a.t, g);
+  const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
                oz = pos_origin<T>(a.z, c.gs);
-  P px = 0, py = 0, pz = 0;
-  T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
-  T ph = 1, prho = 1, pprho2 = 0, pc = 0, vs = 0, dv = 0;
   const T alpha = (T)a.ph.alpha, beta = (T)a.ph.beta, eps = (T)a.ph.eps;
-  const int32_t ia = (int32_t)(c.gs + c.k);
-  if (c.active) {
-    const int64_t i = c.gs + c.k;
-    px = (P)(a.x[i] - ox); py = (P)(a.y[i] - oy); pz = (P)(a.z[i] - oz);
-    pvhx = (T)a.vhx[i]; pvhy = (T)a.vhy[i]; pvhz = (T)a.vhz[i];
+  if (t < c.gc) {
+    const int64_t i = c.gs + t;
+    qx[t] = (P)(a.x[i] - ox); qy[t] = (P)(a.y[i] - oy);
+    qz[t] = (P)(a.z[i] - oz);
+    qvhx[t] = (T)a.vhx[i]; qvhy[t] = (T)a.vhy[i]; qvhz[t] = (T)a.vhz[i];
     if (V) {
-      pvx = (T)a.vx[i]; pvy = (T)a.vy[i]; pvz = (T)a.vz[i];
+      qvx[t] = (T)a.vx[i]; qvy[t] = (T)a.vy[i]; qvz[t] = (T)a.vz[i];
     }
-    ph = (T)a.h[i];
-    prho = a.rho[i];
-    pprho2 = a.P[i] / (prho * prho);
-    pc = a.c[i];
+    qh[t] = (T)a.h[i];
+    const T rho = a.rho[i];
+    qrho[t] = rho;
+    qprho2[t] = a.P[i] / (rho * rho);
+    qc[t] = a.c[i];
   }
-  KSum<T> s0, s1, s2;
+  KSum<T> tot[NS];  // thread t: particle t's totals
+  T vs = 0, dv = 0;
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = threadIdx.x; j < nch; j += kB) {
+    for (int j = t; j < nch; j += kB) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
@@ -410,65 +550,100 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     }
     stage_masks(a.t, c, c0, nw, smask);
     __syncthreads();
-    walk(c, smask, nw, [&](int j) {
-      const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-      if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
-      const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-      const T r2 = dx * dx + dy * dy + dz * dz;
-      const T ir = rsq(r2);
-      const T r = r2 * ir;
-      const T hab = T(0.5) * (ph + sh[j]);
-      const T ih = rcp(hab);
-      const T F = kernel_F<T>(a.ph.kernel, r, ir, ih);
-      if (VS) {
-        // default_physics.h:237-265 (see vsig_group)
-        const T vdr =
-            (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
-        const T vdn = vdr * ir;
-        const T cb = sid[j] >= ia ? sc[j] : sco[j];
-        const T cmax = pc < cb ? cb : pc;
-        const T v = cmax - (vdn < T(0) ? vdn : T(0));
-        vs = vs < v ? v : vs;
-        if (VS == 2) dv += sm[j] * vdr * F;
-      }
-      if (F == T(0)) return;
-      const T vhdr = (pvhx - svhx[j]) * dx + (pvhy - svhy[j]) * dy +
-                     (pvhz - svhz[j]) * dz;
-      T mu = T(0);
-      if (vhdr < T(0)) mu = hab * vhdr * rcp(r2 + eps * hab * hab);
-      const T rhoab = T(0.5) * (prho + srho[j]);
-      const T cab = T(0.5) * (pc + sc[j]);
-      const T Pi = (-alpha * cab + beta * mu) * mu * rcp(rhoab);
-      if (!DUDT) {
-        const T f = -sm[j] * (pprho2 + sprho2[j] + Pi) * F;
-        s0.add(f * dx);
-        s1.add(f * dy);
-        s2.add(f * dz);
-      } else {
-        const T vdw = ((pvx - svx[j]) * dx + (pvy - svy[j]) * dy +
-                       (pvz - svz[j]) * dz) * F;
-        const T mv = sm[j] * vdw;
-        s0.add(mv);
-        s1.add(mv * Pi);
-      }
-    });
+    const int np = schedule(S, smask, nw, c.gc);
+    P px = 0, py = 0, pz = 0;
+    T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
+    T ph = 1, prho = 1, pprho2 = 0, pc = 0, rvs = 0, rdv = 0;
+    int32_t ia = 0;
+    KSum<T> s[NS];
+    walk_run(
+        S, smask, c.gc, np,
+        [&](int k) {
+          px = qx[k]; py = qy[k]; pz = qz[k];
+          pvhx = qvhx[k]; pvhy = qvhy[k]; pvhz = qvhz[k];
+          if (V) {
+            pvx = qvx[k]; pvy = qvy[k]; pvz = qvz[k];
+          }
+          ph = qh[k]; prho = qrho[k]; pprho2 = qprho2[k]; pc = qc[k];
+          ia = (int32_t)(c.gs + k);
+        },
+        [&](int j) {
+          const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+          if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
+          const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
+          const T r2 = dx * dx + dy * dy + dz * dz;
+          const T ir = rsq(r2);
+          const T r = r2 * ir;
+          const T hab = T(0.5) * (ph + sh[j]);
+          const T ih = rcp(hab);
+          const T F = kernel_F<T>(a.ph.kernel, r, ir, ih);
+          if (VS) {
+            // default_physics.h:237-265 (see vsig_group)
+            const T vdr = (pvx - svx[j]) * dx + (pvy - svy[j]) * dy +
+                          (pvz - svz[j]) * dz;
+            const T vdn = vdr * ir;
+            const T cb = sid[j] >= ia ? sc[j] : sco[j];
+            const T cmax = pc < cb ? cb : pc;
+            const T v = cmax - (vdn < T(0) ? vdn : T(0));
+            rvs = rvs < v ? v : rvs;
+            if (VS == 2) rdv += sm[j] * vdr * F;
+          }
+          if (F == T(0)) return;
+          const T vhdr = (pvhx - svhx[j]) * dx + (pvhy - svhy[j]) * dy +
+                         (pvhz - svhz[j]) * dz;
+          T mu = T(0);
+          if (vhdr < T(0)) mu = hab * vhdr * rcp(r2 + eps * hab * hab);
+          const T rhoab = T(0.5) * (prho + srho[j]);
+          const T cab = T(0.5) * (pc + sc[j]);
+          const T Pi = (-alpha * cab + beta * mu) * mu * rcp(rhoab);
+          if (!DUDT) {
+            const T f = -sm[j] * (pprho2 + sprho2[j] + Pi) * F;
+            s[0].add(f * dx);
+            s[1].add(f * dy);
+            s[NS - 1].add(f * dz);
+          } else {
+            const T vdw = ((pvx - svx[j]) * dx + (pvy - svy[j]) * dy +
+                           (pvz - svz[j]) * dz) * F;
+            const T mv = sm[j] * vdw;
+            s[0].add(mv);
+            s[1].add(mv * Pi);
+          }
+        },
+        [&](int k, int kind) {
+#pragma unroll
+          for (int n = 0; n < NS; ++n) {
+            put(R, n, k, kind, s[n].s - s[n].c);
+            s[n] = KSum<T>();
+          }
+          if (VS) put(R, NS, k, kind, rvs);
+          if (VS == 2) put(R, NV - 1, k, kind, rdv);
+          rvs = 0;
+          rdv = 0;
+        });
     __syncthreads();
+    if (t < c.gc && S.off[t + 1] > S.off[t]) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) tot[n].add(row_total<false>(S, R, n, t, np));
+      if (VS) {
+        const T u = row_total<true>(S, R, NS, t, np);
+        vs = vs < u ? u : vs;
+      }
+      if (VS == 2) dv += row_total<false>(S, R, NV - 1, t, np);
+    }
   }
-  const T t0 = ksum_total(s0, c.LP), t1 = ksum_total(s1, c.LP);
-  const T t2 = DUDT ? T(0) : ksum_total(s2, c.LP);
-  if (VS) vs = xor_max(vs, c.LP);
-  if (VS == 2) dv = xor_sum(dv, c.LP);
-  if (!(c.active && c.r == 0)) return;
-  const int64_t i = c.gs + c.k;
+  if (t >= c.gc) return;
+  const int64_t i = c.gs + t;
+  const T prho = qrho[t];
   if (VS) a.vsig[i] = vs;
   if (VS == 2) a.divv[i] = -dv / prho;
   if (!DUDT) {
-    a.ax[i] = t0;
-    a.ay[i] = t1;
-    a.az[i] = t2;
+    a.ax[i] = tot[0].s - tot[0].c;
+    a.ay[i] = tot[1].s - tot[1].c;
+    a.az[i] = tot[NS - 1].s - tot[NS - 1].c;
     a.gx[i] = T(0); a.gy[i] = T(0); a.gz[i] = T(0); a.phi[i] = T(0);
   } else {
-    const T du = pprho2 * t0 + T(0.5) * t1;
+    const T du = qprho2[t] * (tot[0].s - tot[0].c) +
+                 T(0.5) * (tot[1].s - tot[1].c);
     a.dudt[i] = du;
     // recompute_pressure_soundspeed (default_physics.h:154-161)
     const double dti = a.dtv ? a.dtv[a.inst[i]] : a.dt;
