@@ -220,12 +220,21 @@ struct MaskArgs {
 };
 
 // GATHER selects the gather-mode predicate at compile time (the SPH form
-// keeps its inner loop free of the other mode's code).
+// keeps its inner loop free of the other mode's code).  The wavefront
+// stages kR candidates per round and deals the round's (particle, mask
+// word) combinations over its lanes, so groups smaller than the wavefront
+// keep every lane busy.  A candidate that fails the centre-of-mass filter
+// (or lies past the tile's end) is staged with a NaN x, which fails every
+// distance test.  max(h_k, h_b)^2 is taken as max(h_k^2, h_b^2): squaring
+// and rounding are monotone, so the bound is the same number.
 template <bool GATHER>
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
 #pragma clang fp contract(off)
-  __shared__ double cx[32], cy[32], cz[32], chh[32];
-  __shared__ int cok[32];
+  constexpr int kR = 128;  // candidates per round (4 mask words)
+  __shared__ double cx[kR], cy[kR], cz[kR], ch2[kR];
+  __shared__ double px[kMaxGroup], py[kMaxGroup], pz[kMaxGroup],
+      ph2[kMaxGroup];
+  __shared__ int cnt[kMaxGroup];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
@@ -235,18 +244,15 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
   const int cc = a.ccount[gi];
   const int64_t mo = a.moff[gi];
   const int64_t cbase = gi * a.ccap;
-  double px[2], py[2], pz[2], ph[2];
-  for (int s = 0; s < 2; ++s) {
-    const int k = lane + kW * s;
-    if (k < gc) {
-      px[s] = a.x[gs + k];
-      py[s] = a.y[gs + k];
-      pz[s] = a.z[gs + k];
-      ph[s] = !GATHER
-                  ? a.h[gs + k]
-                  : a.rscale *
-                        (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
-    }
+  for (int k = lane; k < gc; k += kW) {
+    px[k] = a.x[gs + k];
+    py[k] = a.y[gs + k];
+    pz[k] = a.z[gs + k];
+    const double r =
+        !GATHER ? a.h[gs + k]
+                : a.rscale * (a.hprop ? (double)a.hprop[gs + k] : a.h[gs + k]);
+    ph2[k] = r * r;
+    cnt[k] = 0;
   }
   double gcom[3] = {0, 0, 0}, gext_lap = 0, grad = 0;
   if (gn >= 0) {
@@ -256,57 +262,54 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     gext_lap = a.lap[gn];
     grad = a.rad[gn];
   }
-  int cnt[2] = {0, 0};
-  const int nw = (cc + 31) / 32;
-  for (int w = 0; w < nw; ++w) {
-    if (lane < 32) {
-      const int j = w * 32 + lane;
-      int ok = 0;
+  for (int r0 = 0; r0 < cc; r0 += kR) {
+    for (int i = lane; i < kR; i += kW) {
+      const int j = r0 + i;
+      double bx = __builtin_nan(""), by = 0, bz = 0, bh = 0;
       if (j < cc) {
         const int64_t b = a.cand[cbase + j];
-        const double bx = a.x[b], by = a.y[b], bz = a.z[b], bh = a.h[b];
-        cx[lane] = bx;
-        cy[lane] = by;
-        cz[lane] = bz;
-        chh[lane] = bh;
-        ok = 1;
+        by = a.y[b];
+        bz = a.z[b];
+        bh = a.h[b];
+        bx = a.x[b];
         if (gn >= 0) {
           // COM filter, tree_topology.h:365-371
           const double ext = ((bh < gext_lap) ? gext_lap : bh) + grad;
-          ok = within_distance2(bx, by, bz, gcom[0], gcom[1], gcom[2], ext);
+          if (!within_distance2(bx, by, bz, gcom[0], gcom[1], gcom[2], ext))
+            bx = __builtin_nan("");
         }
       }
-      cok[lane] = ok;
+      cx[i] = bx;
+      cy[i] = by;
+      cz[i] = bz;
+      ch2[i] = bh * bh;
     }
     __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-      const int k = lane + kW * s;
-      if (k >= gc) break;
+    const int nwr = min(kR / 32, (cc - r0 + 31) >> 5);
+    for (int q = lane; q < nwr * gc; q += kW) {
+      const int w = q / gc, k = q - w * gc;
+      const double qx = px[k], qy = py[k], qz = pz[k], qh2 = ph2[k];
       uint32_t word = 0;
-      for (int j = 0; j < 32; ++j) {
-        if (!cok[j]) continue;
+      for (int jj = 0; jj < 32; ++jj) {
+        const int j = w * 32 + jj;
+        const double d0 = qx - cx[j], d1 = qy - cy[j], d2 = qz - cz[j];
+        const double d = d0 * d0 + d1 * d1 + d2 * d2;
         bool in;
         if (GATHER) {
-          const double d0 = px[s] - cx[j], d1 = py[s] - cy[j],
-                       d2 = pz[s] - cz[j];
-          in = d0 * d0 + d1 * d1 + d2 * d2 < ph[s] * ph[s];
+          in = d < qh2;
         } else {
           // tree_topology.h:372-379: d^2 <= max(h_k, h_b)^2
-          const double r = (ph[s] < chh[j]) ? chh[j] : ph[s];
-          in = within_distance2(px[s], py[s], pz[s], cx[j], cy[j], cz[j], r);
+          in = d <= fmax(qh2, ch2[j]);
         }
-        if (in) word |= 1u << j;
+        if (in) word |= 1u << jj;
       }
-      a.mask[mo + (int64_t)w * gc + k] = word;
-      cnt[s] += __popc(word);
+      a.mask[mo + (int64_t)((r0 >> 5) + w) * gc + k] = word;
+      if (a.nngb) atomicAdd(&cnt[k], __popc(word));
     }
     __syncthreads();
   }
   if (a.nngb)
-    for (int s = 0; s < 2; ++s) {
-      const int k = lane + kW * s;
-      if (k < gc) a.nngb[gs + k] = cnt[s];
-    }
+    for (int k = lane; k < gc; k += kW) a.nngb[gs + k] = cnt[k];
 }
 
 struct TileBufs {
