@@ -293,9 +293,12 @@ __global__ __launch_bounds__(kT) void k_keys(
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort, 8 bits per pass, 4096 keys per block (256 threads x 16).
+// LSD radix sort, 8 bits per pass, 2048 keys per tile (256 threads x 8).
+// Per pass: tile digit counts, a scan of every digit's row of tile counts
+// (one workgroup per digit), and a stable scatter that adds the digits'
+// bases (a scan of the 256 row totals) itself.
 // ---------------------------------------------------------------------------
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 8;
 constexpr int kRsTile = kT * kRsItems;
 
 // Digit of a pass: 8 key bits at `shift`, or (inst given) the instance of
@@ -305,72 +308,189 @@ __device__ inline uint32_t rs_digit(uint64_t k, uint32_t v, int shift,
   return inst ? (uint32_t)inst[v] : (uint32_t)((k >> shift) & 255u);
 }
 
+// Lanes of the wavefront holding the same digit (valid lanes only).
+__device__ inline uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int bit = (d >> b) & 1;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// Digit counts of every tile: hist[d * nblocks + tile].  Lanes with equal
+// digits are matched first, so a digit costs one LDS atomic per wavefront
+// however many lanes hold it (sorted input has long runs of equal digits).
 __global__ __launch_bounds__(kT) void k_radix_hist(
     const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
     int64_t n, int shift, const int32_t *__restrict__ inst,
     int32_t *__restrict__ hist, int64_t nblocks) {
   __shared__ int32_t hs[256];
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   hs[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kRsTile;
   for (int r = 0; r < kRsItems; ++r) {
     const int64_t i = base + r * kT + threadIdx.x;
-    if (i < n) atomicAdd(&hs[rs_digit(key[i], val[i], shift, inst)], 1);
+    const bool valid = i < n;
+    const uint32_t d = valid ? rs_digit(key[i], val[i], shift, inst) : 0u;
+    const uint64_t peers = match_digit(d, valid);
+    if (valid && __popcll(peers & lt) == 0) atomicAdd(&hs[d], __popcll(peers));
   }
   __syncthreads();
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = hs[threadIdx.x];
 }
 
+// hoff[d * nblocks + b] = entries of digit d in tiles before b;
+// rowtot[d] = entries of digit d.  One workgroup per digit.
+__global__ __launch_bounds__(kT) void k_radix_rowscan(
+    const int32_t *__restrict__ hist, int64_t *__restrict__ hoff,
+    int64_t *__restrict__ rowtot, int64_t nblocks) {
+  __shared__ int64_t wsum[kT / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int32_t *row = hist + (int64_t)blockIdx.x * nblocks;
+  int64_t *orow = hoff + (int64_t)blockIdx.x * nblocks;
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nblocks; c0 += kT * 8) {
+    const int64_t b0 = c0 + (int64_t)t * 8;
+    int32_t v[8];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      v[i] = b0 + i < nblocks ? row[b0 + i] : 0;
+      s += v[i];
+    }
+    int64_t inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int64_t ex = carry + inc - s, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kT / 64; ++i) {
+      if (i < w) ex += wsum[i];
+      tot += wsum[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (b0 + i < nblocks) {
+        orow[b0 + i] = ex;
+        ex += v[i];
+      }
+    carry += tot;
+    __syncthreads();
+  }
+  if (t == 0) rowtot[blockIdx.x] = carry;
+}
+
+// Stable scatter of one pass.  Wavefront w of a tile ranks its 64 x
+// kRsItems consecutive entries by digit (ballot matching, input order);
+// the tile's entries are then placed in LDS in (digit, input order) and
+// written out, consecutive threads storing consecutive positions of a
+// digit's run.
 __global__ __launch_bounds__(kT) void k_radix_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
     int shift, const int32_t *__restrict__ inst,
-    const int64_t *__restrict__ hoff, int64_t nblocks) {
+    const int64_t *__restrict__ hoff, const int64_t *__restrict__ rowtot,
+    int64_t nblocks) {
+  __shared__ uint64_t sk[kRsTile];
+  __shared__ uint32_t sv[kRsTile];
+  __shared__ uint8_t sd[kRsTile];
   __shared__ int32_t wcnt[kT / 64][256];
-  __shared__ int32_t wbase[kT / 64][256];
-  __shared__ int32_t run[256];
+  __shared__ int32_t dstart[256];
+  __shared__ int32_t wsum[kT / 64];
+  __shared__ int64_t gsum[kT / 64];
   __shared__ int64_t gofs[256];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  gofs[t] = hoff[(int64_t)t * nblocks + blockIdx.x];
-  run[t] = 0;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * kRsTile;
+  // digit t starts at (entries of smaller digits) + (entries of digit t in
+  // earlier tiles)
+  const int64_t rt = rowtot[t];
+  int64_t ginc = rt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t u = __shfl_up(ginc, o, 64);
+    if (lane >= o) ginc += u;
+  }
+  if (lane == 63) gsum[w] = ginc;
+  const int64_t gpart = ginc - rt + hoff[(int64_t)t * nblocks + blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) wcnt[w][lane * 4 + j] = 0;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int64_t base = (int64_t)blockIdx.x * kRsTile + w * 64 * kRsItems;
+  uint64_t k[kRsItems];
+  uint32_t v[kRsItems];
+  int d[kRsItems], rk[kRsItems];
+#pragma unroll
   for (int r = 0; r < kRsItems; ++r) {
-    const int64_t i = base + r * kT + t;
-    const bool valid = i < n;
-    const uint64_t k = valid ? kin[i] : 0ull;
-    const uint32_t v = valid ? vin[i] : 0u;
-    const uint32_t d = valid ? rs_digit(k, v, shift, inst) : 0u;
-    uint64_t peers = __ballot(valid);
+    const int64_t i = base + r * 64 + lane;
+    k[r] = i < n ? kin[i] : 0ull;
+    v[r] = i < n ? vin[i] : 0u;
+  }
+  __syncthreads();
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int bit = (d >> b) & 1;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
+  for (int r = 0; r < kRsItems; ++r) {
+    const bool valid = base + r * 64 + lane < n;
+    const uint32_t dg = valid ? rs_digit(k[r], v[r], shift, inst) : 0u;
+    const uint64_t peers = match_digit(dg, valid);
     const int rank = __popcll(peers & lt);
-    const int cnt = __popcll(peers);
+    const int prev = wcnt[w][dg];
+    if (valid && rank == 0) wcnt[w][dg] = prev + __popcll(peers);
+    d[r] = valid ? (int)dg : -1;
+    rk[r] = prev + rank;
+  }
+  __syncthreads();
+  // thread t: digit t.  Tile start of the digit, then the wavefronts' bases.
+  int cw[kT / 64], tot = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) wcnt[w][lane * 4 + j] = 0;
-    __syncthreads();
-    if (valid && rank == 0) wcnt[w][d] = cnt;
-    __syncthreads();
-    {
-      int acc = run[t];
+  for (int ww = 0; ww < kT / 64; ++ww) {
+    cw[ww] = wcnt[ww][t];
+    tot += cw[ww];
+  }
+  int inc = tot;
 #pragma unroll
-      for (int ww = 0; ww < kT / 64; ++ww) {
-        wbase[ww][t] = acc;
-        acc += wcnt[ww][t];
-      }
-      run[t] = acc;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int ex = inc - tot;
+  int64_t gex = gpart;
+  for (int ww = 0; ww < w; ++ww) {
+    ex += wsum[ww];
+    gex += gsum[ww];
+  }
+  dstart[t] = ex;
+  gofs[t] = gex;
+#pragma unroll
+  for (int ww = 0; ww < kT / 64; ++ww) {
+    wcnt[ww][t] = ex;
+    ex += cw[ww];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRsItems; ++r)
+    if (d[r] >= 0) {
+      const int pos = wcnt[w][d[r]] + rk[r];
+      sk[pos] = k[r];
+      sv[pos] = v[r];
+      sd[pos] = (uint8_t)d[r];
     }
-    __syncthreads();
-    if (valid) {
-      const int64_t pos = gofs[d] + wbase[w][d] + rank;
-      kout[pos] = k;
-      vout[pos] = v;
-    }
-    __syncthreads();
+  __syncthreads();
+  const int64_t rem = n - (int64_t)blockIdx.x * kRsTile;
+  const int m = rem < kRsTile ? (int)rem : kRsTile;
+  for (int i = t; i < m; i += kT) {
+    const int dg = sd[i];
+    const int64_t pos = gofs[dg] + (i - dstart[dg]);
+    kout[pos] = sk[i];
+    vout[pos] = sv[i];
   }
 }
 
@@ -539,10 +659,11 @@ int stage_sort(Ctx *c) {
   FSPH_TRY(ensure(c, c->key2, (size_t)c->pcap * 8));
   FSPH_TRY(ensure(c, c->val2, (size_t)c->pcap * 4));
   FSPH_TRY(ensure(c, c->hist, (size_t)256 * nb * sizeof(int32_t) +
-                                  (size_t)(256 * nb + 1) * sizeof(int64_t)));
+                                  (size_t)(256 * nb + 264) * sizeof(int64_t)));
   int32_t *hist = c->hist.as<int32_t>();
   int64_t *hoff = reinterpret_cast<int64_t *>(
       c->hist.as<char>() + (((size_t)256 * nb * sizeof(int32_t) + 7) & ~(size_t)7));
+  int64_t *rowtot = hoff + 256 * nb;
   uint64_t *k0 = c->key.as<uint64_t>(), *k1 = c->key2.as<uint64_t>();
   uint32_t *v0 = c->val.as<uint32_t>(), *v1 = c->val2.as<uint32_t>();
   // 8 passes over the key bytes; in batched mode a ninth, stable pass on
@@ -554,9 +675,9 @@ int stage_sort(Ctx *c) {
     const int32_t *pi = pass == 8 ? inst : nullptr;
     k_radix_hist<<<(unsigned)nb, kT, 0, c->stream>>>(k0, v0, n, shift, pi,
                                                      hist, nb);
-    FSPH_TRY(scan_i32_to_i64(c, hist, hoff, 256 * nb));
-    k_radix_scatter<<<(unsigned)nb, kT, 0, c->stream>>>(k0, v0, k1, v1, n,
-                                                        shift, pi, hoff, nb);
+    k_radix_rowscan<<<256, kT, 0, c->stream>>>(hist, hoff, rowtot, nb);
+    k_radix_scatter<<<(unsigned)nb, kT, 0, c->stream>>>(
+        k0, v0, k1, v1, n, shift, pi, hoff, rowtot, nb);
     FSPH_HIP(c, hipGetLastError());
     std::swap(k0, k1);
     std::swap(v0, v1);

@@ -1,5 +1,6 @@
-// k_prim.hip -- device primitives: block/wave scans and a three-kernel
-// exclusive scan over int64 (reduce-then-scan, 2048 items per block).
+// k_prim.hip -- device primitives: block/wave scans and an exclusive scan
+// over int64: one workgroup for up to 16,384 items, otherwise three kernels
+// (reduce-then-scan, 2048 items per block).
 #include "fsph_internal.h"
 
 namespace fsph {
@@ -91,11 +92,51 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(
     out[n] = partial[gridDim.x];
 }
 
+// One-workgroup scan: thread t sums its kSmallItems consecutive items, the
+// workgroup scans the sums, and thread t writes its items' prefixes (each
+// item is read before its prefix is written, so in may equal out).
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallItems = 16;
+constexpr int64_t kSmallMax = (int64_t)kSmallThreads * kSmallItems;
+
+template <class In>
+__global__ __launch_bounds__(kSmallThreads) void k_scan_small(
+    const In *in, int64_t n, int64_t *out) {
+  __shared__ int64_t wsum[kSmallThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t base = (int64_t)t * kSmallItems;
+  int64_t s = 0;
+#pragma unroll 16
+  for (int i = 0; i < kSmallItems; ++i)
+    if (base + i < n) s += (int64_t)in[base + i];
+  const int64_t inc = wave_inclusive_scan(s);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int64_t ex = inc - s, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kSmallThreads / 64; ++i) {
+    if (i < w) ex += wsum[i];
+    tot += wsum[i];
+  }
+#pragma unroll 16
+  for (int i = 0; i < kSmallItems; ++i)
+    if (base + i < n) {
+      const int64_t x = (int64_t)in[base + i];
+      out[base + i] = ex;
+      ex += x;
+    }
+  if (t == 0) out[n] = tot;
+}
+
 template <class In>
 int scan_impl(Ctx *c, const In *in, int64_t *out, int64_t n) {
   if (n <= 0) {
     FSPH_HIP(c, hipMemsetAsync(out, 0, sizeof(int64_t), c->stream));
     return 0;
+  }
+  if (n <= kSmallMax) {
+    k_scan_small<In><<<1, kSmallThreads, 0, c->stream>>>(in, n, out);
+    return hip_check(c, hipGetLastError(), "scan");
   }
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
   FSPH_TRY(ensure(c, c->scan_tmp, (size_t)(nb + 1) * sizeof(int64_t)));
