@@ -63,6 +63,14 @@ int ensure_particles(Ctx *c, int64_t cap, bool keep) {
   return 0;
 }
 
+// Copies the first `words` words of the device mailbox to the host and waits.
+int read_mail(Ctx *c, int words) {
+  FSPH_HIP(c, hipMemcpyAsync(c->mail_h, c->mail.p, (size_t)words * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int read_flags(Ctx *c, int *flags) {
   FSPH_HIP(c, hipMemcpyAsync(flags, c->flags.p, sizeof(int),
                              hipMemcpyDeviceToHost, c->stream));
@@ -87,13 +95,7 @@ int do_update_iteration(Ctx *c) {
     PhaseTimer t(c, FSPH_T_SORT);
     FSPH_TRY(stage_sort(c));
   }
-  int fl = 0;
-  FSPH_TRY(read_flags(c, &fl));
-  if (fl & kFlagDupKey)
-    return fail(c, FSPH_E_DUPKEY,
-                "two particles share a Morton key (positions closer than "
-                "range/2^21)");
-  {
+  {  // (stage_tree fails on the sort's duplicate-key flag)
     PhaseTimer t(c, FSPH_T_TREE);
     FSPH_TRY(stage_tree(c));
     FSPH_TRY(stage_groups(c));
@@ -301,7 +303,13 @@ int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
     delete c;
     return FSPH_E_HIP;
   }
-  if (fsph::ensure(c, c->flags, 64)) {
+  if (fsph::ensure(c, c->flags, 64) ||
+      fsph::ensure(c, c->mail, fsph::kMail * sizeof(int64_t)) ||
+      hipHostMalloc(reinterpret_cast<void **>(&c->mail_h),
+                    fsph::kMail * sizeof(int64_t), hipHostMallocDefault) !=
+          hipSuccess) {
+    if (c->flags.p) hipFree(c->flags.p);
+    if (c->mail.p) hipFree(c->mail.p);
     hipStreamDestroy(c->stream);
     delete c;
     return FSPH_E_OOM;
@@ -318,6 +326,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   hipStreamSynchronize(c->stream);
   fsph::DBuf *all[] = {&c->key, &c->key2, &c->val, &c->val2, &c->perm,
                        &c->hist, &c->scan_tmp, &c->red_tmp, &c->flags,
+                       &c->mail, &c->range_d,
                        &c->L, &c->pair_cnt, &c->pair_off, &c->nkey,
                        &c->ndepth, &c->nstart, &c->ncount, &c->nparent,
                        &c->nchild, &c->nbox, &c->ncom, &c->nmass, &c->nrad,
@@ -344,6 +353,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   }
   for (int i = 0; i < 2 * Ctx::kEvPool; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->mail_h) hipHostFree(c->mail_h);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -597,7 +607,6 @@ int fsph_set_dt(fsph_ctx *ctx, double dt) {
 int fsph_reductions(fsph_ctx *ctx, double out[10]) {
   CTX_OR_FAIL;
   NOT_BATCH;
-  NOT_BATCH;
   if (!out) return FSPH_E_ARG;
   return fsph::stage_reductions(c, out);
 }
@@ -605,6 +614,17 @@ int fsph_reductions(fsph_ctx *ctx, double out[10]) {
 int fsph_get_range(fsph_ctx *ctx, double lo[3], double hi[3]) {
   CTX_OR_FAIL;
   NOT_BATCH;
+  if (c->range_pending) {
+    double r[6];
+    FSPH_HIP(c, hipMemcpyAsync(r, c->range_d.p, sizeof(r),
+                               hipMemcpyDeviceToHost, c->stream));
+    FSPH_HIP(c, hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; ++d) {
+      c->lo[d] = r[d];
+      c->hi[d] = r[3 + d];
+    }
+    c->range_pending = false;
+  }
   for (int d = 0; d < 3; ++d) {
     lo[d] = c->lo[d];
     hi[d] = c->hi[d];

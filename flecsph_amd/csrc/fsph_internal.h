@@ -17,6 +17,7 @@
 namespace fsph {
 
 constexpr int kMaxDepth = 21;     // filling_curve.h:44-47 (64-bit keys, 3D)
+constexpr int kMail = 64;         // words of the device/host mailbox
 constexpr int32_t kChildEmpty = -1;
 // child codes: >= 0 node index; <= -2 leaf particle p stored as -(p+2)
 __host__ __device__ inline int32_t leaf_code(int64_t p) {
@@ -97,7 +98,10 @@ struct Ctx {
   DBuf scan_tmp;
   DBuf red_tmp;        // reduction partials
   DBuf flags;          // int error flags
-  DBuf host_flags_dummy;
+  DBuf mail;           // device mailbox: scalars read back in one copy
+  int64_t *mail_h = nullptr;  // its pinned host copy (kMail words)
+  DBuf range_d;        // range of the last key computation: lo[3], hi[3]
+  bool range_pending = false;  // range_d newer than lo/hi
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 
   // tree
@@ -290,7 +294,9 @@ int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
 int stage_update_h(Ctx *c);            // k_step.hip
 int stage_dt_min(Ctx *c, double *dtmin);     // k_step.hip
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
-int stage_max_h(Ctx *c, double *hmax);       // k_step.hip
+// device pointer to 2.5 max h (the periodic halo), no host wait
+int stage_halo(Ctx *c, const double **halo);  // k_step.hip
+int read_mail(Ctx *c, int words);             // fsph_api.hip
 int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
                       int64_t n);              // k_step.hip
 // Batched mode (k_batch.hip): per-instance reductions over the particles

@@ -104,13 +104,14 @@ __device__ inline int copies_of(int on) {
   return a + bb + cc + (a & bb) + (a & cc) + (cc & bb) + (a & bb & cc);
 }
 
-// halo of the particle's instance (batched mode) or the one halo
+// halo of the particle's instance (batched mode) or the one halo (device
+// memory, written by the h_max reduction)
 struct Halo {
-  double h;
+  const double *h;
   const double *per_inst;
   const int32_t *inst;
   __device__ double of(int64_t i) const {
-    return inst ? per_inst[inst[i]] : h;
+    return inst ? per_inst[inst[i]] : *h;
   }
 };
 
@@ -238,7 +239,8 @@ __global__ __launch_bounds__(kT) void k_range_partial(
 }
 
 __global__ __launch_bounds__(kT) void k_range_final(double *__restrict__ part,
-                                                    int nb) {
+                                                    int nb,
+                                                    double *__restrict__ keep) {
   __shared__ double s[6][kT];
   for (int d = 0; d < 6; ++d) {
     double v = d < 3 ? INFINITY : -INFINITY;
@@ -257,7 +259,10 @@ __global__ __launch_bounds__(kT) void k_range_final(double *__restrict__ part,
     __syncthreads();
   }
   // result at the end of the partial array: lo[3], hi[3]
-  if (threadIdx.x < 6) part[nb * 6 + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < 6) {
+    part[nb * 6 + threadIdx.x] = s[threadIdx.x][0];
+    keep[threadIdx.x] = s[threadIdx.x][0];
+  }
 }
 
 // range: lo[3], hi[3] (one simulation) or 8 doubles per instance
@@ -551,7 +556,7 @@ int stage_boundary(Ctx *c) {
   }
   // getSmoothinglength (bodies_system.h:112-124): halo = 2.5 h_max (of
   // each instance in batched mode).
-  Halo hl{0.0, nullptr, nullptr};
+  Halo hl{nullptr, nullptr, nullptr};
   if (c->batch.n_inst) {
     FSPH_TRY(batch_reduce(c, 0));
     const int ni = c->batch.n_inst;
@@ -565,9 +570,7 @@ int stage_boundary(Ctx *c) {
     hl.per_inst = c->batch.halo.as<double>();
     hl.inst = c->f[PF_INST].as<int32_t>();
   } else {
-    double hmax = 0;
-    FSPH_TRY(stage_max_h(c, &hmax));
-    hl.h = 2.5 * hmax;
+    FSPH_TRY(stage_halo(c, &hl.h));
   }
   Box b{};
   for (int d = 0; d < 3; ++d) {
@@ -634,23 +637,16 @@ int stage_range_keys(Ctx *c) {
   k_range_partial<<<nb, kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
       c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(), n, part);
-  k_range_final<<<1, kT, 0, c->stream>>>(part, nb);
+  FSPH_TRY(ensure(c, c->range_d, 6 * sizeof(double)));
+  k_range_final<<<1, kT, 0, c->stream>>>(part, nb, c->range_d.as<double>());
   FSPH_TRY(ensure(c, c->key, (size_t)c->pcap * 8));
   FSPH_TRY(ensure(c, c->val, (size_t)c->pcap * 4));
   k_keys<<<blocks_for(n), kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
       c->f[PF_Z].as<double>(), n, part + nb * 6, nullptr,
       c->key.as<uint64_t>(), c->val.as<uint32_t>());
-  FSPH_HIP(c, hipGetLastError());
-  double r[6];
-  FSPH_HIP(c, hipMemcpyAsync(r, part + nb * 6, sizeof(r), hipMemcpyDeviceToHost,
-                             c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  for (int d = 0; d < 3; ++d) {
-    c->lo[d] = r[d];
-    c->hi[d] = r[3 + d];
-  }
-  return 0;
+  c->range_pending = true;  // read back on request (fsph_get_range)
+  return hip_check(c, hipGetLastError(), "k_keys");
 }
 
 int stage_sort(Ctx *c) {

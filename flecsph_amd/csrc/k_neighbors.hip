@@ -203,6 +203,16 @@ __global__ __launch_bounds__(256) void k_mask_words(
   }
 }
 
+// Mailbox of the tile stage: [0] error flags, [1] mask words.
+__global__ void k_tile_mail(const int *__restrict__ flags,
+                            const int64_t *__restrict__ words,
+                            int64_t *__restrict__ mail) {
+  if (threadIdx.x == 0) {
+    mail[0] = *flags;
+    mail[1] = *words;
+  }
+}
+
 struct MaskArgs {
   const double *x, *y, *z, *h;
   const double *com, *rad, *lap;
@@ -343,8 +353,18 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
       k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
     }
     FSPH_HIP(c, hipGetLastError());
-    int fl = 0;
-    FSPH_TRY(read_flags(c, &fl));
+    // mask offsets (an overflowed traversal leaves zero candidates), then
+    // the flags and the word count in one read-back
+    k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
+        tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), glist, ng,
+        tb.moff.as<int64_t>());
+    FSPH_HIP(c, hipGetLastError());
+    FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), ng));
+    k_tile_mail<<<1, 64, 0, c->stream>>>(c->flags.as<int>(),
+                                         tb.moff.as<int64_t>() + ng,
+                                         c->mail.as<int64_t>());
+    FSPH_TRY(read_mail(c, 2));
+    const int fl = (int)c->mail_h[0];
     if (fl & kFlagStack)
       return fail(c, FSPH_E_CAP, "neighbour traversal stack overflow");
     if (fl & kFlagCand) {
@@ -356,15 +376,7 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
     }
     break;
   }
-  k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
-      tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), glist, ng,
-      tb.moff.as<int64_t>());
-  FSPH_HIP(c, hipGetLastError());
-  FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), ng));
-  int64_t words = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&words, tb.moff.as<int64_t>() + ng, 8,
-                             hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  const int64_t words = c->mail_h[1];
   tb.words = words;
   FSPH_TRY(ensure(c, tb.mask, (size_t)(words > 0 ? words : 1) * 4));
   MaskArgs m{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),

@@ -157,7 +157,10 @@ __global__ __launch_bounds__(kT) void k_max_final(double *part, int nb) {
       s[threadIdx.x] = s[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[nb] = s[0];
+  if (threadIdx.x == 0) {
+    part[nb] = s[0];
+    part[nb + 1] = 2.5 * s[0];  // getSmoothinglength's halo
+  }
 }
 
 template <class T> struct RedArgs {
@@ -585,17 +588,14 @@ int stage_reductions(Ctx *c, double out[10]) {
   return c->rsize == 4 ? red_impl<float>(c, out) : red_impl<double>(c, out);
 }
 
-int stage_max_h(Ctx *c, double *hmax) {
+int stage_halo(Ctx *c, const double **halo) {
   const int nb = (int)std::min<int64_t>(1024, (c->n + kT - 1) / kT);
-  FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 8));
+  FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 2) * 8));
   double *part = c->red_tmp.as<double>();
   k_max_partial<<<nb, kT, 0, c->stream>>>(c->f[PF_H].as<double>(), c->n, part);
   k_max_final<<<1, kT, 0, c->stream>>>(part, nb);
-  FSPH_HIP(c, hipGetLastError());
-  FSPH_HIP(c, hipMemcpyAsync(hmax, part + nb, 8, hipMemcpyDeviceToHost,
-                             c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  return 0;
+  *halo = part + nb + 1;
+  return hip_check(c, hipGetLastError(), "k_max_final");
 }
 
 int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,

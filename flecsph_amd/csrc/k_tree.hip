@@ -92,13 +92,42 @@ __global__ __launch_bounds__(kT) void k_prefix(const uint64_t *__restrict__ k,
   L[i] = d > kMaxDepth - 1 ? kMaxDepth - 1 : d;
 }
 
-__global__ __launch_bounds__(kT) void k_pair_count(const int32_t *__restrict__ L,
-                                                   int64_t n,
-                                                   int32_t *__restrict__ cnt) {
+// Nodes owned by entry i (depths Lm1(i)+1 .. L[i]) and, summed over the
+// grid, the number of nodes at each depth.
+__global__ __launch_bounds__(kT) void k_pair_count(
+    const int32_t *__restrict__ L, int64_t n, int32_t *__restrict__ cnt,
+    unsigned long long *__restrict__ lvl) {
+  __shared__ int32_t h[kMaxDepth + 1];
+  if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  const int v = L[i] - Lm1(L, i);
-  cnt[i] = v > 0 ? v : 0;
+  if (i < n) {
+    const int d0 = Lm1(L, i) + 1, d1 = L[i];
+    cnt[i] = d1 >= d0 ? d1 - d0 + 1 : 0;
+    for (int d = d0; d <= d1; ++d) atomicAdd(&h[d], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x <= kMaxDepth && h[threadIdx.x])
+    atomicAdd(&lvl[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// Mailbox of the tree stage: [0] nodes, [1] error flags, [2 + d] nodes at
+// depth d.  cursor[d] = nodes at depths above d: where the level order
+// places depth d.
+__global__ void k_tree_mail(const int64_t *__restrict__ nn,
+                            const int *__restrict__ flags,
+                            const unsigned long long *__restrict__ lvl,
+                            unsigned long long *__restrict__ cursor,
+                            int64_t *__restrict__ mail) {
+  if (threadIdx.x != 0) return;
+  mail[0] = *nn;
+  mail[1] = *flags;
+  unsigned long long acc = 0;
+  for (int d = 0; d <= kMaxDepth; ++d) {
+    mail[2 + d] = (int64_t)lvl[d];
+    cursor[d] = acc;
+    acc += lvl[d];
+  }
 }
 
 struct NodeOut {
@@ -156,20 +185,6 @@ __global__ __launch_bounds__(kT) void k_leaves(const uint64_t *__restrict__ k,
                               : containing_node(k, sg, off, L, i, D);
   child[par * 8 + digit_at(k[i], D + 1)] = leaf_code(i);
   leaf_parent[i] = (int32_t)par;
-}
-
-__global__ __launch_bounds__(kT) void k_level_hist(const int32_t *__restrict__ depth,
-                                                   int64_t nn,
-                                                   int64_t *__restrict__ lvl) {
-  __shared__ int32_t h[kMaxDepth + 1];
-  if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < nn) atomicAdd(&h[depth[i]], 1);
-  __syncthreads();
-  if (threadIdx.x <= kMaxDepth && h[threadIdx.x])
-    atomicAdd((unsigned long long *)&lvl[threadIdx.x],
-              (unsigned long long)h[threadIdx.x]);
 }
 
 // Counting-sort scatter of node ids by depth: positions are reserved per
@@ -334,16 +349,37 @@ int stage_tree(Ctx *c) {
   int32_t *L = c->L.as<int32_t>();
   const Seg sg{c->batch.n_inst ? c->f[PF_INST].as<int32_t>() : nullptr,
                c->batch.seg.as<int64_t>(), n};
+  // level histogram and the level order's cursors
+  FSPH_TRY(ensure(c, c->red_tmp, 2 * (kMaxDepth + 2) * 8));
+  unsigned long long *lvl = c->red_tmp.as<unsigned long long>();
+  unsigned long long *cursor = lvl + (kMaxDepth + 2);
+  FSPH_HIP(c, hipMemsetAsync(lvl, 0, (kMaxDepth + 2) * 8, c->stream));
   k_prefix<<<blocks_for(n), kT, 0, c->stream>>>(k, n, sg.inst, L);
-  k_pair_count<<<blocks_for(n), kT, 0, c->stream>>>(L, n,
-                                                    c->pair_cnt.as<int32_t>());
+  k_pair_count<<<blocks_for(n), kT, 0, c->stream>>>(
+      L, n, c->pair_cnt.as<int32_t>(), lvl);
   FSPH_HIP(c, hipGetLastError());
   int64_t *off = c->pair_off.as<int64_t>();
   FSPH_TRY(scan_i32_to_i64(c, c->pair_cnt.as<int32_t>(), off, n));
-  int64_t nn = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&nn, off + n, 8, hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  k_tree_mail<<<1, 64, 0, c->stream>>>(off + n, c->flags.as<int>(), lvl,
+                                       cursor, c->mail.as<int64_t>());
+  FSPH_TRY(read_mail(c, 2 + kMaxDepth + 1));
+  if (c->mail_h[1] & kFlagDupKey)
+    return fail(c, FSPH_E_DUPKEY,
+                "two particles share a Morton key (positions closer than "
+                "range/2^21)");
+  const int64_t nn = c->mail_h[0];
   c->nn = nn;
+  {
+    int64_t acc = 0;
+    c->max_depth = 0;
+    for (int d = 0; d <= kMaxDepth; ++d) {
+      const int64_t cd = c->mail_h[2 + d];
+      c->level_off[d] = acc;
+      acc += cd;
+      if (cd) c->max_depth = d;
+    }
+    c->level_off[kMaxDepth + 1] = acc;
+  }
   if (nn > c->ncap) {
     const int64_t cap = (int64_t)(nn * 1.25) + 1024;
     FSPH_TRY(ensure(c, c->nkey, cap * 8));
@@ -370,27 +406,6 @@ int stage_tree(Ctx *c) {
                                                 c->leaf_parent.as<int32_t>());
   FSPH_HIP(c, hipGetLastError());
   // level order
-  FSPH_TRY(ensure(c, c->red_tmp, 2 * (kMaxDepth + 2) * 8));
-  int64_t *lvl = c->red_tmp.as<int64_t>();
-  FSPH_HIP(c, hipMemsetAsync(lvl, 0, (kMaxDepth + 2) * 8, c->stream));
-  k_level_hist<<<blocks_for(nn), kT, 0, c->stream>>>(c->ndepth.as<int32_t>(),
-                                                     nn, lvl);
-  int64_t cnt[kMaxDepth + 1];
-  FSPH_HIP(c, hipMemcpyAsync(cnt, lvl, sizeof(cnt), hipMemcpyDeviceToHost,
-                             c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  int64_t acc = 0;
-  c->max_depth = 0;
-  for (int d = 0; d <= kMaxDepth; ++d) {
-    c->level_off[d] = acc;
-    acc += cnt[d];
-    if (cnt[d]) c->max_depth = d;
-  }
-  c->level_off[kMaxDepth + 1] = acc;
-  unsigned long long *cursor =
-      reinterpret_cast<unsigned long long *>(lvl + (kMaxDepth + 2));
-  FSPH_HIP(c, hipMemcpyAsync(cursor, c->level_off, (kMaxDepth + 1) * 8,
-                             hipMemcpyHostToDevice, c->stream));
   k_level_scatter<<<blocks_for(nn), kT, 0, c->stream>>>(
       c->ndepth.as<int32_t>(), nn, cursor, c->nlevel.as<int32_t>());
   FSPH_HIP(c, hipGetLastError());
