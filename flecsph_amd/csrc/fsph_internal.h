@@ -118,6 +118,7 @@ struct Ctx {
   int64_t ng = 0;
   DBuf gstart, gcount, gnode, gflag, gscan;
   int64_t ccap = 2048;       // candidate capacity per group
+  bool big_stack = false;    // traversal with the large LDS stack
   int chunk = 128;           // candidates per LDS round of the pair kernels
   DBuf cand, ccount, moff;   // cand: ng*ccap int32; ccount int32; moff int64
   DBuf mask;                 // uint32 words

@@ -22,7 +22,10 @@ namespace fsph {
 namespace {
 
 constexpr int kW = 64;          // one wavefront per group
-constexpr int kStack = 4096;    // LDS traversal stack (node indices)
+// LDS traversal stack (node indices): the small one keeps more groups
+// resident per CU; a traversal that overflows it is run again with the
+// large one (and the context keeps the large one from then on).
+constexpr int kStackSmall = 1024, kStackLarge = 4096;
 constexpr int kMaxGroup = 128;  // tree_topology::sub_entities_
 
 __device__ inline bool box_box(const double *a, const double *b) {
@@ -87,6 +90,7 @@ struct TravArgs {
   const int32_t *inst, *root;
 };
 
+template <int kStack>
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
 #pragma clang fp contract(off)
   __shared__ double sx[kMaxGroup], sy[kMaxGroup], sz[kMaxGroup],
@@ -350,7 +354,10 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                c->batch.n_inst ? c->batch.root.as<int32_t>() : nullptr};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
-      k_traverse<<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      if (c->big_stack)
+        k_traverse<kStackLarge><<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      else
+        k_traverse<kStackSmall><<<(unsigned)ng, kW, 0, c->stream>>>(t);
     }
     FSPH_HIP(c, hipGetLastError());
     // mask offsets (an overflowed traversal leaves zero candidates), then
@@ -365,8 +372,12 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                                          c->mail.as<int64_t>());
     FSPH_TRY(read_mail(c, 2));
     const int fl = (int)c->mail_h[0];
-    if (fl & kFlagStack)
-      return fail(c, FSPH_E_CAP, "neighbour traversal stack overflow");
+    if (fl & kFlagStack) {
+      if (c->big_stack)
+        return fail(c, FSPH_E_CAP, "neighbour traversal stack overflow");
+      c->big_stack = true;
+      continue;
+    }
     if (fl & kFlagCand) {
       if (attempt >= 6)
         return fail(c, FSPH_E_CAP, "candidate tile overflow (cap %lld)",

@@ -30,6 +30,11 @@ def cases():
             kernel=_abi.KERNEL_WENDLAND_C6),
         "sedov16_periodic": generators.sedov(n_side=16),
         "sod_small": generators.sod(scale=1),
+        # about 1500 neighbours each: a group's traversal accepts most of
+        # the tree, which overflows the small LDS stack (the traversal is
+        # run again with the large one) and the default candidate capacity
+        "random_dense": generators.random_box(
+            6000, seed=103, h_spread=0.5, n_ngb=1000),
     }
 
 
