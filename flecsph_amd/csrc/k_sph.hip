@@ -527,29 +527,80 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   }
   KSum<T> tot[NS];  // thread t: particle t's totals
   T vs = 0, dv = 0;
+  // The next chunk's candidates and mask words are loaded into registers
+  // while the current chunk is walked, and stored to LDS after it.
+  constexpr int PER = CH > kB ? CH / kB : 1, MPER = CH / 32;
+  int32_t pidx[PER], rid[PER];
+  double rx[PER], ry[PER], rz[PER], rvhx[PER], rvhy[PER], rvhz[PER];
+  double rvx[PER], rvy[PER], rvz[PER], rh[PER], rm[PER];
+  T rrho[PER], rP[PER], rc[PER], rco[PER];
+  uint32_t pm[MPER];
+  auto load_idx = [&](int c0) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int j = t + q * kB;
+      pidx[q] = (j < CH && c0 + j < c.cc) ? a.t.cand[c.cb + c0 + j] : -1;
+    }
+  };
+  auto gather = [&](int c0) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int32_t b = pidx[q];
+      rid[q] = b;
+      if (b < 0) continue;
+      rx[q] = a.x[b]; ry[q] = a.y[b]; rz[q] = a.z[b];
+      rvhx[q] = a.vhx[b]; rvhy[q] = a.vhy[b]; rvhz[q] = a.vhz[b];
+      if (V) {
+        rvx[q] = a.vx[b]; rvy[q] = a.vy[b]; rvz[q] = a.vz[b];
+      }
+      rh[q] = a.h[b]; rm[q] = a.m[b];
+      rrho[q] = a.rho[b]; rP[q] = a.P[b]; rc[q] = a.c[b];
+      if (VS) rco[q] = a.c_old[b];
+    }
+    const int nw = (min(CH, c.cc - c0) + 31) >> 5;
+    const uint32_t *src = a.t.mask + c.mo + (int64_t)(c0 >> 5) * c.gc;
+#pragma unroll
+    for (int q = 0; q < MPER; ++q) {
+      const int w = t + q * kB;
+      pm[q] = w < nw * c.gc ? src[w] : 0u;
+    }
+  };
+  load_idx(0);
+  gather(0);
+  if (CH < c.cc) load_idx(CH);
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = t; j < nch; j += kB) {
-      const int64_t b = a.t.cand[c.cb + c0 + j];
-      sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
-      sz[j] = (P)(a.z[b] - oz);
-      svhx[j] = (T)a.vhx[b]; svhy[j] = (T)a.vhy[b]; svhz[j] = (T)a.vhz[b];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int j = t + q * kB;
+      if (j >= nch) continue;
+      sx[j] = (P)(rx[q] - ox); sy[j] = (P)(ry[q] - oy);
+      sz[j] = (P)(rz[q] - oz);
+      svhx[j] = (T)rvhx[q]; svhy[j] = (T)rvhy[q]; svhz[j] = (T)rvhz[q];
       if (V) {
-        svx[j] = (T)a.vx[b]; svy[j] = (T)a.vy[b]; svz[j] = (T)a.vz[b];
+        svx[j] = (T)rvx[q]; svy[j] = (T)rvy[q]; svz[j] = (T)rvz[q];
       }
       if (VS) {
-        sco[j] = a.c_old[b];
-        sid[j] = (int32_t)b;
+        sco[j] = rco[q];
+        sid[j] = rid[q];
       }
-      sh[j] = (T)a.h[b];
-      sm[j] = (T)a.m[b];
-      const T rb = a.rho[b];
+      sh[j] = (T)rh[q];
+      sm[j] = (T)rm[q];
+      const T rb = rrho[q];
       srho[j] = rb;
-      sprho2[j] = a.P[b] / (rb * rb);
-      sc[j] = a.c[b];
+      sprho2[j] = rP[q] / (rb * rb);
+      sc[j] = rc[q];
     }
-    stage_masks(a.t, c, c0, nw, smask);
+#pragma unroll
+    for (int q = 0; q < MPER; ++q) {
+      const int w = t + q * kB;
+      if (w < nw * c.gc) smask[w] = pm[q];
+    }
     __syncthreads();
+    if (c0 + CH < c.cc) {
+      gather(c0 + CH);
+      if (c0 + 2 * CH < c.cc) load_idx(c0 + 2 * CH);
+    }
     const int np = schedule(S, smask, nw, c.gc);
     P px = 0, py = 0, pz = 0;
     T pvhx = 0, pvhy = 0, pvhz = 0, pvx = 0, pvy = 0, pvz = 0;
