@@ -5,7 +5,11 @@ box (Wendland C6, 100 neighbours, FP32 pair sums with FP64 integrator, keys
 and tree), one full driver iteration per step: leapfrog kick/drift, periodic
 copies, range/keys/sort, octree, neighbour tiles, density/EOS/signal speed,
 acceleration, du/dt, smoothing length and time step.  Inputs are resident in
-HBM before the timed region; WALL copies are not counted in N.
+HBM before the timed region; WALL copies are not counted in N.  Before the
+warm-up the library's autotuner picks the leaf-group width and LDS chunk
+(untimed; it times every candidate from the same saved state and restores
+it, and neighbour sets do not depend on the choice) unless --no-autotune is
+given; the choice is reported in config.tuning.
 
 Two further measurements ride along in the same JSON line:
   "learned_h": Newton-h against the learned smoothing length on the cold
@@ -104,6 +108,10 @@ def parse():
     ap.add_argument("--lh-steps", type=int, default=200)
     ap.add_argument("--lh-learn", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep group_max 128 and chunk 128 instead of "
+                         "autotuning them before the warm-up")
+    ap.add_argument("--tune-steps", type=int, default=3)
     ap.add_argument("--batch-inst", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
@@ -177,7 +185,7 @@ def batched_run(n_inst, steps):
     b = BatchSystem(_abi.default_params(**prm), n_inst)
     b.upload(sets)
     b.step()
-    gm, ch = autotune(b, steps=1)
+    gm, ch = autotune(b, steps=2)
     log(f"batched: autotune chose group_max {gm}, chunk {ch}")
     b.enable_graph(True)
     b.step()
@@ -291,11 +299,21 @@ def main():
                          "sedov_batch64"):
         side_workload(args)
         return
-    from flecsph_amd import BodySystem, _abi
+    from flecsph_amd import BodySystem, _abi, autotune
 
     parts, prm, label, step_bytes = workload(args.workload)
     g = BodySystem(_abi.default_params(**prm), device=0)
     g.upload(parts)
+    tuning = {"group_max": 128, "chunk": 128}
+    if not args.no_autotune:
+        # untimed: each (group_max, chunk) candidate is timed over
+        # tune_steps steps from the state after the initial iteration, which
+        # the autotuner restores afterwards (the warm-up then continues from
+        # there)
+        gm, ch = autotune(g, steps=args.tune_steps)
+        tuning = {"group_max": gm, "chunk": ch,
+                  "tune_steps_per_candidate": args.tune_steps}
+        log(f"autotune chose group_max {gm}, chunk {ch}")
     for _ in range(args.warmup):
         g.step()
     g.enable_timing(True)
@@ -350,6 +368,7 @@ def main():
             "kernel": "wendland_c6", "n_ngb": 100,
             "arithmetic": "FP32 pair sums (compensated), FP64 keys/tree/"
                           "integrator/reductions",
+            "tuning": tuning,
         },
         "roofline": {
             "bound": "hbm",

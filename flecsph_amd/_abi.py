@@ -161,6 +161,7 @@ PROTOTYPES = {
     "instance_download": (C.c_int, [_P, C.c_int, C.c_int, _P]),
     "instance_time": (C.c_int, [_P, C.c_int, _PI64, _PD, _PD]),
     "autotune": (C.c_int, [_P, C.c_int, _PI32, _PI32]),
+    "set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "enable_graph": (C.c_int, [_P, C.c_int]),
 }
 

@@ -45,6 +45,93 @@ int batch_alloc(Ctx *c, int ni) {
   return 0;
 }
 
+// State of a simulation that a step changes: every particle field (first n
+// entries), the counts, the clocks and, in batched mode, the instances'
+// clocks and segments.  The autotuner restores it before timing each
+// candidate, so all candidates are timed on the same steps.
+struct Snapshot {
+  fsph::DBuf f[fsph::PF_COUNT];
+  int64_t n = 0, n_real = 0, iteration = 0;
+  double dt = 0, totaltime = 0, totaltime_next = 0;
+  bool initial_done = false, dt_ready = false;
+  std::vector<double> dt_h, t_h, tnext_h;
+  std::vector<int64_t> seg_h;
+  fsph::DBuf seg, bdt;
+};
+
+int snap_take(Ctx *c, Snapshot &s) {
+  for (int q = 0; q < fsph::PF_COUNT; ++q) {
+    const size_t b = (size_t)c->n * fsph::field_size(c, q);
+    if (!c->f[q].p || !b) continue;
+    FSPH_TRY(fsph::ensure(c, s.f[q], b));
+    FSPH_HIP(c, hipMemcpyAsync(s.f[q].p, c->f[q].p, b,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
+  s.n = c->n;
+  s.n_real = c->n_real;
+  s.iteration = c->iteration;
+  s.dt = c->dt;
+  s.totaltime = c->totaltime;
+  s.totaltime_next = c->totaltime_next;
+  s.initial_done = c->initial_done;
+  auto &B = c->batch;
+  if (B.n_inst) {
+    s.dt_h = B.dt_h;
+    s.t_h = B.t_h;
+    s.tnext_h = B.tnext_h;
+    s.seg_h = B.seg_h;
+    s.dt_ready = B.dt_ready;
+    const size_t ni = (size_t)B.n_inst;
+    FSPH_TRY(fsph::ensure(c, s.seg, (ni + 1) * 8));
+    FSPH_TRY(fsph::ensure(c, s.bdt, ni * 8));
+    FSPH_HIP(c, hipMemcpyAsync(s.seg.p, B.seg.p, (ni + 1) * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+    FSPH_HIP(c, hipMemcpyAsync(s.bdt.p, B.dt.p, ni * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int snap_restore(Ctx *c, const Snapshot &s) {
+  for (int q = 0; q < fsph::PF_COUNT; ++q) {
+    if (!s.f[q].p) continue;
+    FSPH_HIP(c, hipMemcpyAsync(c->f[q].p, s.f[q].p,
+                               (size_t)s.n * fsph::field_size(c, q),
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->n = s.n;
+  c->n_real = s.n_real;
+  c->iteration = s.iteration;
+  c->dt = s.dt;
+  c->totaltime = s.totaltime;
+  c->totaltime_next = s.totaltime_next;
+  c->initial_done = s.initial_done;
+  c->have_tree = false;
+  auto &B = c->batch;
+  if (B.n_inst) {
+    B.dt_h = s.dt_h;
+    B.t_h = s.t_h;
+    B.tnext_h = s.tnext_h;
+    B.seg_h = s.seg_h;
+    B.dt_ready = s.dt_ready;
+    const size_t ni = (size_t)B.n_inst;
+    FSPH_HIP(c, hipMemcpyAsync(B.seg.p, s.seg.p, (ni + 1) * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+    FSPH_HIP(c, hipMemcpyAsync(B.dt.p, s.bdt.p, ni * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void snap_free(Snapshot &s) {
+  for (auto &b : s.f)
+    if (b.p) hipFree(b.p);
+  if (s.seg.p) hipFree(s.seg.p);
+  if (s.bdt.p) hipFree(s.bdt.p);
+}
+
 #define BATCH_OR_FAIL                                             \
   Ctx *c = reinterpret_cast<Ctx *>(ctx);                          \
   if (!c) return FSPH_E_ARG;                                      \
@@ -194,27 +281,51 @@ int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
   static const int kChunks[] = {64, 128, 256};
   double best = 1e300;
   int bg = c->p.group_max, bc = c->chunk;
-  for (int gm : kGroup)
-    for (int ch : kChunks) {
-      c->p.group_max = gm;
-      c->chunk = ch;
-      FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  Snapshot snap;
+  int rc = snap_take(c, snap);
+  for (int gi = 0; gi < 3 && !rc; ++gi)
+    for (int ci = 0; ci < 3 && !rc; ++ci) {
+      c->p.group_max = kGroup[gi];
+      c->chunk = kChunks[ci];
+      // one untimed step (buffers grow, graphs are captured), then the
+      // timed steps from the same state
+      rc = snap_restore(c, snap);
+      if (!rc) rc = fsph_step(ctx);
+      if (!rc) rc = snap_restore(c, snap);
+      if (rc) break;
       const auto t0 = std::chrono::steady_clock::now();
-      for (int s = 0; s < steps; ++s) FSPH_TRY(fsph_step(ctx));
-      FSPH_HIP(c, hipStreamSynchronize(c->stream));
+      for (int s = 0; s < steps && !rc; ++s) rc = fsph_step(ctx);
+      if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = FSPH_E_HIP;
       const double dt = std::chrono::duration<double>(
                             std::chrono::steady_clock::now() - t0)
                             .count();
-      if (dt < best) {
+      if (!rc && dt < best) {
         best = dt;
-        bg = gm;
-        bc = ch;
+        bg = kGroup[gi];
+        bc = kChunks[ci];
       }
     }
   c->p.group_max = bg;
   c->chunk = bc;
+  if (!rc) rc = snap_restore(c, snap);
+  snap_free(snap);
+  if (rc) return rc;
   if (group_max) *group_max = bg;
   if (chunk) *chunk = bc;
+  return FSPH_OK;
+}
+
+int fsph_set_tuning(fsph_ctx *ctx, int32_t group_max, int32_t chunk) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  if (group_max < 1 || group_max > 128 ||
+      (chunk != 64 && chunk != 128 && chunk != 256))
+    return fsph::fail(c, FSPH_E_ARG,
+                      "group_max must lie in 1..128 and chunk be 64, 128 "
+                      "or 256");
+  c->p.group_max = group_max;
+  c->chunk = chunk;
   return FSPH_OK;
 }
 

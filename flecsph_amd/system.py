@@ -346,6 +346,11 @@ class BatchSystem(BodySystem):
         return it.value, dt.value, t.value
 
 
+def set_tuning(system, group_max, chunk):
+    """Applies (group_max, chunk), e.g. autotune's result, to a system."""
+    system._call("set_tuning", int(group_max), int(chunk))
+
+
 def autotune(system, steps=2):
     """Runs the library's autotuner on a BodySystem or BatchSystem; returns
     (group_max, chunk)."""

@@ -364,11 +364,21 @@ int fsph_enable_graph(fsph_ctx *ctx, int on);
 /* Autotuner (new): times `steps` calls of fsph_step for each candidate
    leaf-group width (group_max 32, 64, 128) and candidate tile chunk of the
    pair kernels (64, 128, 256 candidates staged in LDS per round), keeps the
-   fastest and returns it.  The timed steps are real steps of the
-   simulation(s); the physics does not depend on the choice (neighbour sets
-   are the same; sums are only re-associated). */
+   fastest and returns it.  Every candidate is timed from the same state: the
+   particle fields, clocks and instance clocks are saved first (one device
+   copy of every field) and restored before each candidate and at the end,
+   so the call leaves the simulation where it was (after the initial
+   iteration, which it runs if it has not run).  The learned predictor of
+   FSPH_H_LEARNED keeps what it learns during the timed steps.  The physics
+   does not depend on the choice (neighbour sets are the same; sums are only
+   re-associated). */
 int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
                   int32_t *chunk);
+
+/* Sets the two tuning choices the autotuner searches (new): group_max
+   (1..128, the reference's sub_entities_ limit of a leaf group) and chunk
+   (64, 128 or 256).  Takes effect at the next fsph_update_iteration. */
+int fsph_set_tuning(fsph_ctx *ctx, int32_t group_max, int32_t chunk);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,8 @@ several steps is bit-identical to a separate run with the same parameters.
 import numpy as np
 import pytest
 
-from flecsph_amd import BatchSystem, BodySystem, _abi, autotune, generators
+from flecsph_amd import (BatchSystem, BodySystem, FsphError, _abi, autotune,
+                         generators)
 
 pytestmark = pytest.mark.gpu
 
@@ -93,20 +94,55 @@ def test_batch_rejects_one_simulation_calls():
 
 
 def test_autotune_keeps_the_physics():
+    """The autotuner times every candidate from the same saved state and
+    leaves the simulation where it was: after the call and one more step the
+    run is the untuned run's second step, up to the re-association of the
+    pair sums."""
     parts, prm = generators.sedov(n_side=16)
     s = BodySystem(_abi.default_params(**prm))
     s.upload(parts)
+    s.step()  # the initial iteration
+    before = {f: s.get(f).copy() for f in ("x", "vx", "u", "h")}
+    t_before = s.time()
     g, ch = autotune(s, steps=1)
     assert g in (32, 64, 128) and ch in (64, 128, 256)
+    assert s.time() == t_before
+    for f, v in before.items():
+        assert np.array_equal(s.get(f), v), f
     s.step()
-    # the tuned run stays on the untuned trajectory up to re-association
-    ref = _single(parts, prm, 11)
+    ref = _single(parts, prm, 2)
     ids = s.get("id")
     o = np.argsort(ids)
+    assert s.time()[0] == ref["time"][0]
+    assert np.isclose(s.time()[2], ref["time"][2], rtol=1e-6)
     for f in ("x", "h", "rho"):
         v = s.get(f)[o]
         assert np.allclose(v, ref[f], rtol=1e-5, atol=1e-7), f
     s.close()
+
+
+def test_set_tuning_keeps_neighbour_sets():
+    """group_max and chunk change the leaf groups and the LDS rounds, not the
+    neighbour sets: the centre-of-mass filter of a group is conservative."""
+    from flecsph_amd import set_tuning
+    parts, prm = generators.sedov(n_side=16)
+    lists = []
+    for gm, ch in ((128, 128), (64, 64), (32, 256)):
+        s = BodySystem(_abi.default_params(**prm))
+        s.upload(parts)
+        set_tuning(s, gm, ch)
+        s.update_iteration()
+        off, idx = s.neighbor_lists()
+        lists.append((off, np.concatenate(
+            [np.sort(idx[off[i]:off[i + 1]]) for i in range(len(off) - 1)])))
+        with pytest.raises(FsphError):
+            set_tuning(s, 129, 128)
+        with pytest.raises(FsphError):
+            set_tuning(s, 64, 100)
+        s.close()
+    for off, idx in lists[1:]:
+        assert np.array_equal(off, lists[0][0])
+        assert np.array_equal(idx, lists[0][1])
 
 
 @pytest.mark.parametrize("precision", [_abi.PREC_FP32, _abi.PREC_FP64])
