@@ -42,18 +42,45 @@ struct Seg {
   }
 };
 
-// First sorted index in [lo, n) whose key is >= v.
-__device__ inline int64_t lower_bound(const uint64_t *__restrict__ k,
-                                      int64_t lo, int64_t n, uint64_t v) {
-  int64_t hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (k[mid] < v)
-      lo = mid + 1;
-    else
-      hi = mid;
+// First index in [lo, i] whose key is >= v, given k[i] >= v: steps of 1, 2,
+// 4, ... back from i bracket it, bisection finds it (a node's particles are
+// near i, so this costs about 2 log2 of the distance, not log2 n).
+__device__ inline int64_t lower_back(const uint64_t *__restrict__ k,
+                                     int64_t lo, int64_t i, uint64_t v) {
+  int64_t hi = i, step = 1;  // k[hi] >= v
+  while (hi - step >= lo && k[hi - step] >= v) {
+    hi -= step;
+    step <<= 1;
   }
-  return lo;
+  int64_t a = (hi - step >= lo ? hi - step : lo - 1) + 1, b = hi;
+  while (a < b) {
+    const int64_t mid = (a + b) >> 1;
+    if (k[mid] < v)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return a;
+}
+
+// First index in (i, hi) whose key is > v, or hi, given k[i] <= v: steps of
+// 1, 2, 4, ... forward from i, then bisection.
+__device__ inline int64_t upper_fwd(const uint64_t *__restrict__ k, int64_t i,
+                                    int64_t hi, uint64_t v) {
+  int64_t lo = i, step = 1;  // k[lo] <= v
+  while (lo + step < hi && k[lo + step] <= v) {
+    lo += step;
+    step <<= 1;
+  }
+  int64_t a = lo + 1, b = lo + step < hi ? lo + step : hi;
+  while (a < b) {
+    const int64_t mid = (a + b) >> 1;
+    if (k[mid] <= v)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return a;
 }
 
 // Node index of the depth-D node whose key range starts at sorted index s.
@@ -71,7 +98,7 @@ __device__ inline int64_t containing_node(const uint64_t *__restrict__ k,
                                           int64_t i, int D) {
   const int sh = 3 * (kMaxDepth - D);
   const uint64_t first = (D == 0) ? (1ull << 63) : ((k[i] >> sh) << sh);
-  return node_id(off, L, lower_bound(k, sg.lo(i), sg.hi(i), first), D);
+  return node_id(off, L, lower_back(k, sg.lo(i), i, first), D);
 }
 
 // Common-prefix depth of sorted keys i and i+1; -1 at the end of the array
@@ -153,15 +180,8 @@ __global__ __launch_bounds__(kT) void k_nodes(const uint64_t *__restrict__ k,
     // last index with the same prefix
     const uint64_t last_key =
         (d == 0) ? ~0ull : ((prefix << sh) | ((1ull << sh) - 1ull));
-    int64_t lo = i, hi = sg.hi(i);  // first index with key > last_key
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (k[mid] <= last_key)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    o.count[id] = (int32_t)(lo - i);
+    // first index with key > last_key
+    o.count[id] = (int32_t)(upper_fwd(k, i, sg.hi(i), last_key) - i);
     int64_t par = -1;
     if (d > 0) {
       par = (d - 1 >= d0) ? id - 1 : containing_node(k, sg, off, L, i, d - 1);
@@ -218,13 +238,8 @@ struct CofmArgs {
 
 // physics::compute_cofm (default_physics.h:60-119): leaf children first,
 // then node children, each in digit order (tree_topology.h:1949-1966).
-__global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
-                                             const int32_t *__restrict__ ids,
-                                             int64_t cnt) {
+__device__ inline void cofm_node(const CofmArgs &a, int64_t nd) {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= cnt) return;
-  const int64_t nd = ids[t];
   int32_t ch[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) ch[j] = a.child[nd * 8 + j];
@@ -301,6 +316,14 @@ __global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
     a.box[6 * nd + d] = bmin[d];
     a.box[6 * nd + 3 + d] = bmax[d];
   }
+}
+
+// One level of the tree (the level's node ids).
+__global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
+                                             const int32_t *__restrict__ ids,
+                                             int64_t cnt) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t < cnt) cofm_node(a, ids[t]);
 }
 
 // Groups (tree_topology.h:222-235): nodes with at most gmax particles whose
