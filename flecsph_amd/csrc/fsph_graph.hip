@@ -32,6 +32,7 @@ struct NodeSpec {
   const void *func;
   unsigned block;
   void *args[2];
+  bool pair;  // a pair pass (resident grid), else a per-particle kernel
 };
 
 int graph_setup(Ctx *c) {
@@ -59,12 +60,12 @@ int step_specs(Ctx *c, NodeSpec *sp) {
   for (int pass = 0; pass < 4; ++pass)
     sph_graph_work(c, pass, work + pass * kWorkBytes);
   auto pair = [&](int pass) {
-    return NodeSpec{sph_graph_func(c, pass), 128,
-                    {h + pass * kGraphSlot, work + pass * kWorkBytes}};
+    return NodeSpec{sph_graph_func(c, pass), sph_graph_block(c),
+                    {h + pass * kGraphSlot, work + pass * kWorkBytes}, true};
   };
   auto step = [&](int what) {
     return NodeSpec{step_graph_func(c, what), 256,
-                    {h + kSlotStep * kGraphSlot, n}};
+                    {h + kSlotStep * kGraphSlot, n}, false};
   };
   int k = 0;
   sp[k++] = pair(kSlotDens);
@@ -78,7 +79,7 @@ int step_specs(Ctx *c, NodeSpec *sp) {
 }
 
 unsigned grid_of(Ctx *c, const NodeSpec &s) {
-  if (s.block == 256) return (unsigned)(16 * c->graph.cus);
+  if (!s.pair) return (unsigned)(16 * c->graph.cus);
   // pair passes: the resident size (at least 8 workgroups per CU, so that
   // groups of uneven cost keep every CU busy)
   int per_cu = 0;
@@ -140,7 +141,7 @@ int graph_capture(Ctx *c, const NodeSpec *sp, int ns) {
   if (G.n_nodes != ns)
     return fail(c, FSPH_E_HIP, "graph capture: %d of %d kernel nodes found",
                 G.n_nodes, ns);
-  G.key[0] = c->chunk;
+  G.key[0] = c->chunk * 2 + (int)(sph_graph_block(c) == 64);
   G.key[1] = (int)c->rsize;
   G.key[2] = c->p.adaptive_dt;
   G.key[3] = c->batch.n_inst;
@@ -163,8 +164,8 @@ int graph_step_tail(Ctx *c) {
   FSPH_TRY(graph_setup(c));
   NodeSpec sp[Ctx::Graph::kNodes];
   const int ns = step_specs(c, sp);
-  const int key[4] = {c->chunk, (int)c->rsize, c->p.adaptive_dt,
-                      c->batch.n_inst};
+  const int key[4] = {c->chunk * 2 + (int)(sph_graph_block(c) == 64),
+                      (int)c->rsize, c->p.adaptive_dt, c->batch.n_inst};
   if (!G.exec || key[0] != G.key[0] || key[1] != G.key[1] ||
       key[2] != G.key[2] || key[3] != G.key[3])
     FSPH_TRY(graph_capture(c, sp, ns));

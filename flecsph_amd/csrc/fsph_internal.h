@@ -315,6 +315,7 @@ int batch_roots(Ctx *c);
 // per-pass work descriptor
 void sph_graph_args(Ctx *c, char *slots);
 const void *sph_graph_func(const Ctx *c, int pass);
+unsigned sph_graph_block(const Ctx *c);  // threads of the pair kernels
 void sph_graph_work(Ctx *c, int pass, void *work);
 void step_graph_args(Ctx *c, char *slots);
 const void *step_graph_func(const Ctx *c, int what);

@@ -20,7 +20,8 @@ namespace fsph {
 
 namespace {
 
-constexpr int kB = 128;      // threads per workgroup
+// Threads per workgroup: a template parameter NT of the pair kernels, 128,
+// or 64 (one wavefront) when no group exceeds 64 particles.
 // Candidates staged per LDS round: a template parameter CH (64, 128 or 256)
 // of the pair kernels, chosen by the autotuner.
 
@@ -137,10 +138,11 @@ __device__ inline Grp grp_of(const Tiles &t, int64_t g) {
 }
 
 // Stages the chunk's mask words ([word][particle] layout, contiguous) in LDS.
+template <int NT>
 __device__ inline void stage_masks(const Tiles &t, const Grp &c, int c0,
                                    int nw, uint32_t *smask) {
   const uint32_t *src = t.mask + c.mo + (int64_t)(c0 >> 5) * c.gc;
-  for (int q = threadIdx.x; q < nw * c.gc; q += kB) smask[q] = src[q];
+  for (int q = threadIdx.x; q < nw * c.gc; q += NT) smask[q] = src[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -148,8 +150,8 @@ __device__ inline void stage_masks(const Tiles &t, const Grp &c, int c0,
 //
 // Row k of the chunk's mask (nw words; candidate j is bit j%32 of word
 // j/32) marks the neighbours of group particle k among the chunk's
-// candidates.  The chunk's P pairs, in (k, j) order, are cut into kB runs
-// of q = ceil(P/kB) pairs and thread l walks run l, so every thread walks
+// candidates.  The chunk's P pairs, in (k, j) order, are cut into NT runs
+// of q = ceil(P/NT) pairs and thread l walks run l, so every thread walks
 // the same number of pairs however unevenly they fall on the particles.
 // A run covers whole rows plus at most two partial rows: the row it starts
 // in (which may have begun in an earlier run) and the row it ends in (which
@@ -161,19 +163,20 @@ __device__ inline void stage_masks(const Tiles &t, const Grp &c, int c0,
 // and last runs la < lb holding its pairs.  The result depends only on the
 // tile, not on timing.
 // ---------------------------------------------------------------------------
-struct Sched {
-  int off[kB + 1];  // pairs before row k; off[k] = P for k >= gc
-  int wsum[2];
+template <int NT> struct Sched {
+  int off[NT + 1];  // pairs before row k; off[k] = P for k >= gc
+  int wsum[NT / 64];
 };
 
-template <class T, int NV> struct Parts {
-  T first[NV][kB], last[NV][kB], own[NV][kB];
+template <class T, int NV, int NT> struct Parts {
+  T first[NV][NT], last[NV][NT], own[NV][NT];
 };
 
 enum { kOwn = 0, kFirst = 1, kLast = 2 };
 
-// Row offsets of the chunk (requires the staged masks); returns P.
-__device__ inline int schedule(Sched &S, const uint32_t *smask, int nw,
+// Row offsets of the chunk (requires the staged masks; gc <= NT); returns P.
+template <int NT>
+__device__ inline int schedule(Sched<NT> &S, const uint32_t *smask, int nw,
                                int gc) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   int n = 0;
@@ -187,9 +190,14 @@ __device__ inline int schedule(Sched &S, const uint32_t *smask, int nw,
   }
   if (lane == 63) S.wsum[wv] = inc;
   __syncthreads();
-  const int total = S.wsum[0] + S.wsum[1];
-  S.off[t] = inc - n + (wv ? S.wsum[0] : 0);
-  if (t == 0) S.off[kB] = total;
+  int total = 0, before = 0;
+#pragma unroll
+  for (int v = 0; v < NT / 64; ++v) {
+    if (v < wv) before += S.wsum[v];
+    total += S.wsum[v];
+  }
+  S.off[t] = inc - n + before;
+  if (t == 0) S.off[NT] = total;
   __syncthreads();
   return total;
 }
@@ -212,16 +220,16 @@ __device__ inline uint32_t drop_low(uint32_t x, int r) {
 
 // Walks this thread's run: load(k) when the run enters row k, pair(j) for
 // every pair (k, j), flush(k, kind) when the run leaves row k.
-template <class Load, class Pair, class Flush>
-__device__ inline void walk_run(const Sched &S, const uint32_t *smask, int gc,
-                                int total, Load &&load, Pair &&pair,
+template <int NT, class Load, class Pair, class Flush>
+__device__ inline void walk_run(const Sched<NT> &S, const uint32_t *smask,
+                                int gc, int total, Load &&load, Pair &&pair,
                                 Flush &&flush) {
-  const int q = (total + kB - 1) / kB;
+  const int q = (total + NT - 1) / NT;
   const int p0 = (int)threadIdx.x * q, p1 = min(p0 + q, total);
   if (p0 >= p1) return;
   int k = 0;
 #pragma unroll
-  for (int s = kB / 2; s > 0; s >>= 1)
+  for (int s = NT / 2; s > 0; s >>= 1)
     if (S.off[k + s] <= p0) k += s;
   int rend = S.off[k + 1];
   int r = p0 - S.off[k];
@@ -253,8 +261,8 @@ __device__ inline void walk_run(const Sched &S, const uint32_t *smask, int gc,
   flush(k, began ? kFirst : (rend > p1 ? kLast : kOwn));
 }
 
-template <class T, int NV>
-__device__ inline void put(Parts<T, NV> &R, int i, int k, int kind, T v) {
+template <class T, int NV, int NT>
+__device__ inline void put(Parts<T, NV, NT> &R, int i, int k, int kind, T v) {
   if (kind == kOwn)
     R.own[i][k] = v;
   else if (kind == kFirst)
@@ -265,10 +273,10 @@ __device__ inline void put(Parts<T, NV> &R, int i, int k, int kind, T v) {
 
 // Row k's result of this chunk, component i (after the walk; the row must
 // hold pairs); MAX combines with max instead of +.
-template <bool MAX, class T, int NV>
-__device__ inline T row_total(const Sched &S, const Parts<T, NV> &R, int i,
-                              int k, int total) {
-  const int q = (total + kB - 1) / kB;
+template <bool MAX, class T, int NV, int NT>
+__device__ inline T row_total(const Sched<NT> &S, const Parts<T, NV, NT> &R,
+                              int i, int k, int total) {
+  const int q = (total + NT - 1) / NT;
   const int a = S.off[k], b = S.off[k + 1];
   const int la = a / q, lb = (b - 1) / q;
   if (la == lb) return R.own[i][k];
@@ -290,16 +298,16 @@ template <class T> struct DensArgs {
   Phys ph;
 };
 
-template <class T, int CH>
+template <class T, int CH, int NT>
 __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
   using P = Pos<T>;
   __shared__ P sx[CH], sy[CH], sz[CH];
   __shared__ T sh[CH], sm[CH];
-  __shared__ uint32_t smask[(CH / 32) * kB];
-  __shared__ P qx[kB], qy[kB], qz[kB];
-  __shared__ T qh[kB];
-  __shared__ Sched S;
-  __shared__ Parts<T, 1> R;
+  __shared__ uint32_t smask[(CH / 32) * NT];
+  __shared__ P qx[NT], qy[NT], qz[NT];
+  __shared__ T qh[NT];
+  __shared__ Sched<NT> S;
+  __shared__ Parts<T, 1, NT> R;
   const Grp c = grp_of(a.t, g);
   const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
@@ -312,13 +320,13 @@ __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
   KSum<T> rho;
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = t; j < nch; j += kB) {
+    for (int j = t; j < nch; j += NT) {
       const int64_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
       sh[j] = (T)a.h[b]; sm[j] = (T)a.m[b];
     }
-    stage_masks(a.t, c, c0, nw, smask);
+    stage_masks<NT>(a.t, c, c0, nw, smask);
     __syncthreads();
     const int np = schedule(S, smask, nw, c.gc);
     P px = 0, py = 0, pz = 0;
@@ -369,7 +377,7 @@ template <class T> struct VsigArgs {
   int kernel;
 };
 
-template <class T, bool DIV, int CH>
+template <class T, bool DIV, int CH, int NT>
 __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
   using P = Pos<T>;
   constexpr int NV = DIV ? 2 : 1;
@@ -377,12 +385,12 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
   __shared__ T svx[CH], svy[CH], svz[CH], scn[CH], sco[CH];
   __shared__ T sh[DIV ? CH : 1], sm[DIV ? CH : 1];
   __shared__ int32_t sid[CH];
-  __shared__ uint32_t smask[(CH / 32) * kB];
-  __shared__ P qx[kB], qy[kB], qz[kB];
-  __shared__ T qvx[kB], qvy[kB], qvz[kB], qc[kB];
-  __shared__ T qh[DIV ? kB : 1];
-  __shared__ Sched S;
-  __shared__ Parts<T, NV> R;
+  __shared__ uint32_t smask[(CH / 32) * NT];
+  __shared__ P qx[NT], qy[NT], qz[NT];
+  __shared__ T qvx[NT], qvy[NT], qvz[NT], qc[NT];
+  __shared__ T qh[DIV ? NT : 1];
+  __shared__ Sched<NT> S;
+  __shared__ Parts<T, NV, NT> R;
   const Grp c = grp_of(a.t, g);
   const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
@@ -398,7 +406,7 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
   T vs = 0, dv = 0;  // thread t: particle t's totals
   for (int c0 = 0; c0 < c.cc; c0 += CH) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
-    for (int j = t; j < nch; j += kB) {
+    for (int j = t; j < nch; j += NT) {
       const int32_t b = a.t.cand[c.cb + c0 + j];
       sx[j] = (P)(a.x[b] - ox); sy[j] = (P)(a.y[b] - oy);
       sz[j] = (P)(a.z[b] - oz);
@@ -410,7 +418,7 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
         sm[j] = (T)a.m[b];
       }
     }
-    stage_masks(a.t, c, c0, nw, smask);
+    stage_masks<NT>(a.t, c, c0, nw, smask);
     __syncthreads();
     const int np = schedule(S, smask, nw, c.gc);
     P px = 0, py = 0, pz = 0;
@@ -487,7 +495,7 @@ template <class T, bool DUDT> struct ForceArgs {
 // VS (acceleration pass only): 0 no signal speed, 1 signal speed, 2 signal
 // speed and div v.  The pair set and the pair arithmetic of the signal
 // speed are those of vsig_group, so the results are the same.
-template <class T, bool DUDT, int CH, int VS = 0>
+template <class T, bool DUDT, int CH, int NT, int VS = 0>
 __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   using P = Pos<T>;
   constexpr bool V = DUDT || VS > 0;  // velocities staged
@@ -499,13 +507,13 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   __shared__ T sh[CH], sm[CH], srho[CH], sprho2[CH], sc[CH];
   __shared__ T sco[VS ? CH : 1];
   __shared__ int32_t sid[VS ? CH : 1];
-  __shared__ uint32_t smask[(CH / 32) * kB];
-  __shared__ P qx[kB], qy[kB], qz[kB];
-  __shared__ T qvhx[kB], qvhy[kB], qvhz[kB];
-  __shared__ T qvx[V ? kB : 1], qvy[V ? kB : 1], qvz[V ? kB : 1];
-  __shared__ T qh[kB], qrho[kB], qprho2[kB], qc[kB];
-  __shared__ Sched S;
-  __shared__ Parts<T, NV> R;
+  __shared__ uint32_t smask[(CH / 32) * NT];
+  __shared__ P qx[NT], qy[NT], qz[NT];
+  __shared__ T qvhx[NT], qvhy[NT], qvhz[NT];
+  __shared__ T qvx[V ? NT : 1], qvy[V ? NT : 1], qvz[V ? NT : 1];
+  __shared__ T qh[NT], qrho[NT], qprho2[NT], qc[NT];
+  __shared__ Sched<NT> S;
+  __shared__ Parts<T, NV, NT> R;
   const Grp c = grp_of(a.t, g);
   const int t = threadIdx.x;
   const double ox = pos_origin<T>(a.x, c.gs), oy = pos_origin<T>(a.y, c.gs),
@@ -529,7 +537,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   T vs = 0, dv = 0;
   // The next chunk's candidates and mask words are loaded into registers
   // while the current chunk is walked, and stored to LDS after it.
-  constexpr int PER = CH > kB ? CH / kB : 1, MPER = CH / 32;
+  constexpr int PER = CH > NT ? CH / NT : 1, MPER = CH / 32;
   int32_t pidx[PER], rid[PER];
   double rx[PER], ry[PER], rz[PER], rvhx[PER], rvhy[PER], rvhz[PER];
   double rvx[PER], rvy[PER], rvz[PER], rh[PER], rm[PER];
@@ -538,7 +546,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
   auto load_idx = [&](int c0) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int j = t + q * kB;
+      const int j = t + q * NT;
       pidx[q] = (j < CH && c0 + j < c.cc) ? a.t.cand[c.cb + c0 + j] : -1;
     }
   };
@@ -561,7 +569,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     const uint32_t *src = a.t.mask + c.mo + (int64_t)(c0 >> 5) * c.gc;
 #pragma unroll
     for (int q = 0; q < MPER; ++q) {
-      const int w = t + q * kB;
+      const int w = t + q * NT;
       pm[q] = w < nw * c.gc ? src[w] : 0u;
     }
   };
@@ -572,7 +580,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     const int nch = min(CH, c.cc - c0), nw = (nch + 31) >> 5;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int j = t + q * kB;
+      const int j = t + q * NT;
       if (j >= nch) continue;
       sx[j] = (P)(rx[q] - ox); sy[j] = (P)(ry[q] - oy);
       sz[j] = (P)(rz[q] - oz);
@@ -593,7 +601,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     }
 #pragma unroll
     for (int q = 0; q < MPER; ++q) {
-      const int w = t + q * kB;
+      const int w = t + q * NT;
       if (w < nw * c.gc) smask[w] = pm[q];
     }
     __syncthreads();
@@ -707,17 +715,17 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
 }
 
 // One workgroup per group (grid = number of groups).
-template <class T, int CH>
-__global__ __launch_bounds__(kB) void k_density(DensArgs<T> a) {
-  density_group<T, CH>(a, xcd_item(blockIdx.x, gridDim.x));
+template <class T, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_density(DensArgs<T> a) {
+  density_group<T, CH, NT>(a, xcd_item(blockIdx.x, gridDim.x));
 }
-template <class T, bool DIV, int CH>
-__global__ __launch_bounds__(kB) void k_vsig(VsigArgs<T> a) {
-  vsig_group<T, DIV, CH>(a, xcd_item(blockIdx.x, gridDim.x));
+template <class T, bool DIV, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_vsig(VsigArgs<T> a) {
+  vsig_group<T, DIV, CH, NT>(a, xcd_item(blockIdx.x, gridDim.x));
 }
-template <class T, bool DUDT, int CH, int VS = 0>
-__global__ __launch_bounds__(kB) void k_force(ForceArgs<T, DUDT> a) {
-  force_group<T, DUDT, CH, VS>(a, xcd_item(blockIdx.x, gridDim.x));
+template <class T, bool DUDT, int CH, int NT, int VS = 0>
+__global__ __launch_bounds__(NT) void k_force(ForceArgs<T, DUDT> a) {
+  force_group<T, DUDT, CH, NT, VS>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
 // Persistent form (for graph replay): a grid of resident size whose
@@ -739,29 +747,48 @@ __device__ inline int64_t next_group(const Work &w, int64_t *s_g) {
   return g;
 }
 
-template <class T, int CH>
-__global__ __launch_bounds__(kB) void k_density_p(DensArgs<T> a, Work w) {
+template <class T, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_density_p(DensArgs<T> a, Work w) {
   __shared__ int64_t s_g;
   for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
-    density_group<T, CH>(a, g);
+    density_group<T, CH, NT>(a, g);
     __syncthreads();
   }
 }
-template <class T, int CH>
-__global__ __launch_bounds__(kB) void k_vsig_p(VsigArgs<T> a, Work w) {
+template <class T, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_vsig_p(VsigArgs<T> a, Work w) {
   __shared__ int64_t s_g;
   for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
-    vsig_group<T, false, CH>(a, g);
+    vsig_group<T, false, CH, NT>(a, g);
     __syncthreads();
   }
 }
-template <class T, bool DUDT, int CH, int VS = 0>
-__global__ __launch_bounds__(kB) void k_force_p(ForceArgs<T, DUDT> a,
+template <class T, bool DUDT, int CH, int NT, int VS = 0>
+__global__ __launch_bounds__(NT) void k_force_p(ForceArgs<T, DUDT> a,
                                                 Work w) {
   __shared__ int64_t s_g;
   for (int64_t g; (g = next_group(w, &s_g)) < w.ng;) {
-    force_group<T, DUDT, CH, VS>(a, g);
+    force_group<T, DUDT, CH, NT, VS>(a, g);
     __syncthreads();
+  }
+}
+
+// Calls f(CH, NT) (as std::integral_constant values) for the context's
+// chunk and workgroup width: one wavefront when no group exceeds 64
+// particles, two otherwise.
+template <int V> using IC = std::integral_constant<int, V>;
+inline bool narrow(const Ctx *c) { return c->p.group_max <= 64; }
+template <class F> void dispatch(const Ctx *c, F &&f) {
+  const bool nr = narrow(c);
+  switch (c->chunk) {
+    case 64:
+      if (nr) f(IC<64>{}, IC<64>{}); else f(IC<64>{}, IC<128>{});
+      break;
+    case 256:
+      if (nr) f(IC<256>{}, IC<64>{}); else f(IC<256>{}, IC<128>{});
+      break;
+    default:
+      if (nr) f(IC<128>{}, IC<64>{}); else f(IC<128>{}, IC<128>{});
   }
 }
 
@@ -790,11 +817,10 @@ Tiles tiles_of(const Ctx *c) {
 }
 
 template <class T, bool DIV> void vsig_launch(Ctx *c, const VsigArgs<T> &v) {
-  switch (c->chunk) {
-    case 64: k_vsig<T, DIV, 64><<<(unsigned)c->ng, kB, 0, c->stream>>>(v); break;
-    case 256: k_vsig<T, DIV, 256><<<(unsigned)c->ng, kB, 0, c->stream>>>(v); break;
-    default: k_vsig<T, DIV, 128><<<(unsigned)c->ng, kB, 0, c->stream>>>(v);
-  }
+  dispatch(c, [&](auto ch, auto nt) {
+    k_vsig<T, DIV, decltype(ch)::value, decltype(nt)::value>
+        <<<(unsigned)c->ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(v);
+  });
 }
 
 template <class T> DensArgs<T> dens_args(Ctx *c) {
@@ -865,11 +891,10 @@ template <class T> int density_impl(Ctx *c, bool with_vsig) {
   const DensArgs<T> a = dens_args<T>(c);
   {
     PhaseTimer tm(c, FSPH_T_DENSITY);
-    switch (c->chunk) {
-      case 64: k_density<T, 64><<<(unsigned)c->ng, kB, 0, c->stream>>>(a); break;
-      case 256: k_density<T, 256><<<(unsigned)c->ng, kB, 0, c->stream>>>(a); break;
-      default: k_density<T, 128><<<(unsigned)c->ng, kB, 0, c->stream>>>(a);
-    }
+    dispatch(c, [&](auto ch, auto nt) {
+      k_density<T, decltype(ch)::value, decltype(nt)::value>
+          <<<(unsigned)c->ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(a);
+    });
   }
   FSPH_HIP(c, hipGetLastError());
   if (!with_vsig) return 0;
@@ -889,11 +914,10 @@ template <class T, bool DUDT, int VS = 0> int force_impl(Ctx *c) {
   {
     PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
     const unsigned ng = (unsigned)c->ng;
-    switch (c->chunk) {
-      case 64: k_force<T, DUDT, 64, VS><<<ng, kB, 0, c->stream>>>(a); break;
-      case 256: k_force<T, DUDT, 256, VS><<<ng, kB, 0, c->stream>>>(a); break;
-      default: k_force<T, DUDT, 128, VS><<<ng, kB, 0, c->stream>>>(a);
-    }
+    dispatch(c, [&](auto ch, auto nt) {
+      k_force<T, DUDT, decltype(ch)::value, decltype(nt)::value, VS>
+          <<<ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(a);
+    });
   }
   FSPH_HIP(c, hipGetLastError());
   if (DUDT) {
@@ -923,22 +947,24 @@ template <class T> void sph_graph_args_t(Ctx *c, char *slots) {
   std::swap(c->f[PF_C], c->g[PF_C]);
 }
 
-template <class T, int CH> const void *sph_graph_func_t(int pass) {
+template <class T, int CH, int NT> const void *sph_graph_func_t(int pass) {
   switch (pass) {
-    case kSlotDens: return reinterpret_cast<const void *>(k_density_p<T, CH>);
-    case kSlotVsig: return reinterpret_cast<const void *>(k_vsig_p<T, CH>);
+    case kSlotDens:
+      return reinterpret_cast<const void *>(k_density_p<T, CH, NT>);
+    case kSlotVsig: return reinterpret_cast<const void *>(k_vsig_p<T, CH, NT>);
     case kSlotAcc:
-      return reinterpret_cast<const void *>(k_force_p<T, false, CH, 1>);
-    default: return reinterpret_cast<const void *>(k_force_p<T, true, CH>);
+      return reinterpret_cast<const void *>(k_force_p<T, false, CH, NT, 1>);
+    default:
+      return reinterpret_cast<const void *>(k_force_p<T, true, CH, NT>);
   }
 }
 
-template <class T> const void *sph_graph_func_c(int chunk, int pass) {
-  switch (chunk) {
-    case 64: return sph_graph_func_t<T, 64>(pass);
-    case 256: return sph_graph_func_t<T, 256>(pass);
-    default: return sph_graph_func_t<T, 128>(pass);
-  }
+template <class T> const void *sph_graph_func_c(const Ctx *c, int pass) {
+  const void *f = nullptr;
+  dispatch(c, [&](auto ch, auto nt) {
+    f = sph_graph_func_t<T, decltype(ch)::value, decltype(nt)::value>(pass);
+  });
+  return f;
 }
 
 }  // namespace
@@ -951,9 +977,11 @@ void sph_graph_args(Ctx *c, char *slots) {
 }
 
 const void *sph_graph_func(const Ctx *c, int pass) {
-  return c->rsize == 4 ? sph_graph_func_c<float>(c->chunk, pass)
-                       : sph_graph_func_c<double>(c->chunk, pass);
+  return c->rsize == 4 ? sph_graph_func_c<float>(c, pass)
+                       : sph_graph_func_c<double>(c, pass);
 }
+
+unsigned sph_graph_block(const Ctx *c) { return narrow(c) ? 64u : 128u; }
 
 void sph_graph_work(Ctx *c, int pass, void *work) {
   const Work w{c->graph.counters.as<unsigned long long>() + pass, c->ng};

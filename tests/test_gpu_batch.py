@@ -145,10 +145,14 @@ def test_set_tuning_keeps_neighbour_sets():
         assert np.array_equal(idx, lists[0][1])
 
 
+@pytest.mark.parametrize("tuning", [(128, 128), (64, 64)])
 @pytest.mark.parametrize("precision", [_abi.PREC_FP32, _abi.PREC_FP64])
-def test_graph_replay_matches_eager_every_step(precision):
+def test_graph_replay_matches_eager_every_step(precision, tuning):
     """The graph-replayed step tail gives the eager batched results, step
-    after step (each replay follows eager tree building and host syncs)."""
+    after step (each replay follows eager tree building and host syncs), with
+    two-wavefront (group_max 128) and one-wavefront (group_max 64) pair
+    kernels."""
+    from flecsph_amd import set_tuning
     sets, prm = [], None
     for ns in (10, 14, 12):
         parts, prm = generators.sedov(n_side=ns)
@@ -158,6 +162,7 @@ def test_graph_replay_matches_eager_every_step(precision):
     graph = BatchSystem(_abi.default_params(**prm), len(sets))
     graph.enable_graph(True)
     for b in (eager, graph):
+        set_tuning(b, *tuning)
         b.upload(sets)
     for step in range(6):
         eager.step()

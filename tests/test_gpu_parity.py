@@ -138,6 +138,34 @@ def test_sph_passes(oracle_mod, name, precision):
         assert e <= tol[f], (f, e)
 
 
+@pytest.mark.parametrize("precision", [_abi.PREC_FP64, _abi.PREC_FP32])
+@pytest.mark.parametrize("name", ["random_varh", "sedov16_periodic"])
+def test_sph_passes_one_wavefront(oracle_mod, name, precision):
+    """group_max 64 selects the one-wavefront pair kernels (and chunk 64 the
+    smallest LDS rounds): same bars as the two-wavefront kernels."""
+    from flecsph_amd import set_tuning
+    parts, prm = get_case(name, precision)
+    prm = dict(prm, group_max=64)
+    g, o = both(oracle_mod, parts, prm)
+    set_tuning(g, 64, 64)
+    tol = TOL[precision]
+    for s in (g, o):
+        s.update_iteration()
+        s.density_eos()
+    for f in ("rho", "P", "c", "vsig"):
+        assert rel_err(g.get(f), o.get(f)) <= tol[f], f
+    for s in (g, o):
+        s.integrate(_abi.SAVE_VHALF)
+        s.acceleration()
+    A = [np.stack([s.get("ax"), s.get("ay"), s.get("az")], 1) for s in (g, o)]
+    assert rel_err(A[0], A[1]) <= tol["a"]
+    for s in (g, o):
+        s.set_dt(1e-4)
+        s.integrate(_abi.KICK_V)
+        s.dudt()
+    assert rel_err(g.get("dudt"), o.get("dudt")) <= tol["dudt"]
+
+
 @pytest.mark.parametrize("name", ["random_varh", "sedov16_periodic",
                                   "sod_small"])
 def test_three_steps_fp64(oracle_mod, name):
