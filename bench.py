@@ -71,10 +71,9 @@ VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
 # FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each scaled by what the
 # counter reads for a plain 256 MiB device copy (FETCH_SIZE x 2.00,
 # WRITE_SIZE x 1.00); profiles/sedov64_hbm_traffic.md has the commands.
-SEDOV64_TRAFFIC_BYTES = {
-    "masks": 224_360_354, "traverse": 124_931_367, "density": 234_199_386,
-    "vsig": 383_764_120, "accel": 562_229_812, "dudt": 717_041_808,
-}
+# Measured with the tuning below; other tunings report no traffic.
+SEDOV64_TRAFFIC_TUNING = (64, 64)
+SEDOV64_TRAFFIC_BYTES = {}
 
 
 def workload(name):
@@ -112,6 +111,9 @@ def parse():
                     help="keep group_max 128 and chunk 128 instead of "
                          "autotuning them before the warm-up")
     ap.add_argument("--tune-steps", type=int, default=3)
+    ap.add_argument("--tuning", default="",
+                    help="GROUP_MAX,CHUNK: use these instead of autotuning "
+                         "(profiling runs use the bench line's choice)")
     ap.add_argument("--batch-inst", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
@@ -305,7 +307,12 @@ def main():
     g = BodySystem(_abi.default_params(**prm), device=0)
     g.upload(parts)
     tuning = {"group_max": 128, "chunk": 128}
-    if not args.no_autotune:
+    if args.tuning:
+        from flecsph_amd import set_tuning
+        gm, ch = (int(v) for v in args.tuning.split(","))
+        set_tuning(g, gm, ch)
+        tuning = {"group_max": gm, "chunk": ch, "given": True}
+    elif not args.no_autotune:
         # untimed: each (group_max, chunk) candidate is timed over
         # tune_steps steps from the state after the initial iteration, which
         # the autotuner restores afterwards (the warm-up then continues from
@@ -337,7 +344,9 @@ def main():
     dom = max(single, key=single.get)
     achieved = KERNEL_BYTES[dom] * avg_ntot / (single[dom] * 1e-3) / 1e9
     traffic = (SEDOV64_TRAFFIC_BYTES.get(dom)
-               if args.workload == "sedov64" else None)
+               if args.workload == "sedov64" and
+               (tuning["group_max"], tuning["chunk"]) == SEDOV64_TRAFFIC_TUNING
+               else None)
     # pair count of the step: the neighbour-mask bits, one per (a, b)
     pairs = float(g.get("nngb").sum())
     valu = None

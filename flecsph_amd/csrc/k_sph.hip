@@ -60,44 +60,40 @@ struct Phys {
 };
 
 // kernels.h: cubic spline (106-153) and Wendland C6 3D (502-566); h is the
-// support radius.  W(r, 1/h), and F with grad W = r_vec * F.
+// support radius.  W(r, 1/h), and F with grad W = r_vec * F.  Written
+// without branches (clamped polynomials and selects): a pair outside the
+// support gets exactly 0, a pair inside the same value as the piecewise
+// form.
 template <class T> __device__ inline T kernel_W(int kind, T r, T ih) {
   const T ih3 = ih * ih * ih;
   if (kind == FSPH_KERNEL_CUBIC_SPLINE) {
     const T q = T(2) * r * ih;
-    if (q >= T(2)) return T(0);
     const T sig = T(8.0 / M_PI) * ih3;
-    if (q < T(1)) return sig * (T(1) - T(1.5) * q * q + T(0.75) * q * q * q);
-    const T t = T(2) - q;
-    return sig * T(0.25) * t * t * t;
+    const T t = fmax(T(2) - q, T(0));
+    const T inner = sig * (T(1) - T(1.5) * q * q + T(0.75) * q * q * q);
+    const T outer = sig * T(0.25) * t * t * t;
+    return q < T(1) ? inner : outer;
   }
   const T q = r * ih;
-  if (q >= T(1)) return T(0);
-  const T om = T(1) - q;
+  const T om = fmax(T(1) - q, T(0));
   const T om2 = om * om, om4 = om2 * om2;
   return T(1365.0 / (64.0 * M_PI)) * ih3 * om4 * om4 *
          (T(1) + q * (T(8) + q * (T(25) + q * T(32))));
 }
 
-// F = (dW/dr) / r for r > 0 (callers skip coincident pairs).
+// F = (dW/dr) / r for r > 0, and 0 for coincident pairs given ir = 0.
 template <class T> __device__ inline T kernel_F(int kind, T r, T ir, T ih) {
   const T ih4 = (ih * ih) * (ih * ih);
   if (kind == FSPH_KERNEL_CUBIC_SPLINE) {
     const T q = T(2) * r * ih;
-    if (q >= T(2)) return T(0);
     const T sig = T(2.0 * 8.0 / M_PI) * ih4;
-    T dW;
-    if (q < T(1))
-      dW = T(-3) * q + T(9.0 / 4.0) * q * q;
-    else {
-      const T t = T(2) - q;
-      dW = T(-0.75) * t * t;
-    }
+    const T t = fmax(T(2) - q, T(0));
+    const T dW = q < T(1) ? T(-3) * q + T(9.0 / 4.0) * q * q
+                          : T(-0.75) * t * t;
     return sig * dW * ir;
   }
   const T q = r * ih;
-  if (q >= T(1)) return T(0);
-  const T om = T(1) - q;
+  const T om = fmax(T(1) - q, T(0));
   const T om2 = om * om, om3 = om2 * om, om4 = om2 * om2;
   const T dW = T(-22) * q * om4 * om3 * (T(1) + q * (T(7) + q * T(16)));
   return T(1365.0 / (64.0 * M_PI)) * ih4 * dW * ir;
@@ -434,17 +430,18 @@ __device__ inline void vsig_group(const VsigArgs<T> &a, int64_t g) {
           ia = (int32_t)(c.gs + k);
         },
         [&](int j) {
+          // coincident pairs (m_b := 0, default_physics.h:378) add 0
           const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-          if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
+          const bool same = ddx == P(0) && ddy == P(0) && ddz == P(0);
           const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
           const T r2 = dx * dx + dy * dy + dz * dz;
-          const T ir = rsq(r2);
+          const T ir = same ? T(0) : rsq(r2);
           const T vdr =
               (pvx - svx[j]) * dx + (pvy - svy[j]) * dy + (pvz - svz[j]) * dz;
           const T vdn = vdr * ir;
           const T cb = sid[j] >= ia ? scn[j] : sco[j];
           const T cab = pc < cb ? cb : pc;
-          const T v = cab - (vdn < T(0) ? vdn : T(0));
+          const T v = same ? T(0) : cab - (vdn < T(0) ? vdn : T(0));
           rvs = rvs < v ? v : rvs;
           if (DIV) {
             const T ih = rcp(T(0.5) * (ph + sh[j]));
@@ -627,11 +624,13 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
           ia = (int32_t)(c.gs + k);
         },
         [&](int j) {
+          // coincident pairs (m_b := 0, default_physics.h:378) add 0: with
+          // ir = 0, F = 0 and every term below is 0
           const P ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-          if (ddx == P(0) && ddy == P(0) && ddz == P(0)) return;
+          const bool same = ddx == P(0) && ddy == P(0) && ddz == P(0);
           const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
           const T r2 = dx * dx + dy * dy + dz * dz;
-          const T ir = rsq(r2);
+          const T ir = same ? T(0) : rsq(r2);
           const T r = r2 * ir;
           const T hab = T(0.5) * (ph + sh[j]);
           const T ih = rcp(hab);
@@ -643,11 +642,10 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
             const T vdn = vdr * ir;
             const T cb = sid[j] >= ia ? sc[j] : sco[j];
             const T cmax = pc < cb ? cb : pc;
-            const T v = cmax - (vdn < T(0) ? vdn : T(0));
+            const T v = same ? T(0) : cmax - (vdn < T(0) ? vdn : T(0));
             rvs = rvs < v ? v : rvs;
             if (VS == 2) rdv += sm[j] * vdr * F;
           }
-          if (F == T(0)) return;
           const T vhdr = (pvhx - svhx[j]) * dx + (pvhy - svhy[j]) * dy +
                          (pvhz - svhz[j]) * dz;
           T mu = T(0);
