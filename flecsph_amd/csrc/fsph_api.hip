@@ -139,15 +139,9 @@ int do_timestep(Ctx *c) {
     FSPH_HIP(c, hipStreamSynchronize(c->stream));
     return 0;
   }
-  double dtmin = 0;
-  FSPH_TRY(stage_dt_min(c, &dtmin));
-  // set_adaptive_timestep, default_physics.h:762-823 (no output matching)
-  if (dtmin < c->dt) c->dt = (c->dt / 2.0 < dtmin) ? c->dt / 2.0 : dtmin;
-  if (dtmin > 2.0 * c->dt) c->dt = c->dt * 2.0;
-  c->totaltime_next = c->totaltime + c->dt;
-  volatile double tn = c->totaltime_next;
-  c->dt = tn - c->totaltime;
-  return 0;
+  // set_adaptive_timestep (default_physics.h:762-823, no output matching)
+  // on the device clock: no host round trip
+  return stage_dt_rule(c);
 }
 
 int do_advance(Ctx *c) {
@@ -158,11 +152,7 @@ int do_advance(Ctx *c) {
       B.t_h[i] = c->p.adaptive_dt ? B.tnext_h[i] : B.t_h[i] + B.dt_h[i];
     return 0;
   }
-  if (c->p.adaptive_dt)
-    c->totaltime = c->totaltime_next;
-  else
-    c->totaltime = c->totaltime + c->dt;
-  return 0;
+  return stage_advance(c);
 }
 
 int do_gravity(Ctx *c);
@@ -315,6 +305,7 @@ int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
     return FSPH_E_OOM;
   }
   for (int i = 0; i < 2 * Ctx::kEvPool; ++i) hipEventCreate(&c->ev[i]);
+  hipEventCreateWithFlags(&c->ev_groups, hipEventDisableTiming);
   *out = reinterpret_cast<fsph_ctx *>(c);
   return FSPH_OK;
 }
@@ -354,6 +345,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   for (int i = 0; i < 2 * Ctx::kEvPool; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->mail_h) hipHostFree(c->mail_h);
+  if (c->ev_groups) hipEventDestroy(c->ev_groups);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -442,6 +434,7 @@ int fsph::upload_particles(Ctx *c, const fsph_particles *in) {
   c->totaltime = 0;
   c->totaltime_next = 0;
   c->initial_done = false;
+  FSPH_TRY(put_time(c));
   return FSPH_OK;
 }
 
@@ -563,6 +556,7 @@ int fsph_timestep(fsph_ctx *ctx, double *dt) {
   CTX_OR_FAIL;
   NOT_BATCH;
   FSPH_TRY(fsph::do_timestep(c));
+  FSPH_TRY(fsph::sync_time(c));
   if (dt) *dt = c->dt;
   return FSPH_OK;
 }
@@ -589,8 +583,11 @@ int fsph_step(fsph_ctx *ctx) {
 
 int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,
               double *totaltime) {
-  const Ctx *c = reinterpret_cast<const Ctx *>(ctx);
+  // the host copies of the clock are a cache of the device's
+  Ctx *c = const_cast<Ctx *>(reinterpret_cast<const Ctx *>(ctx));
   if (!c || c->batch.n_inst) return FSPH_E_ARG;
+  if (hipSetDevice(c->device) != hipSuccess) return FSPH_E_HIP;
+  FSPH_TRY(fsph::sync_time(c));
   if (iteration) *iteration = c->iteration;
   if (dt) *dt = c->dt;
   if (totaltime) *totaltime = c->totaltime;
@@ -600,7 +597,10 @@ int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,
 int fsph_set_dt(fsph_ctx *ctx, double dt) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c || !(dt > 0)) return FSPH_E_ARG;
+  if (hipSetDevice(c->device) != hipSuccess) return FSPH_E_HIP;
+  FSPH_TRY(fsph::sync_time(c));
   c->dt = dt;
+  FSPH_TRY(fsph::put_time(c));
   return FSPH_OK;
 }
 

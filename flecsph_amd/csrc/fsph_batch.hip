@@ -67,6 +67,7 @@ int snap_take(Ctx *c, Snapshot &s) {
     FSPH_HIP(c, hipMemcpyAsync(s.f[q].p, c->f[q].p, b,
                                hipMemcpyDeviceToDevice, c->stream));
   }
+  FSPH_TRY(fsph::sync_time(c));
   s.n = c->n;
   s.n_real = c->n_real;
   s.iteration = c->iteration;
@@ -108,6 +109,7 @@ int snap_restore(Ctx *c, const Snapshot &s) {
   c->totaltime_next = s.totaltime_next;
   c->initial_done = s.initial_done;
   c->have_tree = false;
+  FSPH_TRY(fsph::put_time(c));
   auto &B = c->batch;
   if (B.n_inst) {
     B.dt_h = s.dt_h;

@@ -202,10 +202,16 @@ struct Ctx {
 
   // time stepping
   int64_t iteration = 0;
+  // dt, totaltime and totaltime_next live on the device (tstate, read by
+  // the kernels); these are host copies, refreshed by sync_time when
+  // time_dirty (set when a kernel has changed tstate)
   double dt = 0, totaltime = 0, totaltime_next = 0;
+  DBuf tstate;
+  bool time_dirty = false;
   bool initial_done = false;
 
   // timing: event pairs handed out per timed region of a step
+  hipEvent_t ev_groups = nullptr;  // group count copied to mail_h[kMail-1]
   static constexpr int kEvPool = 96;
   bool timing = false;
   hipEvent_t ev[2 * kEvPool] = {};
@@ -293,11 +299,15 @@ int stage_gravity(Ctx *c);             // k_grav.hip
 void free_gravity(Ctx *c);             // k_grav.hip
 int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
 int stage_update_h(Ctx *c);            // k_step.hip
-int stage_dt_min(Ctx *c, double *dtmin);     // k_step.hip
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
 // device pointer to 2.5 max h (the periodic halo), no host wait
 int stage_halo(Ctx *c, const double **halo);  // k_step.hip
 int read_mail(Ctx *c, int words);             // fsph_api.hip
+// device clock of a single simulation (k_step.hip)
+int put_time(Ctx *c);      // tstate := host dt, totaltime, totaltime_next
+int sync_time(Ctx *c);     // host copies := tstate (waits) if time_dirty
+int stage_dt_rule(Ctx *c); // dt minimum and set_adaptive_timestep's rule
+int stage_advance(Ctx *c); // totaltime := totaltime_next (or += dt)
 int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
                       int64_t n);              // k_step.hip
 // Batched mode (k_batch.hip): per-instance reductions over the particles

@@ -337,7 +337,8 @@ template <class T> struct FeatArgs {
   const int32_t *idx;  // sample -> particle (or null: sample = particle)
   const float *norm;   // mean[6], inv_std[6]
   int64_t B;
-  double dt, cN, invN;
+  const double *dt;  // the simulation's clock: dt at dt[0]
+  double cN, invN;
   float *X;
 };
 
@@ -353,13 +354,14 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs<T> a) {
   const double ih = 1.0 / h;
   const double rho = (double)a.rho[i];
   const double ax = a.ax[i], ay = a.ay[i], az = a.az[i];
+  const double dt = *a.dt;
   float f[kFeat];
   f[0] = (float)log(a.cN * rho * h * h * h / a.m[i]);
   f[1] = (float)(a.nngb[i] * a.invN - 1.0);
-  f[2] = (float)(a.dt * (double)a.divv[i]);
-  f[3] = (float)(a.dt * (double)a.vsig[i] * ih);
+  f[2] = (float)(dt * (double)a.divv[i]);
+  f[3] = (float)(dt * (double)a.vsig[i] * ih);
   f[4] = (float)a.dlh[i];
-  f[5] = (float)(sqrt(ax * ax + ay * ay + az * az) * a.dt * a.dt * ih);
+  f[5] = (float)(sqrt(ax * ax + ay * ay + az * az) * dt * dt * ih);
   float *o = a.X + j * kFeatP;
   for (int q = 0; q < kFeat; ++q) {
     float v = (f[q] - a.norm[q]) * a.norm[kFeat + q];
@@ -442,7 +444,7 @@ template <class T> FeatArgs<T> feat_args(Ctx *c) {
   a.dlh = c->f[PF_DLH].as<T>();
   a.nngb = c->f[PF_NNGB].as<int32_t>();
   a.norm = c->mlp.norm.as<float>();
-  a.dt = c->dt;
+  a.dt = c->tstate.as<double>();
   const double nt = c->p.n_target > 1 ? c->p.n_target : 2;
   a.cN = 4.0 * M_PI / 3.0 / (nt - 1.0);
   a.invN = 1.0 / nt;

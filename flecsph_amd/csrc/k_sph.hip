@@ -479,8 +479,9 @@ template <class T, bool DUDT> struct ForceArgs {
   const T *rho, *P, *c;
   T *ax, *ay, *az, *gx, *gy, *gz, *phi;  // acceleration pass
   T *dudt, *P_out, *c_out;               // du/dt pass
-  double dt;
-  const double *dtv;    // batched mode: dt of the particle's instance
+  // dt of the particle: dtv[inst[i]] (batched mode) or dtv[0] (the
+  // simulation's clock)
+  const double *dtv;
   const int32_t *inst;
   // signal speed (and div v) in the acceleration pass, as the separate
   // signal-speed pass computes them: c_old, vsig, divv
@@ -703,7 +704,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
                  T(0.5) * (tot[1].s - tot[1].c);
     a.dudt[i] = du;
     // recompute_pressure_soundspeed (default_physics.h:154-161)
-    const double dti = a.dtv ? a.dtv[a.inst[i]] : a.dt;
+    const double dti = a.dtv[a.inst ? a.inst[i] : 0];
     const double uh = a.u[i] + 0.5 * dti * (double)du;
     double P, cs;
     eos_pc(a.ph, (double)prho, uh, &P, &cs);
@@ -871,7 +872,7 @@ template <class T, bool DUDT> ForceArgs<T, DUDT> force_args(Ctx *c) {
   // group has read the current values.
   a.P_out = c->g[PF_P].as<T>();
   a.c_out = c->g[PF_C].as<T>();
-  a.dt = c->dt;
+  a.dtv = c->tstate.as<double>();
   if (c->batch.n_inst) {
     a.dtv = c->batch.dt.as<double>();
     a.inst = c->f[PF_INST].as<int32_t>();

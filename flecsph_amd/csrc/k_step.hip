@@ -52,7 +52,35 @@ __global__ __launch_bounds__(kT) void k_integrate(IntArgs<T> a, int64_t n,
                                                   const int32_t *inst) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
-  integrate_one(a, i, what, dtv ? dtv[inst[i]] : dt);
+  integrate_one(a, i, what, dtv ? dtv[inst ? inst[i] : 0] : dt);
+}
+
+// The clock of a single simulation: ts = {dt, totaltime, totaltime_next}.
+// set_adaptive_timestep (default_physics.h:762-823, no output matching) on
+// the reduced minimum, in the host code's operation order.
+__global__ void k_dt_rule(const double *__restrict__ dtmin, double *ts) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double m = *dtmin, t = ts[1];
+  double dt = ts[0];
+  if (m < dt) dt = (dt / 2.0 < m) ? dt / 2.0 : m;
+  if (m > 2.0 * dt) dt = dt * 2.0;
+  const double tn = t + dt;
+  ts[0] = tn - t;
+  ts[2] = tn;
+}
+
+__global__ void k_advance(double *ts, int adaptive) {
+#pragma clang fp contract(off)
+  if (threadIdx.x == 0) ts[1] = adaptive ? ts[2] : ts[1] + ts[0];
+}
+
+__global__ void k_set_time(double *ts, double dt, double t, double tn) {
+  if (threadIdx.x == 0) {
+    ts[0] = dt;
+    ts[1] = t;
+    ts[2] = tn;
+  }
 }
 
 template <class T>
@@ -421,12 +449,13 @@ template <class T> int integrate_impl(Ctx *c, uint32_t what) {
                c->f[PF_DUDT].as<T>()};
   const bool b = c->batch.n_inst > 0;
   k_integrate<T><<<blocks_for(c->n), kT, 0, c->stream>>>(
-      a, c->n, what, c->dt, b ? c->batch.dt.as<double>() : nullptr,
+      a, c->n, what, 0.0,
+      b ? c->batch.dt.as<double>() : c->tstate.as<double>(),
       b ? c->f[PF_INST].as<int32_t>() : nullptr);
   return hip_check(c, hipGetLastError(), "k_integrate");
 }
 
-template <class T> int dt_impl(Ctx *c, double *dtmin) {
+template <class T> int dt_impl(Ctx *c) {
   const int nb = (int)std::min<int64_t>(1024, (c->n + kT - 1) / kT);
   FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 8));
   double *part = c->red_tmp.as<double>();
@@ -438,11 +467,9 @@ template <class T> int dt_impl(Ctx *c, double *dtmin) {
               c->p.visc_alpha};
   k_dt_partial<T><<<nb, kT, 0, c->stream>>>(a, c->n, part);
   k_min_final<<<1, kT, 0, c->stream>>>(part, nb);
-  FSPH_HIP(c, hipGetLastError());
-  FSPH_HIP(c, hipMemcpyAsync(dtmin, part + nb, 8, hipMemcpyDeviceToHost,
-                             c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  return 0;
+  k_dt_rule<<<1, 64, 0, c->stream>>>(part + nb, c->tstate.as<double>());
+  c->time_dirty = true;
+  return hip_check(c, hipGetLastError(), "k_dt_rule");
 }
 
 template <class T> int red_impl(Ctx *c, double out[10]) {
@@ -580,8 +607,36 @@ int batch_roots(Ctx *c) {
   return hip_check(c, hipGetLastError(), "k_seg_roots");
 }
 
-int stage_dt_min(Ctx *c, double *dtmin) {
-  return c->rsize == 4 ? dt_impl<float>(c, dtmin) : dt_impl<double>(c, dtmin);
+int stage_dt_rule(Ctx *c) {
+  return c->rsize == 4 ? dt_impl<float>(c) : dt_impl<double>(c);
+}
+
+int stage_advance(Ctx *c) {
+  k_advance<<<1, 64, 0, c->stream>>>(c->tstate.as<double>(),
+                                     c->p.adaptive_dt);
+  c->time_dirty = true;
+  return hip_check(c, hipGetLastError(), "k_advance");
+}
+
+int put_time(Ctx *c) {
+  FSPH_TRY(ensure(c, c->tstate, 3 * sizeof(double)));
+  k_set_time<<<1, 64, 0, c->stream>>>(c->tstate.as<double>(), c->dt,
+                                      c->totaltime, c->totaltime_next);
+  c->time_dirty = false;
+  return hip_check(c, hipGetLastError(), "k_set_time");
+}
+
+int sync_time(Ctx *c) {
+  if (!c->time_dirty) return 0;
+  double ts[3];
+  FSPH_HIP(c, hipMemcpyAsync(ts, c->tstate.p, sizeof(ts),
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->dt = ts[0];
+  c->totaltime = ts[1];
+  c->totaltime_next = ts[2];
+  c->time_dirty = false;
+  return 0;
 }
 
 int stage_reductions(Ctx *c, double out[10]) {

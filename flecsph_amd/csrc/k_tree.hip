@@ -128,10 +128,22 @@ __global__ __launch_bounds__(kT) void k_pair_count(
   if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  int d0 = 0, d1 = -1;
   if (i < n) {
-    const int d0 = Lm1(L, i) + 1, d1 = L[i];
+    d0 = Lm1(L, i) + 1;
+    d1 = L[i];
     cnt[i] = d1 >= d0 ? d1 - d0 + 1 : 0;
-    for (int d = d0; d <= d1; ++d) atomicAdd(&h[d], 1);
+  }
+  // per depth, the wavefront's count from a ballot (one LDS atomic per
+  // depth and wavefront instead of one per node)
+  int lo = d1 >= d0 ? d0 : kMaxDepth + 1, hi = d1 >= d0 ? d1 : -1;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  for (int d = lo; d <= hi; ++d) {
+    const uint64_t m = __ballot(d >= d0 && d <= d1);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&h[d], (int)__popcll(m));
   }
   __syncthreads();
   if (threadIdx.x <= kMaxDepth && h[threadIdx.x])
@@ -362,6 +374,8 @@ __global__ __launch_bounds__(kT) void k_group_write(
 
 }  // namespace
 
+int group_flags(Ctx *c);  // below
+
 int stage_tree(Ctx *c) {
   const int64_t n = c->n;
   const uint64_t *k = c->key.as<uint64_t>();
@@ -428,6 +442,10 @@ int stage_tree(Ctx *c) {
                                                 c->nchild.as<int32_t>(),
                                                 c->leaf_parent.as<int32_t>());
   FSPH_HIP(c, hipGetLastError());
+  // the groups need only node counts: flag them now and start copying their
+  // number to the host, which stage_groups waits for while the aggregates
+  // below run
+  FSPH_TRY(group_flags(c));
   // level order
   k_level_scatter<<<blocks_for(nn), kT, 0, c->stream>>>(
       c->ndepth.as<int32_t>(), nn, cursor, c->nlevel.as<int32_t>());
@@ -448,7 +466,7 @@ int stage_tree(Ctx *c) {
   return c->batch.n_inst ? batch_roots(c) : 0;
 }
 
-int stage_groups(Ctx *c) {
+int group_flags(Ctx *c) {
   const int64_t n = c->n, nn = c->nn;
   const int64_t tot = nn + n;
   FSPH_TRY(ensure(c, c->gflag, (size_t)tot * 4));
@@ -459,10 +477,18 @@ int stage_groups(Ctx *c) {
   FSPH_HIP(c, hipGetLastError());
   FSPH_TRY(scan_i32_to_i64(c, c->gflag.as<int32_t>(), c->gscan.as<int64_t>(),
                            tot));
-  int64_t ng = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&ng, c->gscan.as<int64_t>() + tot, 8,
+  FSPH_HIP(c, hipMemcpyAsync(c->mail_h + (kMail - 1),
+                             c->gscan.as<int64_t>() + tot, 8,
                              hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  FSPH_HIP(c, hipEventRecord(c->ev_groups, c->stream));
+  return 0;
+}
+
+int stage_groups(Ctx *c) {
+  const int64_t n = c->n, nn = c->nn;
+  const int64_t tot = nn + n;
+  FSPH_HIP(c, hipEventSynchronize(c->ev_groups));
+  const int64_t ng = c->mail_h[kMail - 1];
   c->ng = ng;
   FSPH_TRY(ensure(c, c->gstart, (size_t)ng * 4));
   FSPH_TRY(ensure(c, c->gcount, (size_t)ng * 4));
