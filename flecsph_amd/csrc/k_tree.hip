@@ -120,30 +120,35 @@ __global__ __launch_bounds__(kT) void k_prefix(const uint64_t *__restrict__ k,
 }
 
 // Nodes owned by entry i (depths Lm1(i)+1 .. L[i]) and, summed over the
-// grid, the number of nodes at each depth.
+// grid, the number of nodes at each depth.  A grid-stride loop over a
+// bounded grid: the depth counts are gathered per block in LDS and added to
+// the global ones once per block (few blocks, so few contended atomics).
+constexpr int kPairCountBlocks = 256;
 __global__ __launch_bounds__(kT) void k_pair_count(
     const int32_t *__restrict__ L, int64_t n, int32_t *__restrict__ cnt,
     unsigned long long *__restrict__ lvl) {
   __shared__ int32_t h[kMaxDepth + 1];
   if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  int d0 = 0, d1 = -1;
-  if (i < n) {
-    d0 = Lm1(L, i) + 1;
-    d1 = L[i];
-    cnt[i] = d1 >= d0 ? d1 - d0 + 1 : 0;
-  }
-  // per depth, the wavefront's count from a ballot (one LDS atomic per
-  // depth and wavefront instead of one per node)
-  int lo = d1 >= d0 ? d0 : kMaxDepth + 1, hi = d1 >= d0 ? d1 : -1;
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, __shfl_xor(lo, o, 64));
-    hi = max(hi, __shfl_xor(hi, o, 64));
-  }
-  for (int d = lo; d <= hi; ++d) {
-    const uint64_t m = __ballot(d >= d0 && d <= d1);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&h[d], (int)__popcll(m));
+  for (int64_t i0 = (int64_t)blockIdx.x * kT; i0 < n;
+       i0 += (int64_t)gridDim.x * kT) {
+    const int64_t i = i0 + threadIdx.x;
+    int d0 = 0, d1 = -1;
+    if (i < n) {
+      d0 = Lm1(L, i) + 1;
+      d1 = L[i];
+      cnt[i] = d1 >= d0 ? d1 - d0 + 1 : 0;
+    }
+    // per depth, the wavefront's count from a ballot
+    int lo = d1 >= d0 ? d0 : kMaxDepth + 1, hi = d1 >= d0 ? d1 : -1;
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    for (int d = lo; d <= hi; ++d) {
+      const uint64_t m = __ballot(d >= d0 && d <= d1);
+      if ((threadIdx.x & 63) == 0 && m) atomicAdd(&h[d], (int)__popcll(m));
+    }
   }
   __syncthreads();
   if (threadIdx.x <= kMaxDepth && h[threadIdx.x])
@@ -392,8 +397,8 @@ int stage_tree(Ctx *c) {
   unsigned long long *cursor = lvl + (kMaxDepth + 2);
   FSPH_HIP(c, hipMemsetAsync(lvl, 0, (kMaxDepth + 2) * 8, c->stream));
   k_prefix<<<blocks_for(n), kT, 0, c->stream>>>(k, n, sg.inst, L);
-  k_pair_count<<<blocks_for(n), kT, 0, c->stream>>>(
-      L, n, c->pair_cnt.as<int32_t>(), lvl);
+  k_pair_count<<<std::min<unsigned>(blocks_for(n), kPairCountBlocks), kT, 0,
+                 c->stream>>>(L, n, c->pair_cnt.as<int32_t>(), lvl);
   FSPH_HIP(c, hipGetLastError());
   int64_t *off = c->pair_off.as<int64_t>();
   FSPH_TRY(scan_i32_to_i64(c, c->pair_cnt.as<int32_t>(), off, n));
