@@ -196,6 +196,7 @@ struct Ctx {
     DBuf cnode, cstart, ccount, cbucket, cchild, ccom, cmass, crad, cQ, cL;
     int64_t cap_q = 0, cap_m2l = 0, cap_p2p = 0;
     int64_t n_m2l = 0, n_p2p = 0;
+    int levels = 0;  // levels of the last pair recursion
     DBuf qa, qb, m2l, p2p, ctr;
     DBuf m2l_off, m2l_src, p2p_off, p2p_src, cursor;
   } grav;

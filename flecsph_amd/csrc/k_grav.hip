@@ -182,10 +182,11 @@ __device__ inline int wave_excl_scan(int v, int *total) {
   return inc - v;
 }
 
-// Reserves c[q] slots for this thread in each of three global lists, one
-// atomic per list per workgroup.  Every thread of the block must call it.
-__device__ inline void block_reserve3(unsigned long long *ctr, const int c[3],
-                                      int64_t o[3]) {
+// Reserves c[q] slots for this thread in each of three global lists (the
+// next level's queue, M2L, P2P), one atomic per list per workgroup.  Every
+// thread of the block must call it.
+__device__ inline void block_reserve3(unsigned long long *ctr, int level,
+                                      const int c[3], int64_t o[3]) {
   constexpr int kWaves = kT / 64;
   __shared__ int wsum[3][kWaves];
   __shared__ unsigned long long base[3][kWaves];
@@ -201,7 +202,8 @@ __device__ inline void block_reserve3(unsigned long long *ctr, const int c[3],
     const int q = threadIdx.x;
     unsigned long long tot = 0;
     for (int i = 0; i < kWaves; ++i) tot += (unsigned long long)wsum[q][i];
-    unsigned long long b = tot ? atomicAdd(&ctr[q], tot) : 0ull;
+    unsigned long long *slot = q == 0 ? &ctr[3 + level] : &ctr[q - 1];
+    unsigned long long b = tot ? atomicAdd(slot, tot) : 0ull;
     for (int i = 0; i < kWaves; ++i) {
       base[q][i] = b;
       b += (unsigned long long)wsum[q][i];
@@ -214,91 +216,130 @@ __device__ inline void block_reserve3(unsigned long long *ctr, const int c[3],
 struct PairArgs {
   Cells c;
   const int2 *in;
-  int64_t n_in;
   int2 *out, *m2l, *p2p;
-  unsigned long long *ctr;  // [0] out, [1] m2l, [2] p2p
+  // ctr[0] M2L pairs, ctr[1] P2P pairs, ctr[2 + k] input pairs of level k
+  unsigned long long *ctr;
+  int level;
   int64_t cap_out, cap_m2l, cap_p2p;
   double theta;
   int *flags;
 };
 
+constexpr int kPairItems = 4;      // pairs per thread per round
+constexpr int kPairBlocks = 2048;  // fixed grid of a level (grid-stride)
+
+// One level of the dual-tree recursion.  The level's input count is read
+// from ctr[2 + level] (written by the previous level), so levels are queued
+// back to back without the host; a level with no input exits at once.
 __global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  int A = -1, B = -1, kind = 0, nout = 0;
-  int32_t ch[8];
-  int nch = 0;
-  if (i < a.n_in) {
-    const int2 pr = a.in[i];
-    A = pr.x;
-    B = pr.y;
-    if (A == B) {
-      if (a.c.bucket[A]) {
-        kind = 3;
-      } else {
-        kind = 1;
-        for (int d = 0; d < 8; ++d) {
-          const int32_t q = a.c.child[8 * (int64_t)A + d];
-          if (q >= 0) ch[nch++] = q;
+  const unsigned long long nin = a.ctr[2 + a.level];
+  const int64_t n_in =
+      (int64_t)(nin < (unsigned long long)a.cap_out ? nin : a.cap_out);
+  for (int64_t i0 = (int64_t)blockIdx.x * kT * kPairItems; i0 < n_in;
+       i0 += (int64_t)gridDim.x * kT * kPairItems) {
+    int A[kPairItems], B[kPairItems], kind[kPairItems], nch[kPairItems];
+    int32_t ch[kPairItems][8];
+    int cnts[3] = {0, 0, 0};
+#pragma unroll
+    for (int it = 0; it < kPairItems; ++it) {
+      // a thread's items are consecutive: the lists keep the input order
+      // (runs of equal sinks stay together for the bucketing)
+      const int64_t i = i0 + (int64_t)threadIdx.x * kPairItems + it;
+      A[it] = B[it] = -1;
+      kind[it] = 0;
+      nch[it] = 0;
+      if (i >= n_in) continue;
+      const int2 pr = a.in[i];
+      A[it] = pr.x;
+      B[it] = pr.y;
+      if (pr.x == pr.y) {
+        if (a.c.bucket[pr.x]) {
+          kind[it] = 3;
+        } else {
+          kind[it] = 1;
+          for (int d = 0; d < 8; ++d) {
+            const int32_t q = a.c.child[8 * (int64_t)pr.x + d];
+            if (q >= 0) ch[it][nch[it]++] = q;
+          }
+          cnts[0] += nch[it] * nch[it];
         }
-        nout = nch * nch;
-      }
-    } else {
+      } else {
 #pragma clang fp contract(off)
-      const double *ca = a.c.com + 3 * (int64_t)A, *cb = a.c.com + 3 * (int64_t)B;
-      double s = 0;
-      for (int d = 0; d < 3; ++d) s += (ca[d] - cb[d]) * (ca[d] - cb[d]);
-      const double ra = a.c.rad[A], rb = a.c.rad[B];
-      const bool ba = a.c.bucket[A], bb = a.c.bucket[B];
-      if (ra + rb < a.theta * sqrt(s)) {
-        kind = 2;
-      } else if (ba && bb) {
-        kind = 3;
-      } else {
-        const bool split_a = !ba && (bb || ra > rb);
-        kind = split_a ? 4 : 5;
-        const int32_t S = split_a ? A : B;
-        for (int d = 0; d < 8; ++d) {
-          const int32_t q = a.c.child[8 * (int64_t)S + d];
-          if (q >= 0) ch[nch++] = q;
+        const double *ca = a.c.com + 3 * (int64_t)pr.x,
+                     *cb = a.c.com + 3 * (int64_t)pr.y;
+        double s = 0;
+        for (int d = 0; d < 3; ++d) s += (ca[d] - cb[d]) * (ca[d] - cb[d]);
+        const double ra = a.c.rad[pr.x], rb = a.c.rad[pr.y];
+        const bool ba = a.c.bucket[pr.x], bb = a.c.bucket[pr.y];
+        if (ra + rb < a.theta * sqrt(s)) {
+          kind[it] = 2;
+        } else if (ba && bb) {
+          kind[it] = 3;
+        } else {
+          const bool split_a = !ba && (bb || ra > rb);
+          kind[it] = split_a ? 4 : 5;
+          const int32_t S = split_a ? pr.x : pr.y;
+          for (int d = 0; d < 8; ++d) {
+            const int32_t q = a.c.child[8 * (int64_t)S + d];
+            if (q >= 0) ch[it][nch[it]++] = q;
+          }
+          cnts[0] += nch[it];
         }
-        nout = nch;
+      }
+      cnts[1] += kind[it] == 2;
+      cnts[2] += kind[it] == 3;
+    }
+    // ctr[2 + level + 1] counts the next level's input
+    int64_t offs[3];
+    block_reserve3(a.ctr, a.level, cnts, offs);
+    int64_t w = offs[0], mo = offs[1], po = offs[2];
+    const bool ok_out = w + cnts[0] <= a.cap_out;
+    const bool ok_m2l = mo + cnts[1] <= a.cap_m2l;
+    const bool ok_p2p = po + cnts[2] <= a.cap_p2p;
+    if (!ok_out || !ok_m2l || !ok_p2p)
+      atomicOr(a.flags, (ok_out ? 0 : 1) | (ok_m2l ? 0 : 2) | (ok_p2p ? 0 : 4));
+#pragma unroll
+    for (int it = 0; it < kPairItems; ++it) {
+      const int k = kind[it], n = nch[it];
+      if (k == 1 && ok_out) {
+        for (int k1 = 0; k1 < n; ++k1) {
+          a.out[w++] = make_int2(ch[it][k1], ch[it][k1]);
+          for (int k2 = k1 + 1; k2 < n; ++k2) {
+            a.out[w++] = make_int2(ch[it][k1], ch[it][k2]);
+            a.out[w++] = make_int2(ch[it][k2], ch[it][k1]);
+          }
+        }
+      } else if (k == 4 && ok_out) {
+        for (int q = 0; q < n; ++q) a.out[w++] = make_int2(ch[it][q], B[it]);
+      } else if (k == 5 && ok_out) {
+        for (int q = 0; q < n; ++q) a.out[w++] = make_int2(A[it], ch[it][q]);
+      } else if (k == 2 && ok_m2l) {
+        a.m2l[mo++] = make_int2(A[it], B[it]);
+      } else if (k == 3 && ok_p2p) {
+        a.p2p[po++] = make_int2(A[it], B[it]);
       }
     }
   }
-  const int cnts[3] = {nout, kind == 2, kind == 3};
-  int64_t offs[3];
-  block_reserve3(a.ctr, cnts, offs);
-  const int64_t o = offs[0], mo = offs[1], po = offs[2];
-  if (kind == 0) return;
-  if (nout) {
-    if (o + nout > a.cap_out) {
-      atomicOr(a.flags, 1);
-      return;
-    }
-    int64_t w = o;
-    if (kind == 1) {
-      for (int k1 = 0; k1 < nch; ++k1) {
-        a.out[w++] = make_int2(ch[k1], ch[k1]);
-        for (int k2 = k1 + 1; k2 < nch; ++k2) {
-          a.out[w++] = make_int2(ch[k1], ch[k2]);
-          a.out[w++] = make_int2(ch[k2], ch[k1]);
-        }
-      }
-    } else if (kind == 4) {
-      for (int k = 0; k < nch; ++k) a.out[w++] = make_int2(ch[k], B);
-    } else {
-      for (int k = 0; k < nch; ++k) a.out[w++] = make_int2(A, ch[k]);
-    }
-  } else if (kind == 2) {
-    if (mo < a.cap_m2l)
-      a.m2l[mo] = make_int2(A, B);
-    else
-      atomicOr(a.flags, 2);
-  } else if (kind == 3) {
-    if (po < a.cap_p2p)
-      a.p2p[po] = make_int2(A, B);
-    else
-      atomicOr(a.flags, 4);
+}
+
+// Mailbox of the pair recursion after `levels` levels: [0] pairs still
+// queued for level `levels`, [1] M2L pairs, [2] P2P pairs, [3] flags,
+// [4 + k] input pairs of level k (k < 56).
+__global__ void k_pair_mail(const unsigned long long *__restrict__ ctr,
+                            int levels, const int *__restrict__ flags,
+                            int64_t *__restrict__ mail) {
+  if (threadIdx.x != 0) return;
+  mail[0] = (int64_t)ctr[2 + levels];
+  mail[1] = (int64_t)ctr[0];
+  mail[2] = (int64_t)ctr[1];
+  mail[3] = *flags;
+  for (int k = 0; k < 56; ++k) mail[4 + k] = k <= levels ? (int64_t)ctr[2 + k] : 0;
+}
+
+__global__ void k_pair_init(int2 *q, unsigned long long *ctr) {
+  if (threadIdx.x == 0) {
+    q[0] = make_int2(0, 0);  // the root meets itself
+    ctr[2] = 1;
   }
 }
 
@@ -702,45 +743,60 @@ template <class T> int gravity_impl(Ctx *c) {
     G.cap_m2l = std::max<int64_t>(1 << 16, 256 * nc);
     G.cap_p2p = std::max<int64_t>(1 << 16, 64 * nc);
   }
-  FSPH_TRY(ensure(c, G.ctr, 64));
+  constexpr int kMaxLevels = 200;
+  FSPH_TRY(ensure(c, G.ctr, (size_t)(2 + kMaxLevels + 2) * 8));
   for (int attempt = 0;; ++attempt) {
     FSPH_TRY(ensure(c, G.qa, (size_t)G.cap_q * 8));
     FSPH_TRY(ensure(c, G.qb, (size_t)G.cap_q * 8));
     FSPH_TRY(ensure(c, G.m2l, (size_t)G.cap_m2l * 8));
     FSPH_TRY(ensure(c, G.p2p, (size_t)G.cap_p2p * 8));
     unsigned long long *ctr = G.ctr.as<unsigned long long>();
-    FSPH_HIP(c, hipMemsetAsync(ctr, 0, 64, c->stream));
-    FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
-    const int2 root = make_int2(0, 0);
-    FSPH_HIP(c, hipMemcpyAsync(G.qa.p, &root, 8, hipMemcpyHostToDevice,
+    FSPH_HIP(c, hipMemsetAsync(ctr, 0, (size_t)(2 + kMaxLevels + 2) * 8,
                                c->stream));
-    int64_t n_in = 1;
-    int2 *qin = G.qa.as<int2>(), *qout = G.qb.as<int2>();
-    unsigned long long h[3] = {0, 0, 0};
-    int fl = 0;
-    while (n_in > 0) {
-      FSPH_HIP(c, hipMemsetAsync(ctr, 0, 8, c->stream));
-      PairArgs pa{cells, qin, n_in, qout, G.m2l.as<int2>(), G.p2p.as<int2>(),
-                  ctr, G.cap_q, G.cap_m2l, G.cap_p2p, c->p.theta,
-                  c->flags.as<int>()};
-      k_fmm_pairs<<<blocks_for(n_in), kT, 0, c->stream>>>(pa);
+    FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+    k_pair_init<<<1, 64, 0, c->stream>>>(G.qa.as<int2>(), ctr);
+    int2 *q[2] = {G.qa.as<int2>(), G.qb.as<int2>()};
+    // Levels are queued in batches without host round trips (a level reads
+    // its input count on the device); after each batch one read-back says
+    // whether pairs are left for the next level.
+    int done = 0, fl = 0;
+    int64_t pending = 1, h1 = 0, h2 = 0;
+    int batch = std::max(8, G.levels + 2);
+    while (pending > 0) {
+      if (done + batch > kMaxLevels)
+        return fail(c, FSPH_E_CAP, "FMM recursion deeper than %d levels",
+                    kMaxLevels);
+      for (int k = done; k < done + batch; ++k) {
+        PairArgs pa{cells, q[k & 1], q[(k + 1) & 1], G.m2l.as<int2>(),
+                    G.p2p.as<int2>(), ctr, k, G.cap_q, G.cap_m2l, G.cap_p2p,
+                    c->p.theta, c->flags.as<int>()};
+        k_fmm_pairs<<<kPairBlocks, kT, 0, c->stream>>>(pa);
+      }
       FSPH_HIP(c, hipGetLastError());
-      FSPH_HIP(c, hipMemcpyAsync(h, ctr, 24, hipMemcpyDeviceToHost,
-                                 c->stream));
-      FSPH_TRY(read_flags(c, &fl));
+      done += batch;
+      k_pair_mail<<<1, 64, 0, c->stream>>>(ctr, done, c->flags.as<int>(),
+                                           c->mail.as<int64_t>());
+      FSPH_TRY(read_mail(c, 4 + 56));
+      pending = c->mail_h[0];
+      h1 = c->mail_h[1];
+      h2 = c->mail_h[2];
+      fl = (int)c->mail_h[3];
       if (fl) break;
-      n_in = (int64_t)h[0];
-      std::swap(qin, qout);
+      batch = 8;
     }
     if (!fl) {
-      G.n_m2l = (int64_t)h[1];
-      G.n_p2p = (int64_t)h[2];
+      int last = 0;
+      for (int k = 0; k < 56 && k <= done; ++k)
+        if (c->mail_h[4 + k]) last = k;
+      G.levels = last + 1;
+      G.n_m2l = h1;
+      G.n_p2p = h2;
       break;
     }
     if (attempt >= 8) return fail(c, FSPH_E_CAP, "FMM pair lists overflow");
     if (fl & 1) G.cap_q *= 2;
-    if (fl & 2) G.cap_m2l = std::max<int64_t>(2 * G.cap_m2l, (int64_t)h[1] + 1);
-    if (fl & 4) G.cap_p2p = std::max<int64_t>(2 * G.cap_p2p, (int64_t)h[2] + 1);
+    if (fl & 2) G.cap_m2l = std::max<int64_t>(2 * G.cap_m2l, h1 + 1);
+    if (fl & 4) G.cap_p2p = std::max<int64_t>(2 * G.cap_p2p, h2 + 1);
   }
   // ---- bucket the lists by sink
   FSPH_TRY(bucket_sort(c, G.m2l.as<int2>(), G.n_m2l, nc, G.m2l_off,
