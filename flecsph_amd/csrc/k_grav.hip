@@ -232,6 +232,9 @@ constexpr int kPairBlocks = 2048;  // fixed grid of a level (grid-stride)
 // from ctr[2 + level] (written by the previous level), so levels are queued
 // back to back without the host; a level with no input exits at once.
 __global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
+  // After an overflow (flagged by an earlier level) the lists are rebuilt
+  // with larger capacities: the rest of the batch does nothing.
+  if (*a.flags) return;
   const unsigned long long nin = a.ctr[2 + a.level];
   const int64_t n_in =
       (int64_t)(nin < (unsigned long long)a.cap_out ? nin : a.cap_out);
@@ -298,25 +301,34 @@ __global__ __launch_bounds__(kT) void k_fmm_pairs(PairArgs a) {
     const bool ok_p2p = po + cnts[2] <= a.cap_p2p;
     if (!ok_out || !ok_m2l || !ok_p2p)
       atomicOr(a.flags, (ok_out ? 0 : 1) | (ok_m2l ? 0 : 2) | (ok_p2p ? 0 : 4));
+    // every reserved slot below a capacity is written (a reservation that
+    // crosses it writes the part that fits), so no list holds unwritten
+    // entries below its capacity
+    auto put = [](int2 *l, int64_t &at, int64_t cap, int2 v) {
+      if (at < cap) l[at] = v;
+      ++at;
+    };
 #pragma unroll
     for (int it = 0; it < kPairItems; ++it) {
       const int k = kind[it], n = nch[it];
-      if (k == 1 && ok_out) {
+      if (k == 1) {
         for (int k1 = 0; k1 < n; ++k1) {
-          a.out[w++] = make_int2(ch[it][k1], ch[it][k1]);
+          put(a.out, w, a.cap_out, make_int2(ch[it][k1], ch[it][k1]));
           for (int k2 = k1 + 1; k2 < n; ++k2) {
-            a.out[w++] = make_int2(ch[it][k1], ch[it][k2]);
-            a.out[w++] = make_int2(ch[it][k2], ch[it][k1]);
+            put(a.out, w, a.cap_out, make_int2(ch[it][k1], ch[it][k2]));
+            put(a.out, w, a.cap_out, make_int2(ch[it][k2], ch[it][k1]));
           }
         }
-      } else if (k == 4 && ok_out) {
-        for (int q = 0; q < n; ++q) a.out[w++] = make_int2(ch[it][q], B[it]);
-      } else if (k == 5 && ok_out) {
-        for (int q = 0; q < n; ++q) a.out[w++] = make_int2(A[it], ch[it][q]);
-      } else if (k == 2 && ok_m2l) {
-        a.m2l[mo++] = make_int2(A[it], B[it]);
-      } else if (k == 3 && ok_p2p) {
-        a.p2p[po++] = make_int2(A[it], B[it]);
+      } else if (k == 4) {
+        for (int q = 0; q < n; ++q)
+          put(a.out, w, a.cap_out, make_int2(ch[it][q], B[it]));
+      } else if (k == 5) {
+        for (int q = 0; q < n; ++q)
+          put(a.out, w, a.cap_out, make_int2(A[it], ch[it][q]));
+      } else if (k == 2) {
+        put(a.m2l, mo, a.cap_m2l, make_int2(A[it], B[it]));
+      } else if (k == 3) {
+        put(a.p2p, po, a.cap_p2p, make_int2(A[it], B[it]));
       }
     }
   }
