@@ -18,6 +18,7 @@
 // Centres of mass and radii are the FP64 octree aggregates; multipole and
 // pair arithmetic is FP32 with FP64 position differences.
 #include <cfloat>
+#include <type_traits>
 
 #include "fsph_internal.h"
 
@@ -569,6 +570,13 @@ constexpr int kEvalB = 64;            // threads per bucket workgroup
 constexpr int kStageBuckets = 16;     // at most this many source buckets
 constexpr int kStage = 256;           // and particles staged per round
 
+// Positions in the P2P sums: FP32 evaluations stage them relative to the
+// sink bucket's centre of mass (one FP64 difference per staged source and
+// per sink, rounded once; the sources lie within a few bucket sizes), FP64
+// evaluations absolute.
+template <class T> using EPos = typename std::conditional<sizeof(T) == 4,
+                                                          float, double>::type;
+
 // One wavefront per bucket: its local expansion, then direct sums over its
 // P2P source buckets (unsoftened, skipping the particle itself, fmm.h:37-46,
 // 124-143).  Sources are staged in LDS kStageBuckets buckets at a time; a
@@ -576,7 +584,8 @@ constexpr int kStage = 256;           // and particles staged per round
 // which split the staged sources and combine by xor shuffles.
 template <class T>
 __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
-  __shared__ double sx[kStage], sy[kStage], sz[kStage];
+  using P = EPos<T>;
+  __shared__ P sx[kStage], sy[kStage], sz[kStage];
   __shared__ T sm[kStage];
   __shared__ int32_t sidx[kStage];
   __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
@@ -590,15 +599,19 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   const int k = t / LP, r = t & (LP - 1);
   const bool act = k < cnt;
   const int64_t p = st + k;
-  double px = 0, py = 0, pz = 0;
+  const double cx = a.c.com[3 * cell], cy = a.c.com[3 * cell + 1],
+               cz = a.c.com[3 * cell + 2];
+  const double ox = sizeof(T) == 4 ? cx : 0.0, oy = sizeof(T) == 4 ? cy : 0.0,
+               oz = sizeof(T) == 4 ? cz : 0.0;
+  P px = 0, py = 0, pz = 0;
   KS<T> f0, f1, f2, pp;
   if (act) {
-    px = a.x[p]; py = a.y[p]; pz = a.z[p];
+    const double X = a.x[p], Y = a.y[p], Z = a.z[p];
+    px = (P)(X - ox); py = (P)(Y - oy); pz = (P)(Z - oz);
     if (r == 0) {
       T phi, g0, g1, g2;
-      local_eval(a.L + 10 * cell, (T)(px - a.c.com[3 * cell]),
-                 (T)(py - a.c.com[3 * cell + 1]),
-                 (T)(pz - a.c.com[3 * cell + 2]), &phi, &g0, &g1, &g2);
+      local_eval(a.L + 10 * cell, (T)(X - cx), (T)(Y - cy), (T)(Z - cz),
+                 &phi, &g0, &g1, &g2);
       pp.add(phi);
       f0.add(-g0);
       f1.add(-g1);
@@ -629,7 +642,8 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
       int i = 0;
       while (boff[i + 1] <= q) ++i;
       const int64_t pi = bst[i] + (q - boff[i]);
-      sx[q] = a.x[pi]; sy[q] = a.y[pi]; sz[q] = a.z[pi];
+      sx[q] = (P)(a.x[pi] - ox); sy[q] = (P)(a.y[pi] - oy);
+      sz[q] = (P)(a.z[pi] - oz);
       sm[q] = (T)a.m[pi];
       sidx[q] = (int32_t)pi;
     }
