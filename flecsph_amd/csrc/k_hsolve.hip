@@ -6,18 +6,23 @@
 //   f(h) = h^3 sum_{b in S_a, b != a} m_b W(r_ab, h) - m_a (3/4pi)(N_t - 1)
 // by Newton steps inside a bisection bracket [lo, hi], hi = H_a, until
 // |dh| < tol h or maxit steps, starting from h_a or from a proposal h0.
-// One 128-thread workgroup per group, LP lanes per particle (as the SPH
-// passes); all iterations run in one launch, re-staging the group's gather
-// candidates in LDS each iteration, until every particle of the group has
-// converged.  Kernel sums in T with Kahan compensation; membership r < h and
-// the h arithmetic in FP64.
+// One 128-thread workgroup per group; all iterations run in one launch,
+// re-staging the group's gather candidates in LDS each iteration, until every
+// particle of the group has converged.  Each iteration's pairs are walked
+// with the balanced schedule of the pair passes (pair_sched.h: equal runs of
+// pairs per thread, rows of particles that are done take no pairs), and
+// thread k updates particle k's h.  Kernel sums in T with Kahan
+// compensation; membership r < h and the h arithmetic in FP64.
 #include <cmath>
 
 #include "fsph_internal.h"
+#include "pair_sched.h"
 
 namespace fsph {
 
 namespace {
+
+using namespace sched;
 
 constexpr int kB = 128;
 constexpr int kChunk = 128;
@@ -32,17 +37,6 @@ template <class T> struct KSum {
     s = t;
   }
 };
-
-template <class T> __device__ inline T xsum(T v, int LP) {
-  for (int o = LP >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ inline int lanes_per_particle(int gc) {
-  int lp = 1;
-  while (lp < 64 && gc * lp * 2 <= kB) lp <<= 1;
-  return lp;
-}
 
 // W(r, h) and r dW/dr of the SPH kernels (kernels.h:106-153, 502-566)
 template <class T>
@@ -95,21 +89,29 @@ template <class T> struct NewtonArgs {
   double kappa, c3, tol, ntol;
 };
 
+constexpr int kBand = 4;  // band entries kept per particle
+
 template <class T>
 __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
   __shared__ T sm[kChunk];
   __shared__ int32_t sid[kChunk];
   __shared__ uint32_t smask[kChunkW * kB];
+  // row k: particle k's position, h^2, band bounds and 1/h for the walk
+  __shared__ double qx[kB], qy[kB], qz[kB], qh2[kB];
+  __shared__ T qih[kB];
+  __shared__ int32_t rcnt[kB], bandn[kB];
+  __shared__ double bandv[kB][kBand];
+  __shared__ Sched<kB> S;
+  __shared__ Parts<T, 2, kB> R;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g], cc = a.ccount[gi];
   const int64_t mo = a.moff[gi], cb = gi * a.ccap;
-  const int LP = lanes_per_particle(gc);
-  const int k = threadIdx.x / LP, r = threadIdx.x & (LP - 1);
-  const bool active = k < gc;
-  const int64_t ia = gs + k;
+  const int t = threadIdx.x;
+  const bool active = t < gc;  // thread t owns particle t
+  const int64_t ia = gs + t;
   double px = 0, py = 0, pz = 0, ma = 0, H = 1, h = 1, lo = 0, hi = 1;
   if (active) {
     px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
@@ -133,69 +135,87 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   int it = 0, nb = 0;
   // The hard count #{b : r_ab < h} is wanted at the final h, one Newton
   // update after the last evaluation.  Each evaluation therefore also keeps
-  // (per lane, up to two) the r^2 of candidates within a relative band
+  // (up to kBand per particle) the r^2 of candidates within a relative band
   // 2 tol around the evaluated h, which covers a converged update; the
   // count at the new h then follows exactly from the count at the old one.
   // Only particles whose band overflowed get a final counting pass.
   const double band = 2.0 * a.tol;
   for (bool final_pass = false;;) {
-    KSum<T> s0, s1;
-    int cnt = 0, nbd = 0;
-    double bnd0 = 0, bnd1 = 0;
-    const double h2 = h * h;
-    const double lo2 = h2 * (1.0 - band) * (1.0 - band);
-    const double hi2 = h2 * (1.0 + band) * (1.0 + band);
-    const T th = (T)h, ih = T(1) / th;
     const bool eval = active && (final_pass ? need_count : !done);
+    const double h2 = h * h;
+    if (active) {
+      qx[t] = px; qy[t] = py; qz[t] = pz;
+      qh2[t] = h2;
+      qih[t] = T(1) / (T)h;
+    }
+    rcnt[t] = 0;
+    bandn[t] = 0;
+    KSum<T> s0, s1;  // particle t's sums over the chunks
     for (int c0 = 0; c0 < cc; c0 += kChunk) {
       const int nch = min(kChunk, cc - c0), nw = (nch + 31) >> 5;
-      for (int j = threadIdx.x; j < nch; j += kB) {
+      for (int j = t; j < nch; j += kB) {
         const int32_t b = a.cand[cb + c0 + j];
         sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
         sm[j] = (T)a.m[b];
         sid[j] = b;
       }
       const uint32_t *src = a.mask + mo + (int64_t)(c0 >> 5) * gc;
-      for (int q = threadIdx.x; q < nw * gc; q += kB) smask[q] = src[q];
+      for (int q = t; q < nw * gc; q += kB) smask[q] = src[q];
       __syncthreads();
-      if (eval) {
-        for (int w = r; w < nw; w += LP) {
-          uint32_t bits = smask[w * gc + k];
-          while (bits) {
-            const int j = w * 32 + __ffs(bits) - 1;
-            bits &= bits - 1u;
-            const double d0 = px - sx[j], d1 = py - sy[j], d2 = pz - sz[j];
+      const int np = schedule(S, smask, nw, gc, eval);
+      double wx = 0, wy = 0, wz = 0, wh2 = 0, wlo2 = 0, whi2 = 0;
+      T wih = 1;
+      int wk = 0, rc = 0;
+      int32_t wia = 0;
+      KSum<T> r0, r1;
+      walk_run(
+          S, smask, gc, np,
+          [&](int k) {
+            wx = qx[k]; wy = qy[k]; wz = qz[k];
+            wh2 = qh2[k];
+            wlo2 = wh2 * (1.0 - band) * (1.0 - band);
+            whi2 = wh2 * (1.0 + band) * (1.0 + band);
+            wih = qih[k];
+            wk = k;
+            wia = (int32_t)(gs + k);
+          },
+          [&](int j) {
+            const double d0 = wx - sx[j], d1 = wy - sy[j], d2 = wz - sz[j];
             const double rr2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (!final_pass && rr2 >= lo2 && rr2 <= hi2) {
-              if (nbd == 0)
-                bnd0 = rr2;
-              else if (nbd == 1)
-                bnd1 = rr2;
-              ++nbd;
+            if (!final_pass && rr2 >= wlo2 && rr2 <= whi2) {
+              const int sl = atomicAdd(&bandn[wk], 1);
+              if (sl < kBand) bandv[wk][sl] = rr2;
             }
-            if (!(rr2 < h2)) continue;
-            ++cnt;
-            if (final_pass || sid[j] == (int32_t)ia) continue;
+            if (!(rr2 < wh2)) return;
+            ++rc;
+            if (final_pass || sid[j] == wia) return;
             const T rr = (T)sqrt(rr2);
             T W, rdW;
-            kernel_w_rdw<T>(a.kernel, rr, ih, &W, &rdW);
-            s0.add(sm[j] * W);
-            s1.add(sm[j] * (-(T(3) * W + rdW) * ih));
-          }
-        }
-      }
+            kernel_w_rdw<T>(a.kernel, rr, wih, &W, &rdW);
+            r0.add(sm[j] * W);
+            r1.add(sm[j] * (-(T(3) * W + rdW) * wih));
+          },
+          [&](int k, int kind) {
+            put(R, 0, k, kind, r0.s - r0.c);
+            put(R, 1, k, kind, r1.s - r1.c);
+            r0 = KSum<T>();
+            r1 = KSum<T>();
+            if (rc) atomicAdd(&rcnt[k], rc);
+            rc = 0;
+          });
       __syncthreads();
+      if (eval && S.off[t + 1] > S.off[t]) {
+        s0.add(row_total<false>(S, R, 0, t, np));
+        s1.add(row_total<false>(S, R, 1, t, np));
+      }
     }
-    const T rho = xsum(s0.s, LP) - xsum(s0.c, LP);
-    const T drho = xsum(s1.s, LP) - xsum(s1.c, LP);
-    for (int o = LP >> 1; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    __syncthreads();  // counts and band entries of the pass are complete
+    const T rho = s0.s - s0.c, drho = s1.s - s1.c;
+    const int cnt = rcnt[t];
     if (final_pass) {
-      if (need_count) nb = cnt;
+      if (eval) nb = cnt;
       break;
     }
-    double hn = h;
-    bool fin = false, cv = false;
-    int adj = 0, ovf = 0;
     if (!done) {
       const double h3 = h * h * h;
       const double f = (double)rho * h3 - ma * a.c3;
@@ -205,25 +225,18 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
         lo = h;
       else
         hi = h;
-      hn = fp > 0 ? h - f / fp : -1.0;
+      double hn = fp > 0 ? h - f / fp : -1.0;
       if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
-      cv = fabs(hn - h) < a.tol * hn;
-      fin = cv || it + 1 >= a.maxit;
+      const bool cv = fabs(hn - h) < a.tol * hn;
+      const bool fin = cv || it + 1 >= a.maxit;
+      ++it;
       if (fin) {
         const double hn2 = hn * hn;
-        if (nbd > 0) adj += (bnd0 < hn2) - (bnd0 < h2);
-        if (nbd > 1) adj += (bnd1 < hn2) - (bnd1 < h2);
-        ovf = nbd > 2 || !(fabs(hn - h) <= 0.99 * band * h);
-      }
-    }
-    for (int o = LP >> 1; o > 0; o >>= 1) {
-      adj += __shfl_xor(adj, o, 64);
-      ovf |= __shfl_xor(ovf, o, 64);
-    }
-    if (!done) {
-      ++it;
-      h = hn;
-      if (fin) {
+        const int nbd = bandn[t];
+        int adj = 0;
+        for (int q = 0; q < nbd && q < kBand; ++q)
+          adj += (bandv[t][q] < hn2) - (bandv[t][q] < h2);
+        const bool ovf = nbd > kBand || !(fabs(hn - h) <= 0.99 * band * h);
         done = true;
         conv = cv;
         if (ovf)
@@ -231,6 +244,7 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
         else
           nb = cnt + adj;
       }
+      h = hn;
     }
     if (!__syncthreads_or(!done)) {
       if (!__syncthreads_or(need_count)) break;
@@ -244,16 +258,16 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
     const bool clamped = solve && (!conv || h >= H * (1.0 - 2.0 * a.tol));
     const bool far = solve && (4.0 * M_PI / 3.0) * fabs(f0) / ma > a.ntol;
     const bool fl = clamped || far;
-    const int nf = __syncthreads_count(fl && r == 0);
-    const int ncl = __syncthreads_count(clamped && r == 0);
-    if (threadIdx.x == 0) {
+    const int nf = __syncthreads_count(fl);
+    const int ncl = __syncthreads_count(clamped);
+    if (t == 0) {
       a.gflag[gi] = nf > 0;
       if (nf) atomicAdd(&a.cnt[0], nf);
       if (ncl) atomicAdd(&a.cnt[1], ncl);
     }
-    if (active && r == 0) a.flag[ia] = fl;
+    if (active) a.flag[ia] = fl;
   }
-  if (solve && r == 0) {
+  if (solve) {
     a.h[ia] = h;
     a.hngb[ia] = nb;
     a.hiter[ia] = a.only ? a.hiter[ia] + it : it;
