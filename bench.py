@@ -254,6 +254,9 @@ def side_workload(args):
         s.upload(parts)
         n = len(parts["x"])
         label = f"collapse_fcc_sphere_{n}_{name.split('_')[1]}_h"
+    if args.tuning:
+        from flecsph_amd import set_tuning
+        set_tuning(s, *(int(v) for v in args.tuning.split(",")))
     for k in range(args.warmup):
         s.step()
         if name == "sedov_batch64" and k == 0:
