@@ -126,17 +126,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# Tuning of both runs of the learned-h comparison (the autotuner is not used
+# there: the predictor would keep learning during its steps); the fastest of
+# the autotuner's candidates on this workload.
+LH_TUNING = (64, 64)
+
+
 def learned_h_comparison(n, steps, learn):
     """Newton-h and learned-h runs of the same collapse; per-step h-phase
     (HIP events) and wall times averaged over the steps after training."""
     import numpy as np
-    from flecsph_amd import BodySystem, _abi, generators
+    from flecsph_amd import BodySystem, _abi, generators, set_tuning
     parts, prm = generators.collapse(n=n)
     n_real = len(parts["x"])
     out = {}
     for name, mode in (("newton", _abi.H_NEWTON), ("learned", _abi.H_LEARNED)):
         g = BodySystem(_abi.default_params(
             **dict(prm, h_mode=mode, learn_steps=learn)))
+        set_tuning(g, *LH_TUNING)
         g.upload(parts)
         g.enable_timing(True)
         hms, sms = [], []
@@ -167,6 +174,7 @@ def learned_h_comparison(n, steps, learn):
         "workload": f"collapse_fcc_sphere_{n_real}_fmm_theta0.5_quad_leaf32",
         "steps": steps, "learn_steps": learn,
         "timed_steps": steps - learn,
+        "tuning": {"group_max": LH_TUNING[0], "chunk": LH_TUNING[1]},
         "newton_h_ms": nw["h_ms"], "learned_h_ms": le["h_ms"],
         "h_phase_speedup": nw["h_ms"] / le["h_ms"],
         "newton_step_ms": nw["step_ms"], "learned_step_ms": le["step_ms"],

@@ -558,11 +558,6 @@ template <class T> struct EvalArgs {
   T G;
 };
 
-template <class T> __device__ inline T lane_sum(T v, int LP) {
-  for (int o = LP >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 __device__ inline float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double fast_rsq(double x) { return 1.0 / sqrt(x); }
 
@@ -580,8 +575,9 @@ template <class T> using EPos = typename std::conditional<sizeof(T) == 4,
 // One wavefront per bucket: its local expansion, then direct sums over its
 // P2P source buckets (unsoftened, skipping the particle itself, fmm.h:37-46,
 // 124-143).  Sources are staged in LDS kStageBuckets buckets at a time; a
-// bucket of c particles gets LP = 64/c lanes per particle (a power of two),
-// which split the staged sources and combine by xor shuffles.
+// bucket of c particles gets LP = floor(64/c) lanes per particle, which
+// split the staged sources; the lanes' partial sums are added in lane order
+// through LDS.
 template <class T>
 __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   using P = EPos<T>;
@@ -589,14 +585,14 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   __shared__ T sm[kStage];
   __shared__ int32_t sidx[kStage];
   __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
+  __shared__ T part[4][kEvalB];
   const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
   if (!a.c.bucket[cell]) return;
   const int cnt = a.c.count[cell];
   const int64_t st = a.c.start[cell];
   const int t = threadIdx.x;
-  int LP = 1;
-  while (LP < 64 && cnt * LP * 2 <= kEvalB) LP <<= 1;
-  const int k = t / LP, r = t & (LP - 1);
+  const int LP = cnt > 0 && cnt < kEvalB ? kEvalB / cnt : 1;
+  const int k = t / LP, r = t - k * LP;
   const bool act = k < cnt;
   const int64_t p = st + k;
   const double cx = a.c.com[3 * cell], cy = a.c.com[3 * cell + 1],
@@ -663,10 +659,19 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
       }
     __syncthreads();
   }
-  const T g0 = lane_sum(f0.s, LP) - lane_sum(f0.c, LP);
-  const T g1 = lane_sum(f1.s, LP) - lane_sum(f1.c, LP);
-  const T g2 = lane_sum(f2.s, LP) - lane_sum(f2.c, LP);
-  const T ph = lane_sum(pp.s, LP) - lane_sum(pp.c, LP);
+  part[0][t] = f0.s - f0.c;
+  part[1][t] = f1.s - f1.c;
+  part[2][t] = f2.s - f2.c;
+  part[3][t] = pp.s - pp.c;
+  __syncthreads();
+  T g0 = 0, g1 = 0, g2 = 0, ph = 0;
+  if (act && r == 0)
+    for (int l = t; l < t + LP; ++l) {
+      g0 += part[0][l];
+      g1 += part[1][l];
+      g2 += part[2][l];
+      ph += part[3][l];
+    }
   if (act && r == 0) {
     a.gx[p] = g0;
     a.gy[p] = g1;
