@@ -356,14 +356,16 @@ __global__ void k_pair_init(int2 *q, unsigned long long *ctr) {
   }
 }
 
-// counting sort of a pair list by sink: counts, then scatter of sources
-// Pairs of one sink come in runs (the children of a split source cell are
-// listed together), so each wavefront updates a sink's counter once per run:
-// the first lane of a run adds the run's length.
+// counting sort of a pair list by sink: counts, then scatter of sources.
+// Each wavefront first sorts its 64 pairs by (sink, lane) with a bitonic
+// network of shuffles, so equal sinks of the wavefront sit together and a
+// sink's counter takes one atomic per wavefront; a sink's sources keep the
+// list order within the wavefront.
 struct SinkRun {
   int sink, head, len;  // head: first lane of this lane's run
 };
 
+// Runs of equal sinks over the wavefront's valid lanes (a prefix).
 __device__ inline SinkRun sink_run(bool valid, int sink) {
   const int lane = threadIdx.x & 63;
   int prev = __shfl_up(sink, 1, 64);
@@ -382,13 +384,49 @@ __device__ inline SinkRun sink_run(bool valid, int sink) {
   return r;
 }
 
+// The wavefront's keys in ascending order (bitonic network over the lanes).
+__device__ inline unsigned long long wave_sort(unsigned long long key) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const unsigned long long o = __shfl_xor(key, j, 64);
+      const bool asc = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      const unsigned long long mn = key < o ? key : o, mx = key < o ? o : key;
+      key = lower == asc ? mn : mx;
+    }
+  return key;
+}
+
+// Sorted view of the wavefront's pairs: this lane's sink (or -1) and the
+// lane the pair came from.
+struct SortedPair {
+  bool valid;
+  int sink, from;
+};
+__device__ inline SortedPair sort_pairs(bool valid, int sink) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long key =
+      valid ? (((unsigned long long)(uint32_t)sink) << 6) | (unsigned)lane
+            : ~0ull;
+  const unsigned long long s = wave_sort(key);
+  SortedPair r;
+  r.valid = s != ~0ull;
+  r.sink = r.valid ? (int)(s >> 6) : -1;
+  r.from = (int)(s & 63ull);
+  return r;
+}
+
 __global__ __launch_bounds__(kT) void k_count_sinks(const int2 *__restrict__ l,
                                                     int64_t n,
                                                     int32_t *__restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   const bool valid = i < n;
-  const SinkRun r = sink_run(valid, valid ? l[i].x : -1);
-  if (valid && r.len) atomicAdd(&cnt[r.sink], r.len);
+  const SortedPair sp = sort_pairs(valid, valid ? l[i].x : -1);
+  const SinkRun r = sink_run(sp.valid, sp.sink);
+  if (sp.valid && r.len) atomicAdd(&cnt[r.sink], r.len);
 }
 __global__ __launch_bounds__(kT) void k_scatter_sinks(
     const int2 *__restrict__ l, int64_t n,
@@ -396,11 +434,14 @@ __global__ __launch_bounds__(kT) void k_scatter_sinks(
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   const bool valid = i < n;
   const int2 pr = valid ? l[i] : make_int2(-1, 0);
-  const SinkRun r = sink_run(valid, pr.x);
+  const SortedPair sp = sort_pairs(valid, pr.x);
+  const int s_src = __shfl(pr.y, sp.from, 64);
+  const SinkRun r = sink_run(sp.valid, sp.sink);
   unsigned long long base = 0;
-  if (valid && r.len) base = atomicAdd(&cursor[r.sink], (unsigned long long)r.len);
+  if (sp.valid && r.len)
+    base = atomicAdd(&cursor[r.sink], (unsigned long long)r.len);
   base = __shfl(base, r.head, 64);
-  if (valid) src[base + (threadIdx.x & 63) - r.head] = pr.y;
+  if (sp.valid) src[base + (threadIdx.x & 63) - r.head] = s_src;
 }
 
 // ---------------------------------------------------------------------------
