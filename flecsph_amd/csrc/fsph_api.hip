@@ -26,10 +26,24 @@ int hip_check(Ctx *c, hipError_t e, const char *what) {
   return fail(c, FSPH_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// A buffer that has to grow again takes half as much again as it held (or
+// what is asked, if more): sizes that creep up step by step (neighbour
+// tiles, interaction lists of a collapsing cloud) then reallocate a
+// logarithmic number of times instead of on every step, each of which costs
+// a device synchronisation and a free.
 int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
   if (bytes <= b.bytes) return 0;
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
+  const size_t grown = b.bytes + b.bytes / 2;
+  hipError_t e = hipErrorOutOfMemory;
+  if (b.p && grown > bytes) {
+    e = hipMalloc(&p, grown);
+    if (e == hipSuccess)
+      bytes = grown;
+    else
+      (void)hipGetLastError();  // fall back to the exact size
+  }
+  if (e != hipSuccess) e = hipMalloc(&p, bytes);
   if (e != hipSuccess) return hip_check(c, e, "hipMalloc");
   if (keep && b.p && b.bytes) {
     e = hipMemcpyAsync(p, b.p, b.bytes, hipMemcpyDeviceToDevice, c->stream);

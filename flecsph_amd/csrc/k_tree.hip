@@ -255,92 +255,118 @@ struct CofmArgs {
 
 // physics::compute_cofm (default_physics.h:60-119): leaf children first,
 // then node children, each in digit order (tree_topology.h:1949-1966).
-__device__ inline void cofm_node(const CofmArgs &a, int64_t nd) {
-#pragma clang fp contract(off)
-  int32_t ch[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) ch[j] = a.child[nd * 8 + j];
-  double cx = 0, cy = 0, cz = 0, mass = 0;
-  double bmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
-  double bmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (int j = 0; j < 8; ++j) {
-    if (ch[j] > -2) continue;
-    const int64_t p = leaf_of(ch[j]);
-    const double m = a.m[p], X[3] = {a.x[p], a.y[p], a.z[p]}, hh = a.h[p];
-    cx += m * X[0];
-    cy += m * X[1];
-    cz += m * X[2];
-    mass += m;
-    for (int d = 0; d < 3; ++d) {
-      const double lo = X[d] - hh / 2., hi = X[d] + hh / 2.;
-      bmin[d] = (lo < bmin[d]) ? lo : bmin[d];
-      bmax[d] = (bmax[d] < hi) ? hi : bmax[d];
-    }
-  }
-  for (int j = 0; j < 8; ++j) {
-    const int32_t c = ch[j];
-    if (c < 0) continue;
-    const double m = a.mass[c];
-    cx += m * a.com[3 * c];
-    cy += m * a.com[3 * c + 1];
-    cz += m * a.com[3 * c + 2];
-    mass += m;
-    for (int d = 0; d < 3; ++d) {
-      const double lo = a.box[6 * c + d], hi = a.box[6 * c + 3 + d];
-      bmin[d] = (lo < bmin[d]) ? lo : bmin[d];
-      bmax[d] = (bmax[d] < hi) ? hi : bmax[d];
-    }
-  }
-  cx /= mass;
-  cy /= mass;
-  cz /= mass;
-  double radius = 0, lap = 0;
-  for (int j = 0; j < 8; ++j) {
-    if (ch[j] > -2) continue;
-    const int64_t p = leaf_of(ch[j]);
-    const double dx = cx - a.x[p], dy = cy - a.y[p], dz = cz - a.z[p];
-    double s = 0;
-    s += dx * dx;
-    s += dy * dy;
-    s += dz * dz;
-    const double dist = sqrt(s);
-    radius = (radius < dist) ? dist : radius;
-    const double l = dist + a.h[p];
-    lap = (lap < l) ? l : lap;
-  }
-  for (int j = 0; j < 8; ++j) {
-    const int32_t c = ch[j];
-    if (c < 0) continue;
-    const double dx = cx - a.com[3 * c], dy = cy - a.com[3 * c + 1],
-                 dz = cz - a.com[3 * c + 2];
-    double s = 0;
-    s += dx * dx;
-    s += dy * dy;
-    s += dz * dz;
-    const double dist = sqrt(s);
-    const double r = dist + a.rad[c];
-    radius = (radius < r) ? r : radius;
-    const double l = dist + a.rad[c] + a.lap[c];
-    lap = (lap < l) ? l : lap;
-  }
-  a.com[3 * nd] = cx;
-  a.com[3 * nd + 1] = cy;
-  a.com[3 * nd + 2] = cz;
-  a.mass[nd] = mass;
-  a.rad[nd] = radius;
-  a.lap[nd] = lap;
-  for (int d = 0; d < 3; ++d) {
-    a.box[6 * nd + d] = bmin[d];
-    a.box[6 * nd + 3 + d] = bmax[d];
-  }
-}
+// Eight lanes per node, one per child digit: each lane loads its child (a
+// particle or a node aggregate) in one round trip, and the order-dependent
+// reductions (mass-weighted sums, box bounds, radius and lap maxima) are then
+// taken serially in the reference's order from LDS, one lane per component,
+// so the results are those of a serial walk over the children.
+constexpr int kCofmNodes = kT / 8;  // nodes per workgroup
 
-// One level of the tree (the level's node ids).
 __global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
                                              const int32_t *__restrict__ ids,
                                              int64_t cnt) {
-  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t < cnt) cofm_node(a, ids[t]);
+#pragma clang fp contract(off)
+  // per child: m*x, m*y, m*z, m, box lo xyz, box hi xyz, radius, lap terms
+  __shared__ double v[kCofmNodes][12][8];
+  __shared__ double sums[kCofmNodes][4];
+  __shared__ int kind[kCofmNodes][8];  // 0 none, 1 particle, 2 node
+  const int ln = threadIdx.x >> 3, j = threadIdx.x & 7;
+  const int64_t t = (int64_t)blockIdx.x * kCofmNodes + ln;
+  const bool valid = t < cnt;
+  const int64_t nd = valid ? ids[t] : 0;
+  const int32_t ch = valid ? a.child[nd * 8 + j] : -1;
+  int kd = 0;
+  double X[3] = {0, 0, 0}, m = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  double ra = 0, rb = 0;  // radius = dist + ra, lap = dist + ra + rb
+  if (ch <= -2) {
+    const int64_t p = leaf_of(ch);
+    kd = 1;
+    m = a.m[p];
+    X[0] = a.x[p];
+    X[1] = a.y[p];
+    X[2] = a.z[p];
+    const double hh = a.h[p];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = X[d] - hh / 2.;
+      hi[d] = X[d] + hh / 2.;
+    }
+    rb = hh;
+  } else if (ch >= 0) {
+    kd = 2;
+    m = a.mass[ch];
+    for (int d = 0; d < 3; ++d) {
+      X[d] = a.com[3 * ch + d];
+      lo[d] = a.box[6 * ch + d];
+      hi[d] = a.box[6 * ch + 3 + d];
+    }
+    ra = a.rad[ch];
+    rb = a.lap[ch];
+  }
+  kind[ln][j] = kd;
+  v[ln][0][j] = m * X[0];
+  v[ln][1][j] = m * X[1];
+  v[ln][2][j] = m * X[2];
+  v[ln][3][j] = m;
+  for (int d = 0; d < 3; ++d) {
+    v[ln][4 + d][j] = lo[d];
+    v[ln][7 + d][j] = hi[d];
+  }
+  __syncthreads();
+  int k8[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) k8[q] = kind[ln][q];
+  // components 0-3 (sums), 4-6 (box minima), 7-9 (box maxima): lane j takes
+  // component j and, for j < 2, component j + 8
+  double out[2] = {0, 0};
+  for (int r = 0; r < 2; ++r) {
+    const int q = j + 8 * r;
+    if (q >= 10) break;
+    double acc = q < 4 ? 0. : (q < 7 ? DBL_MAX : -DBL_MAX);
+    for (int pass = 1; pass <= 2; ++pass)
+      for (int u = 0; u < 8; ++u) {
+        if (k8[u] != pass) continue;
+        const double w = v[ln][q][u];
+        if (q < 4)
+          acc += w;
+        else if (q < 7)
+          acc = (w < acc) ? w : acc;
+        else
+          acc = (acc < w) ? w : acc;
+      }
+    out[r] = acc;
+  }
+  if (j < 4) sums[ln][j] = out[0];
+  __syncthreads();
+  const double mass = sums[ln][3];
+  const double cx = sums[ln][0] / mass, cy = sums[ln][1] / mass,
+               cz = sums[ln][2] / mass;
+  {
+    const double dx = cx - X[0], dy = cy - X[1], dz = cz - X[2];
+    double s = 0;
+    s += dx * dx;
+    s += dy * dy;
+    s += dz * dz;
+    const double rr = sqrt(s) + ra;
+    v[ln][10][j] = rr;
+    v[ln][11][j] = rr + rb;
+  }
+  __syncthreads();
+  if (!valid) return;
+  if (j >= 6) {  // lane 6: radius, lane 7: lap
+    const int q = j + 4;
+    double acc = 0;
+    for (int pass = 1; pass <= 2; ++pass)
+      for (int u = 0; u < 8; ++u) {
+        if (k8[u] != pass) continue;
+        const double w = v[ln][q][u];
+        acc = (acc < w) ? w : acc;
+      }
+    (j == 6 ? a.rad : a.lap)[nd] = acc;
+  }
+  if (j < 3) a.com[3 * nd + j] = j == 0 ? cx : (j == 1 ? cy : cz);
+  if (j == 3) a.mass[nd] = mass;
+  if (j >= 4) a.box[6 * nd + (j - 4)] = out[0];
+  if (j < 2) a.box[6 * nd + 4 + j] = out[1];
 }
 
 // Groups (tree_topology.h:222-235): nodes with at most gmax particles whose
@@ -464,7 +490,8 @@ int stage_tree(Ctx *c) {
   for (int d = c->max_depth; d >= 0; --d) {
     const int64_t cntd = c->level_off[d + 1] - c->level_off[d];
     if (!cntd) continue;
-    k_cofm<<<blocks_for(cntd), kT, 0, c->stream>>>(
+    k_cofm<<<(unsigned)((cntd + kCofmNodes - 1) / kCofmNodes), kT, 0,
+             c->stream>>>(
         a, c->nlevel.as<int32_t>() + c->level_off[d], cntd);
   }
   FSPH_HIP(c, hipGetLastError());
