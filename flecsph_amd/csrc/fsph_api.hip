@@ -330,7 +330,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   fsph::DBuf *all[] = {&c->key, &c->key2, &c->val, &c->val2, &c->perm,
-                       &c->hist, &c->scan_tmp, &c->red_tmp, &c->flags,
+                       &c->hist, &c->scan_tmp, &c->scan_state, &c->red_tmp, &c->flags,
                        &c->mail, &c->range_d,
                        &c->L, &c->pair_cnt, &c->pair_off, &c->nkey,
                        &c->ndepth, &c->nstart, &c->ncount, &c->nparent,

@@ -54,6 +54,7 @@ enum : int {
   kFlagCand = 4,
   kFlagNaN = 8,
   kFlagNeg = 16,
+  kFlagScanStall = 32,
 };
 
 struct Ctx;
@@ -96,6 +97,9 @@ struct Ctx {
   DBuf key, key2, val, val2, perm;
   DBuf hist;           // radix histograms
   DBuf scan_tmp;
+  DBuf scan_state;     // single-pass scan: two tile-state halves, counters
+  int64_t scan_cap = 0;  // tiles per half
+  int scan_parity = 0;   // half used by the next scan
   DBuf red_tmp;        // reduction partials
   DBuf flags;          // int error flags
   DBuf mail;           // device mailbox: scalars read back in one copy

@@ -431,6 +431,8 @@ int stage_tree(Ctx *c) {
   k_tree_mail<<<1, 64, 0, c->stream>>>(off + n, c->flags.as<int>(), lvl,
                                        cursor, c->mail.as<int64_t>());
   FSPH_TRY(read_mail(c, 2 + kMaxDepth + 1));
+  if (c->mail_h[1] & kFlagScanStall)
+    return fail(c, FSPH_E_HIP, "scan: a tile's look-back did not complete");
   if (c->mail_h[1] & kFlagDupKey)
     return fail(c, FSPH_E_DUPKEY,
                 "two particles share a Morton key (positions closer than "

@@ -75,6 +75,31 @@ def test_keys_sort_tree_groups_bit_exact(oracle_mod, name):
     assert tree_sets(g.tree()) == tree_sets(o.tree())
 
 
+def test_tree_and_neighbors_bit_exact_past_one_workgroup_scan(oracle_mod):
+    """Sedov 32^3 with its periodic copies (about 60,000 particles): the
+    scans of the copy counts, node counts and group flags run over many
+    tiles (more than one workgroup's 16,384 items), so the multi-tile scan
+    path is what this case checks against the oracle."""
+    parts, prm = generators.sedov(n_side=32)
+    prm = dict(prm, precision=_abi.PREC_FP32)
+    g, o = both(oracle_mod, parts, prm)
+    for step in range(2):
+        print(f"sedov32 update {step}", flush=True)
+        g.update_iteration()
+        o.update_iteration()
+        assert g.count() == o.count()
+        assert g.count()[0] > 3 * 16384
+        assert np.array_equal(g.get("key"), o.get("key"))
+        assert np.array_equal(g.get("id"), o.get("id"))
+        assert tree_sets(g.tree()) == tree_sets(o.tree())
+    og, ig = g.neighbor_lists()
+    oo, io = o.neighbor_lists()
+    assert np.array_equal(og, oo)
+    io_sorted = np.concatenate([np.sort(io[oo[i]:oo[i + 1]])
+                                for i in range(len(oo) - 1)])
+    assert np.array_equal(ig, io_sorted)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_neighbor_lists_bit_exact(oracle_mod, name):
     parts, prm = get_case(name, _abi.PREC_FP32)
