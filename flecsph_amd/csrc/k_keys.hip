@@ -140,64 +140,79 @@ struct GenArgs {
   int nf;
 };
 
-// Appends the WALL copies of particle i at n_real + off[i] + j in the order
-// of boundary.h:170-215: faces x, y, z, then corners xy, xz, zy, xyz.  In
-// batched mode ids count within the instance (n_real of the instance plus
-// the creation index among its copies; segr: offsets of the instances'
-// real particles).
-__global__ __launch_bounds__(kT) void k_gen_copies(
-    GenArgs a, double *x, double *y, double *z, int64_t *id,
-    const int64_t *__restrict__ off, int64_t n, int64_t n_real, Box b,
-    Halo hl, const int64_t *__restrict__ nreal,
-    const int64_t *__restrict__ segr) {
-#pragma clang fp contract(off)
+// The WALL copies of particle i go to n_real + off[i] + j in the order of
+// boundary.h:170-215: faces x, y, z, then corners xy, xz, zy, xyz.  First
+// each particle on an edge writes, for each of its copies, its index and
+// the copy's kind (0-2 face x/y/z, 3-5 corner xy/xz/zy, 6 corner xyz) to
+// the copy's slot; then one thread per copy fills the copy's fields, so the
+// field stores are coalesced.
+__global__ __launch_bounds__(kT) void k_copy_map(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const int64_t *__restrict__ off, int64_t n,
+    Box b, Halo hl, uint64_t *__restrict__ map) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {x[i], y[i], z[i]};
-  const double halo = hl.of(i);
-  const int on = on_edge_mask(p, b, halo);
+  const int on = on_edge_mask(p, b, hl.of(i));
   if (!on) return;
-  int64_t w = n_real + off[i];
+  int64_t w = off[i];
+  const uint64_t src = (uint64_t)i << 3;
+  for (int d = 0; d < 3; ++d)
+    if (on >> d & 1) map[w++] = src | (uint64_t)d;
+  const int pairs[3][2] = {{0, 1}, {0, 2}, {2, 1}};
+  for (int q = 0; q < 3; ++q)
+    if ((on >> pairs[q][0] & 1) && (on >> pairs[q][1] & 1))
+      map[w++] = src | (uint64_t)(3 + q);
+  if (on == 7) map[w] = src | 6u;
+}
+
+// Fills copy t (at n_real + t) from its source particle.  In batched mode
+// ids count within the instance (n_real of the instance plus the creation
+// index among its copies; segr: offsets of the instances' real particles).
+__global__ __launch_bounds__(kT) void k_copy_fill(
+    GenArgs a, double *x, double *y, double *z, int64_t *id,
+    const uint64_t *__restrict__ map, int64_t extra, int64_t n_real, Box b,
+    Halo hl, const int64_t *__restrict__ off,
+    const int64_t *__restrict__ nreal, const int64_t *__restrict__ segr) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= extra) return;
+  const uint64_t e = map[t];
+  const int64_t i = (int64_t)(e >> 3);
+  const int kind = (int)(e & 7u);
+  const int64_t dst = n_real + t;
+  for (int f = 0; f < a.nf; ++f) {
+    if (a.esize[f] == 8)
+      static_cast<uint64_t *>(a.f[f])[dst] = static_cast<uint64_t *>(a.f[f])[i];
+    else
+      static_cast<uint32_t *>(a.f[f])[dst] = static_cast<uint32_t *>(a.f[f])[i];
+  }
+  const double p[3] = {x[i], y[i], z[i]};
+  const double halo = hl.of(i);
+  double q[3] = {p[0], p[1], p[2]};
+  if (kind < 3) {
+    const int d = kind;
+    q[d] = (p[d] + halo > b.hi[d]) ? p[d] - (b.hi[d] - b.lo[d])
+                                   : p[d] + (b.hi[d] - b.lo[d]);
+  } else if (kind < 6) {
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {2, 1}};
+    const int d0 = pairs[kind - 3][0], d1 = pairs[kind - 3][1];
+    q[d0] = corner_coord(p[d0], b.lo[d0], b.hi[d0], halo);
+    q[d1] = corner_coord(p[d1], b.lo[d1], b.hi[d1], halo);
+  } else {
+    for (int d = 0; d < 3; ++d)
+      q[d] = corner_coord(p[d], b.lo[d], b.hi[d], halo);
+  }
+  x[dst] = q[0];
+  y[dst] = q[1];
+  z[dst] = q[2];
+  // WALL ids: n_real + creation index (boundary.h:235-244)
   int64_t id_base = 0;  // id of copy dst is dst - id_base
   if (hl.inst) {
     const int ii = hl.inst[i];
     id_base = n_real + off[segr[ii]] - nreal[ii];
   }
-  double *X[3] = {x, y, z};
-  auto clone = [&](int64_t dst) {
-    for (int f = 0; f < a.nf; ++f) {
-      if (a.esize[f] == 8)
-        static_cast<uint64_t *>(a.f[f])[dst] = static_cast<uint64_t *>(a.f[f])[i];
-      else
-        static_cast<uint32_t *>(a.f[f])[dst] = static_cast<uint32_t *>(a.f[f])[i];
-    }
-    X[0][dst] = p[0]; X[1][dst] = p[1]; X[2][dst] = p[2];
-    // WALL ids: n_real + creation index (boundary.h:235-244)
-    id[dst] = dst - id_base;
-  };
-  for (int d = 0; d < 3; ++d) {
-    if (!(on >> d & 1)) continue;
-    clone(w);
-    X[d][w] = (p[d] + halo > b.hi[d]) ? p[d] - (b.hi[d] - b.lo[d])
-                                      : p[d] + (b.hi[d] - b.lo[d]);
-    ++w;
-  }
-  const int pairs[3][2] = {{0, 1}, {0, 2}, {2, 1}};
-  for (int q = 0; q < 3; ++q) {
-    const int d0 = pairs[q][0], d1 = pairs[q][1];
-    if ((on >> d0 & 1) && (on >> d1 & 1)) {
-      clone(w);
-      X[d0][w] = corner_coord(p[d0], b.lo[d0], b.hi[d0], halo);
-      X[d1][w] = corner_coord(p[d1], b.lo[d1], b.hi[d1], halo);
-      ++w;
-    }
-  }
-  if (on == 7) {
-    clone(w);
-    for (int d = 0; d < 3; ++d)
-      X[d][w] = corner_coord(p[d], b.lo[d], b.hi[d], halo);
-    ++w;
-  }
+  id[dst] = dst - id_base;
 }
 
 // ---------------------------------------------------------------------------
@@ -600,19 +615,29 @@ int stage_boundary(Ctx *c) {
     FSPH_TRY(ensure_particles(c, (int64_t)((n + extra) * 1.25) + 1024, true));
   }
   if (hl.inst) hl.inst = c->f[PF_INST].as<int32_t>();  // may have moved
-  int pf[kMaxPermFields];
-  const int nf = persistent_fields(c, pf);
-  GenArgs a{};
-  a.nf = nf;
-  for (int q = 0; q < nf; ++q) {
-    a.f[q] = c->f[pf[q]].p;
-    a.esize[q] = (int)field_size(c, pf[q]);
+  if (extra > 0) {
+    int pf[kMaxPermFields];
+    const int np = persistent_fields(c, pf);
+    GenArgs a{};  // the fields copied as they are (positions and ids below)
+    for (int q = 0; q < np; ++q) {
+      if (pf[q] == PF_X || pf[q] == PF_Y || pf[q] == PF_Z || pf[q] == PF_ID)
+        continue;
+      a.f[a.nf] = c->f[pf[q]].p;
+      a.esize[a.nf++] = (int)field_size(c, pf[q]);
+    }
+    // copy slots -> (source, kind), in the sort's spare key buffer
+    FSPH_TRY(ensure(c, c->key2, (size_t)c->pcap * 8));
+    uint64_t *map = c->key2.as<uint64_t>();
+    k_copy_map<<<blocks_for(n), kT, 0, c->stream>>>(
+        c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+        c->f[PF_Z].as<double>(), off, n, b, hl, map);
+    k_copy_fill<<<blocks_for(extra), kT, 0, c->stream>>>(
+        a, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+        c->f[PF_Z].as<double>(), c->f[PF_ID].as<int64_t>(), map, extra,
+        c->n_real, b, hl, off, c->batch.nreal.as<int64_t>(),
+        c->batch.segr.as<int64_t>());
+    FSPH_HIP(c, hipGetLastError());
   }
-  k_gen_copies<<<blocks_for(n), kT, 0, c->stream>>>(
-      a, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), c->f[PF_ID].as<int64_t>(), off, n, c->n_real,
-      b, hl, c->batch.nreal.as<int64_t>(), c->batch.segr.as<int64_t>());
-  FSPH_HIP(c, hipGetLastError());
   c->n = n + extra;
   return 0;
 }
