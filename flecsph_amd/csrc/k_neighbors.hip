@@ -90,11 +90,12 @@ struct TravArgs {
   const int32_t *inst, *root;
 };
 
-template <int kStack>
+// MAXG: the largest group (64 when group_max <= 64, else kMaxGroup), which
+// sizes the group's LDS arrays.
+template <int kStack, int MAXG>
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
 #pragma clang fp contract(off)
-  __shared__ double sx[kMaxGroup], sy[kMaxGroup], sz[kMaxGroup],
-      sh[kMaxGroup];
+  __shared__ double sx[MAXG], sy[MAXG], sz[MAXG], sh[MAXG];
   __shared__ int32_t stk[kStack];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
@@ -221,7 +222,7 @@ struct MaskArgs {
   const double *x, *y, *z, *h;
   const double *com, *rad, *lap;
   const int32_t *gstart, *gcount, *gnode;
-  const int32_t *cand, *ccount;
+  int32_t *cand, *ccount;  // compacted in place (SPH form)
   const int64_t *moff;
   int64_t ccap;
   uint32_t *mask;
@@ -233,6 +234,39 @@ struct MaskArgs {
   const int32_t *glist;  // tile slot -> group (or null)
 };
 
+// Bit compress (the "compress" of Hacker's Delight, 2nd ed., section 7-4):
+// moves the bits of x selected by m to the low end in order.  The five
+// shift masks depend on m only, so they are computed once per mask word and
+// applied to every row's word.
+struct Compress {
+  uint32_t mv[5];
+};
+__device__ inline Compress compress_masks(uint32_t m) {
+  Compress c;
+  uint32_t mk = ~m << 1;
+  for (int i = 0; i < 5; ++i) {
+    uint32_t mp = mk ^ (mk << 1);
+    mp ^= mp << 2;
+    mp ^= mp << 4;
+    mp ^= mp << 8;
+    mp ^= mp << 16;
+    const uint32_t mv = mp & m;
+    c.mv[i] = mv;
+    m = (m ^ mv) | (mv >> (1 << i));
+    mk &= ~mp;
+  }
+  return c;
+}
+__device__ inline uint32_t compress(uint32_t x, uint32_t m, const Compress &c) {
+  x &= m;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const uint32_t t = x & c.mv[i];
+    x = (x ^ t) | (t >> (1 << i));
+  }
+  return x;
+}
+
 // GATHER selects the gather-mode predicate at compile time (the SPH form
 // keeps its inner loop free of the other mode's code).  The wavefront
 // stages kR candidates per round and deals the round's (particle, mask
@@ -241,14 +275,21 @@ struct MaskArgs {
 // (or lies past the tile's end) is staged with a NaN x, which fails every
 // distance test.  max(h_k, h_b)^2 is taken as max(h_k^2, h_b^2): squaring
 // and rounding are monotone, so the bound is the same number.
-template <bool GATHER>
+//
+// The SPH form then drops the candidates that are no neighbour of any of
+// the group's particles (about a third on Sedov 64^3): each round's column
+// of words is ORed over the rows, the used candidates are written back in
+// order to the front of the tile, and every row's bits are compressed to
+// the same used positions, so the pair passes stage and walk only used
+// candidates in an unchanged order.  ccount becomes the used count.
+template <bool GATHER, int MAXG>
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
 #pragma clang fp contract(off)
   constexpr int kR = 128;  // candidates per round (4 mask words)
   __shared__ double cx[kR], cy[kR], cz[kR], ch2[kR];
-  __shared__ double px[kMaxGroup], py[kMaxGroup], pz[kMaxGroup],
-      ph2[kMaxGroup];
-  __shared__ int cnt[kMaxGroup];
+  __shared__ double px[MAXG], py[MAXG], pz[MAXG], ph2[MAXG];
+  __shared__ int cnt[MAXG];
+  __shared__ uint32_t rw[GATHER ? 1 : kR / 32][GATHER ? 1 : MAXG];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
@@ -276,12 +317,20 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     gext_lap = a.lap[gn];
     grad = a.rad[gn];
   }
+  // compaction state (SPH form): used candidates so far, and per row (k =
+  // lane, lane + 64) the bits not yet written, nb of them, wi words written
+  int64_t used_so_far = 0;
+  uint64_t pend[MAXG / kW] = {};
+  int nb = 0, wi = 0;
   for (int r0 = 0; r0 < cc; r0 += kR) {
+    int32_t cidx[kR / kW];
     for (int i = lane; i < kR; i += kW) {
       const int j = r0 + i;
       double bx = __builtin_nan(""), by = 0, bz = 0, bh = 0;
+      cidx[i / kW] = -1;
       if (j < cc) {
         const int64_t b = a.cand[cbase + j];
+        cidx[i / kW] = (int32_t)b;
         by = a.y[b];
         bz = a.z[b];
         bh = a.h[b];
@@ -317,10 +366,79 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
         }
         if (in) word |= 1u << jj;
       }
-      a.mask[mo + (int64_t)((r0 >> 5) + w) * gc + k] = word;
+      if (GATHER)
+        a.mask[mo + (int64_t)((r0 >> 5) + w) * gc + k] = word;
+      else
+        rw[w][k] = word;
       if (a.nngb) atomicAdd(&cnt[k], __popc(word));
     }
     __syncthreads();
+    if (!GATHER) {
+      // candidates of the round that neighbour some row
+      uint32_t used[kR / 32];
+      int before[kR / 32], total = 0;
+#pragma unroll
+      for (int w = 0; w < kR / 32; ++w) {
+        uint32_t v = 0;
+        if (w < nwr)
+          for (int k = lane; k < gc; k += kW) v |= rw[w][k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+        used[w] = __builtin_amdgcn_readfirstlane(v);
+        before[w] = total;
+        total += __popc(used[w]);
+      }
+      Compress cm[kR / 32];
+#pragma unroll
+      for (int w = 0; w < kR / 32; ++w) cm[w] = compress_masks(used[w]);
+#pragma unroll
+      for (int q = 0; q < kR / kW; ++q) {
+        const int i = lane + q * kW, w = i >> 5, bit = i & 31;
+        if (used[w] >> bit & 1u)
+          a.cand[cbase + used_so_far + before[w] +
+                 __popc(used[w] & ((1u << bit) - 1u))] = cidx[q];
+      }
+      // each row's bits at the used positions, appended after the bits of
+      // the earlier rounds (the same positions for every row)
+#pragma unroll
+      for (int r = 0; r < MAXG / kW; ++r) {
+        const int k = lane + r * kW;
+        if (k >= gc) continue;
+        uint64_t acc = pend[r];
+        int nbl = nb, wil = wi;
+#pragma unroll
+        for (int w = 0; w < kR / 32; ++w) {
+          if (w >= nwr) break;
+          acc |= (uint64_t)compress(rw[w][k], used[w], cm[w]) << nbl;
+          nbl += __popc(used[w]);
+          if (nbl >= 32) {
+            a.mask[mo + (int64_t)wil * gc + k] = (uint32_t)acc;
+            acc >>= 32;
+            nbl -= 32;
+            ++wil;
+          }
+        }
+        pend[r] = acc;
+      }
+      for (int w = 0; w < nwr; ++w) {
+        nb += __popc(used[w]);
+        if (nb >= 32) {
+          nb -= 32;
+          ++wi;
+        }
+      }
+      used_so_far += total;
+    }
+    __syncthreads();
+  }
+  if (!GATHER) {
+    if (nb > 0)
+#pragma unroll
+      for (int r = 0; r < MAXG / kW; ++r) {
+        const int k = lane + r * kW;
+        if (k < gc) a.mask[mo + (int64_t)wi * gc + k] = (uint32_t)pend[r];
+      }
+    if (lane == 0) a.ccount[gi] = (int32_t)used_so_far;
   }
   if (a.nngb)
     for (int k = lane; k < gc; k += kW) a.nngb[gs + k] = cnt[k];
@@ -354,10 +472,17 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                c->batch.n_inst ? c->batch.root.as<int32_t>() : nullptr};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
-      if (c->big_stack)
-        k_traverse<kStackLarge><<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      const bool g64 = c->p.group_max <= kW;
+      if (c->big_stack && g64)
+        k_traverse<kStackLarge, kW><<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      else if (c->big_stack)
+        k_traverse<kStackLarge, kMaxGroup>
+            <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      else if (g64)
+        k_traverse<kStackSmall, kW><<<(unsigned)ng, kW, 0, c->stream>>>(t);
       else
-        k_traverse<kStackSmall><<<(unsigned)ng, kW, 0, c->stream>>>(t);
+        k_traverse<kStackSmall, kMaxGroup>
+            <<<(unsigned)ng, kW, 0, c->stream>>>(t);
     }
     FSPH_HIP(c, hipGetLastError());
     // mask offsets (an overflowed traversal leaves zero candidates), then
@@ -402,10 +527,15 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
              hprop,                   glist};
   {
     PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
-    if (gather)
-      k_masks<true><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    const bool g64 = c->p.group_max <= kW;
+    if (gather && g64)
+      k_masks<true, kW><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    else if (gather)
+      k_masks<true, kMaxGroup><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    else if (g64)
+      k_masks<false, kW><<<(unsigned)ng, kW, 0, c->stream>>>(m);
     else
-      k_masks<false><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+      k_masks<false, kMaxGroup><<<(unsigned)ng, kW, 0, c->stream>>>(m);
   }
   return hip_check(c, hipGetLastError(), "k_masks");
 }
