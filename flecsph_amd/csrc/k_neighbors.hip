@@ -322,19 +322,39 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
   int64_t used_so_far = 0;
   uint64_t pend[MAXG / kW] = {};
   int nb = 0, wi = 0;
+  // The next round's candidates are loaded into registers while the
+  // current round's masks are compacted, and stored to LDS after.
+  int32_t cidx[kR / kW];
+  double rx[kR / kW], ry[kR / kW], rz[kR / kW], rh[kR / kW];
+  auto prefetch = [&](int r0) {
+#pragma unroll
+    for (int q = 0; q < kR / kW; ++q) {
+      const int j = r0 + lane + q * kW;
+      cidx[q] = j < cc ? a.cand[cbase + j] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kR / kW; ++q) {
+      const int32_t b = cidx[q];
+      if (b < 0) continue;
+      rx[q] = a.x[b];
+      ry[q] = a.y[b];
+      rz[q] = a.z[b];
+      rh[q] = a.h[b];
+    }
+  };
+  if (cc > 0) prefetch(0);
   for (int r0 = 0; r0 < cc; r0 += kR) {
-    int32_t cidx[kR / kW];
-    for (int i = lane; i < kR; i += kW) {
-      const int j = r0 + i;
+    int32_t ccur[kR / kW];
+#pragma unroll
+    for (int q = 0; q < kR / kW; ++q) {
+      const int i = lane + q * kW;
+      ccur[q] = cidx[q];
       double bx = __builtin_nan(""), by = 0, bz = 0, bh = 0;
-      cidx[i / kW] = -1;
-      if (j < cc) {
-        const int64_t b = a.cand[cbase + j];
-        cidx[i / kW] = (int32_t)b;
-        by = a.y[b];
-        bz = a.z[b];
-        bh = a.h[b];
-        bx = a.x[b];
+      if (cidx[q] >= 0) {
+        bx = rx[q];
+        by = ry[q];
+        bz = rz[q];
+        bh = rh[q];
         if (gn >= 0) {
           // COM filter, tree_topology.h:365-371
           const double ext = ((bh < gext_lap) ? gext_lap : bh) + grad;
@@ -373,6 +393,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       if (a.nngb) atomicAdd(&cnt[k], __popc(word));
     }
     __syncthreads();
+    if (r0 + kR < cc) prefetch(r0 + kR);
     if (!GATHER) {
       // candidates of the round that neighbour some row
       uint32_t used[kR / 32];
@@ -396,7 +417,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
         const int i = lane + q * kW, w = i >> 5, bit = i & 31;
         if (used[w] >> bit & 1u)
           a.cand[cbase + used_so_far + before[w] +
-                 __popc(used[w] & ((1u << bit) - 1u))] = cidx[q];
+                 __popc(used[w] & ((1u << bit) - 1u))] = ccur[q];
       }
       // each row's bits at the used positions, appended after the bits of
       // the earlier rounds (the same positions for every row)
