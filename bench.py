@@ -74,8 +74,8 @@ VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
 # Measured with the tuning below; other tunings report no traffic.
 SEDOV64_TRAFFIC_TUNING = (64, 64)
 SEDOV64_TRAFFIC_BYTES = {
-    "masks": 192_857_490, "traverse": 159_344_814, "density": 299_823_941,
-    "accel": 764_196_713, "dudt": 713_376_773,
+    "masks": 218_964_690, "traverse": 160_346_093, "density": 250_534_949,
+    "accel": 603_936_237, "dudt": 558_935_957,
 }
 
 
