@@ -222,7 +222,7 @@ struct MaskArgs {
   const double *x, *y, *z, *h;
   const double *com, *rad, *lap;
   const int32_t *gstart, *gcount, *gnode;
-  int32_t *cand, *ccount;  // compacted in place (SPH form)
+  int32_t *cand, *ccount;  // compacted in place
   const int64_t *moff;
   int64_t ccap;
   uint32_t *mask;
@@ -276,8 +276,8 @@ __device__ inline uint32_t compress(uint32_t x, uint32_t m, const Compress &c) {
 // distance test.  max(h_k, h_b)^2 is taken as max(h_k^2, h_b^2): squaring
 // and rounding are monotone, so the bound is the same number.
 //
-// The SPH form then drops the candidates that are no neighbour of any of
-// the group's particles (about a third on Sedov 64^3): each round's column
+// Then the candidates that are no neighbour of any of the group's
+// particles are dropped (about a third on Sedov 64^3): each round's column
 // of words is ORed over the rows, the used candidates are written back in
 // order to the front of the tile, and every row's bits are compressed to
 // the same used positions, so the pair passes stage and walk only used
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
   __shared__ double cx[kR], cy[kR], cz[kR], ch2[kR];
   __shared__ double px[MAXG], py[MAXG], pz[MAXG], ph2[MAXG];
   __shared__ int cnt[MAXG];
-  __shared__ uint32_t rw[GATHER ? 1 : kR / 32][GATHER ? 1 : MAXG];
+  __shared__ uint32_t rw[kR / 32][MAXG];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     gext_lap = a.lap[gn];
     grad = a.rad[gn];
   }
-  // compaction state (SPH form): used candidates so far, and per row (k =
+  // compaction state: used candidates so far, and per row (k =
   // lane, lane + 64) the bits not yet written, nb of them, wi words written
   int64_t used_so_far = 0;
   uint64_t pend[MAXG / kW] = {};
@@ -386,15 +386,12 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
         }
         if (in) word |= 1u << jj;
       }
-      if (GATHER)
-        a.mask[mo + (int64_t)((r0 >> 5) + w) * gc + k] = word;
-      else
-        rw[w][k] = word;
+      rw[w][k] = word;
       if (a.nngb) atomicAdd(&cnt[k], __popc(word));
     }
     __syncthreads();
     if (r0 + kR < cc) prefetch(r0 + kR);
-    if (!GATHER) {
+    {
       // candidates of the round that neighbour some row
       uint32_t used[kR / 32];
       int before[kR / 32], total = 0;
@@ -452,7 +449,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     }
     __syncthreads();
   }
-  if (!GATHER) {
+  {
     if (nb > 0)
 #pragma unroll
       for (int r = 0; r < MAXG / kW; ++r) {
