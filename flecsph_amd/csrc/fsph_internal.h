@@ -120,6 +120,8 @@ struct Ctx {
 
   // groups and neighbour tiles
   int64_t ng = 0;
+  int groups_max = 128;      // group_max the current groups were built with
+                             // (sizes the kernels that take a group's rows)
   DBuf gstart, gcount, gnode, gflag, gscan;
   int64_t ccap = 2048;       // candidate capacity per group
   bool big_stack = false;    // traversal with the large LDS stack

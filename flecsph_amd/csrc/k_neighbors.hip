@@ -490,7 +490,7 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                c->batch.n_inst ? c->batch.root.as<int32_t>() : nullptr};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
-      const bool g64 = c->p.group_max <= kW;
+      const bool g64 = c->groups_max <= kW;
       if (c->big_stack && g64)
         k_traverse<kStackLarge, kW><<<(unsigned)ng, kW, 0, c->stream>>>(t);
       else if (c->big_stack)
@@ -545,7 +545,7 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
              hprop,                   glist};
   {
     PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
-    const bool g64 = c->p.group_max <= kW;
+    const bool g64 = c->groups_max <= kW;
     if (gather && g64)
       k_masks<true, kW><<<(unsigned)ng, kW, 0, c->stream>>>(m);
     else if (gather)

@@ -636,7 +636,9 @@ __global__ __launch_bounds__(NT) void k_force_p(ForceArgs<T, DUDT> a,
 // chunk and workgroup width: one wavefront when no group exceeds 64
 // particles, two otherwise.
 template <int V> using IC = std::integral_constant<int, V>;
-inline bool narrow(const Ctx *c) { return c->p.group_max <= 64; }
+// (by the group limit the groups were built with: a group_max changed since
+// applies from the next update_iteration)
+inline bool narrow(const Ctx *c) { return c->groups_max <= 64; }
 template <class F> void dispatch(const Ctx *c, F &&f) {
   const bool nr = narrow(c);
   switch (c->chunk) {

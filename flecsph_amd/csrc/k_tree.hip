@@ -503,6 +503,7 @@ int stage_tree(Ctx *c) {
 int group_flags(Ctx *c) {
   const int64_t n = c->n, nn = c->nn;
   const int64_t tot = nn + n;
+  c->groups_max = c->p.group_max;
   FSPH_TRY(ensure(c, c->gflag, (size_t)tot * 4));
   FSPH_TRY(ensure(c, c->gscan, (size_t)(tot + 1) * 8));
   k_group_flags<<<blocks_for(tot), kT, 0, c->stream>>>(

@@ -145,6 +145,31 @@ def test_set_tuning_keeps_neighbour_sets():
         assert np.array_equal(idx, lists[0][1])
 
 
+def test_group_max_change_applies_from_the_next_update():
+    """A group_max lowered between update_iteration and the pair passes
+    takes effect at the next update_iteration: the passes keep sizing their
+    work for the groups they were given (up to 128 particles here), and give
+    the results of a context that never changed it."""
+    from flecsph_amd import set_tuning
+    parts, prm = generators.random_box(6000, seed=101, h_spread=1.5)
+    out = []
+    for late in (False, True):
+        s = BodySystem(_abi.default_params(**prm))
+        s.upload(parts)
+        set_tuning(s, 128, 128)
+        s.update_iteration()
+        assert s.tree()["group_count"].max() > 64
+        if late:
+            set_tuning(s, 64, 128)
+        s.density_eos()
+        s.integrate(_abi.SAVE_VHALF)
+        s.acceleration()
+        out.append({f: s.get(f) for f in ("rho", "P", "ax", "ay", "az")})
+        s.close()
+    for f in out[0]:
+        assert np.array_equal(out[0][f], out[1][f]), f
+
+
 @pytest.mark.parametrize("tuning", [(128, 128), (64, 64)])
 @pytest.mark.parametrize("precision", [_abi.PREC_FP32, _abi.PREC_FP64])
 def test_graph_replay_matches_eager_every_step(precision, tuning):
