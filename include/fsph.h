@@ -377,7 +377,9 @@ int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
 
 /* Sets the two tuning choices the autotuner searches (new): group_max
    (1..128, the reference's sub_entities_ limit of a leaf group) and chunk
-   (64, 128 or 256).  Takes effect at the next fsph_update_iteration. */
+   (64, 128 or 256).  group_max takes effect at the next
+   fsph_update_iteration (the current groups keep the limit they were built
+   with); chunk at the next pair pass. */
 int fsph_set_tuning(fsph_ctx *ctx, int32_t group_max, int32_t chunk);
 
 #ifdef __cplusplus
