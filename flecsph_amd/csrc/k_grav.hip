@@ -587,6 +587,15 @@ template <class T> struct KS {
   }
   __device__ inline T get() const { return s - c; }
 };
+// FP32 terms go into an FP64 accumulator: without packed FP32 an FP64 add
+// issues at the FP32 rate, so a conversion and an add replace the four
+// operations of the compensated FP32 sum, with a smaller error.
+template <> struct KS<float> {
+  double s = 0;
+  static constexpr float c = 0.f;
+  __device__ inline void add(float v) { s += (double)v; }
+  __device__ inline float get() const { return (float)s; }
+};
 
 template <class T> struct EvalArgs {
   Cells c;
@@ -700,10 +709,10 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
       }
     __syncthreads();
   }
-  part[0][t] = f0.s - f0.c;
-  part[1][t] = f1.s - f1.c;
-  part[2][t] = f2.s - f2.c;
-  part[3][t] = pp.s - pp.c;
+  part[0][t] = f0.get();
+  part[1][t] = f1.get();
+  part[2][t] = f2.get();
+  part[3][t] = pp.get();
   __syncthreads();
   T g0 = 0, g1 = 0, g2 = 0, ph = 0;
   if (act && r == 0)
