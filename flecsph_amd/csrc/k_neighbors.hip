@@ -15,6 +15,8 @@
 // The SPH passes stage candidates in LDS and walk these bits.
 #include <cfloat>
 
+#include <type_traits>
+
 #include "fsph_internal.h"
 
 namespace fsph {
@@ -92,7 +94,7 @@ struct TravArgs {
 
 // MAXG: the largest group (64 when group_max <= 64, else kMaxGroup), which
 // sizes the group's LDS arrays.
-template <int kStack, int MAXG>
+template <int kStack, int MAXG, bool GF>
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
 #pragma clang fp contract(off)
   __shared__ double sx[MAXG], sy[MAXG], sz[MAXG], sh[MAXG];
@@ -135,6 +137,24 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
         gb[d + 3] = fmax(gb[d + 3], __shfl_xor(gb[d + 3], o, 64));
       }
   }
+  // Leaf filter (gather form): a leaf child of an accepted node becomes a
+  // candidate only if it lies within the sphere about the hull's centre
+  // that holds every particle's gather sphere (slightly widened), outside
+  // of which it fails every particle's gather predicate (4M collapse:
+  // Newton-h phase 20.2 -> 18.0 ms).  The SPH form keeps every leaf: the
+  // same test with the centre-of-mass filter of tree_topology.h:365-371
+  // (which the masks apply) cost the traversal more than it saved.
+  constexpr bool filt = GF;  // (GF: gather form, a.gather set)
+  double fc[3] = {0, 0, 0}, frad = 0;
+  if (filt) {
+    for (int d = 0; d < 3; ++d) fc[d] = 0.5 * (gb[d] + gb[d + 3]);
+    for (int k = lane; k < gc; k += kW) {
+      const double e0 = sx[k] - fc[0], e1 = sy[k] - fc[1], e2 = sz[k] - fc[2];
+      frad = fmax(frad, sqrt(e0 * e0 + e1 * e1 + e2 * e2) + sh[k]);
+    }
+    for (int o = 32; o > 0; o >>= 1) frad = fmax(frad, __shfl_xor(frad, o, 64));
+    frad *= 1.0 + 1e-9;
+  }
   if (lane == 0) stk[0] = a.root ? a.root[a.inst[gs]] : 0;  // root
   __syncthreads();
   int sp = 1;
@@ -162,11 +182,24 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
         for (int j = 0; j < 8; ++j) ch[j] = a.child[8 * (int64_t)node + j];
     }
     int nnode = 0, nleaf = 0;
-    if (acc)
+    uint32_t keep = 0;  // leaf children that pass the leaf filter
+    if (acc) {
+#pragma unroll
       for (int j = 0; j < 8; ++j) {
         nnode += ch[j] >= 0;
-        nleaf += ch[j] <= -2;
+        if (ch[j] <= -2) {
+          if (!filt) {
+            keep |= 1u << j;
+            continue;
+          }
+          const int64_t p = leaf_of(ch[j]);
+          if (within_distance2(a.x[p], a.y[p], a.z[p], fc[0], fc[1], fc[2],
+                               frad))
+            keep |= 1u << j;
+        }
       }
+      nleaf = __popc(keep);
+    }
     int tn, tl;
     const int pn = wave_excl_scan(nnode, &tn);
     const int pl = wave_excl_scan(nleaf, &tl);
@@ -186,7 +219,7 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
       for (int j = 0; j < 8; ++j) {
         if (ch[j] >= 0)
           stk[qn++] = ch[j];
-        else if (ch[j] <= -2)
+        else if (keep >> j & 1u)
           a.cand[ql++] = (int32_t)leaf_of(ch[j]);
       }
     }
@@ -491,16 +524,25 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
       const bool g64 = c->groups_max <= kW;
-      if (c->big_stack && g64)
-        k_traverse<kStackLarge, kW><<<(unsigned)ng, kW, 0, c->stream>>>(t);
-      else if (c->big_stack)
-        k_traverse<kStackLarge, kMaxGroup>
-            <<<(unsigned)ng, kW, 0, c->stream>>>(t);
-      else if (g64)
-        k_traverse<kStackSmall, kW><<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      auto go = [&](auto gf) {
+        constexpr bool GF = decltype(gf)::value;
+        if (c->big_stack && g64)
+          k_traverse<kStackLarge, kW, GF>
+              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+        else if (c->big_stack)
+          k_traverse<kStackLarge, kMaxGroup, GF>
+              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+        else if (g64)
+          k_traverse<kStackSmall, kW, GF>
+              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+        else
+          k_traverse<kStackSmall, kMaxGroup, GF>
+              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+      };
+      if (gather)
+        go(std::true_type{});
       else
-        k_traverse<kStackSmall, kMaxGroup>
-            <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+        go(std::false_type{});
     }
     FSPH_HIP(c, hipGetLastError());
     // mask offsets (an overflowed traversal leaves zero candidates), then
