@@ -138,10 +138,10 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
       }
   }
   // Leaf filter (gather form): a leaf child of an accepted node becomes a
-  // candidate only if it lies within the sphere about the hull's centre
-  // that holds every particle's gather sphere (slightly widened), outside
-  // of which it fails every particle's gather predicate (4M collapse:
-  // Newton-h phase 20.2 -> 18.0 ms).  The SPH form keeps every leaf: the
+  // candidate only if it lies within the hull box of the particles' gather
+  // spheres and within the sphere about the box's centre that holds them
+  // all (both slightly widened), outside of which it fails every particle's
+  // gather predicate (4M collapse: Newton-h phase 20.2 -> 18.0 ms).  The SPH form keeps every leaf: the
   // same test with the centre-of-mass filter of tree_topology.h:365-371
   // (which the masks apply) cost the traversal more than it saved.
   constexpr bool filt = GF;  // (GF: gather form, a.gather set)
@@ -193,8 +193,11 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
             continue;
           }
           const int64_t p = leaf_of(ch[j]);
-          if (within_distance2(a.x[p], a.y[p], a.z[p], fc[0], fc[1], fc[2],
-                               frad))
+          const double bx = a.x[p], by = a.y[p], bz = a.z[p];
+          // inside the hull box of the gather spheres and the sphere
+          if (bx >= gb[0] && bx <= gb[3] && by >= gb[1] && by <= gb[4] &&
+              bz >= gb[2] && bz <= gb[5] &&
+              within_distance2(bx, by, bz, fc[0], fc[1], fc[2], frad))
             keep |= 1u << j;
         }
       }
