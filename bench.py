@@ -111,11 +111,11 @@ def parse():
     ap.add_argument("--lh-learn", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep group_max 128 and chunk 128 instead of "
+                    help="keep tile_rows 64 and chunk 128 instead of "
                          "autotuning them before the warm-up")
     ap.add_argument("--tune-steps", type=int, default=3)
     ap.add_argument("--tuning", default="",
-                    help="GROUP_MAX,CHUNK: use these instead of autotuning "
+                    help="TILE_ROWS,CHUNK: use these instead of autotuning "
                          "(profiling runs use the bench line's choice)")
     ap.add_argument("--batch-inst", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -200,7 +200,7 @@ def batched_run(n_inst, steps):
     b.step()
     b.enable_graph(True)  # the tuner times the replayed steps it picks for
     gm, ch = autotune(b, steps=2)
-    log(f"batched: autotune chose group_max {gm}, chunk {ch}")
+    log(f"batched: autotune chose tile_rows {gm}, chunk {ch}")
     b.step()
     lat = []
     for _ in range(steps):
@@ -320,21 +320,22 @@ def main():
     parts, prm, label, step_bytes = workload(args.workload)
     g = BodySystem(_abi.default_params(**prm), device=0)
     g.upload(parts)
-    tuning = {"group_max": 128, "chunk": 128}
+    tuning = {"group_max": prm.get("group_max", 128), "tile_rows": 64,
+              "chunk": 128}
     if args.tuning:
         from flecsph_amd import set_tuning
         gm, ch = (int(v) for v in args.tuning.split(","))
         set_tuning(g, gm, ch)
-        tuning = {"group_max": gm, "chunk": ch, "given": True}
+        tuning.update(tile_rows=gm, chunk=ch, given=True)
     elif not args.no_autotune:
-        # untimed: each (group_max, chunk) candidate is timed over
+        # untimed: each (tile_rows, chunk) candidate is timed over
         # tune_steps steps from the state after the initial iteration, which
         # the autotuner restores afterwards (the warm-up then continues from
         # there)
         gm, ch = autotune(g, steps=args.tune_steps)
-        tuning = {"group_max": gm, "chunk": ch,
-                  "tune_steps_per_candidate": args.tune_steps}
-        log(f"autotune chose group_max {gm}, chunk {ch}")
+        tuning.update(tile_rows=gm, chunk=ch,
+                      tune_steps_per_candidate=args.tune_steps)
+        log(f"autotune chose tile_rows {gm}, chunk {ch}")
     for _ in range(args.warmup):
         g.step()
     n_tot, n_real = g.count()

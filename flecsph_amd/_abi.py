@@ -92,7 +92,7 @@ def default_params(**kw):
     p.n_target = 100
     p.gamma = 1.4
     p.poly_K = 1.0
-    p.eta = 1.0
+    p.eta = 1.5
     p.visc_alpha = 1.0
     p.visc_beta = 2.0
     p.visc_eps = 0.01

@@ -61,7 +61,7 @@ int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
 }
 
 size_t field_size(const Ctx *c, int f) {
-  if (f < PF_NDBL || f == PF_ID) return 8;
+  if (f < PF_NDBL || f == PF_ID || f == PF_KAD) return 8;
   if (f == PF_NNGB || f == PF_HNGB || f == PF_HITER || f == PF_INST) return 4;
   return c->rsize;
 }
@@ -191,6 +191,7 @@ int do_step(Ctx *c) {
   if (!c->initial_done) {
     // hydro/main_driver.cc:102-173 (iteration 0: no integration)
     FSPH_TRY(do_update_iteration(c));
+    FSPH_TRY(stage_eos_init(c));  // hydro/main_driver.cc:106
     FSPH_TRY(stage_density_only(c));
     {
       PhaseTimer t(c, FSPH_T_INTEGRATE);
@@ -272,7 +273,7 @@ void fsph_default_params(fsph_params *p) {
   p->n_target = 100;
   p->gamma = 1.4;
   p->poly_K = 1.0;
-  p->eta = 1.0;
+  p->eta = 1.5;
   p->visc_alpha = 1.0;
   p->visc_beta = 2.0;
   p->visc_eps = 0.01;
@@ -337,6 +338,8 @@ void fsph_destroy(fsph_ctx *ctx) {
                        &c->nchild, &c->nbox, &c->ncom, &c->nmass, &c->nrad,
                        &c->nlap, &c->nlevel, &c->leaf_parent, &c->gstart,
                        &c->gcount, &c->gnode, &c->gflag, &c->gscan, &c->cand,
+                       &c->gtile, &c->tstart, &c->tcount, &c->tslot,
+                       &c->tccount, &c->tmoff,
                        &c->ccount, &c->moff, &c->mask, &c->gcand,
                        &c->gccount, &c->gmoff, &c->gmask, &c->h_prop,
                        &c->h_prev, &c->hflag, &c->glist, &c->hgflag,
@@ -441,8 +444,10 @@ int fsph::upload_particles(Ctx *c, const fsph_particles *in) {
     FSPH_HIP(c, hipMemsetAsync(c->f[f].p, 0, (size_t)n * fsph::field_size(c, f),
                                c->stream));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  c->have_rhoP = in->rho && in->P;
   c->n = c->n_real = n;
   c->have_tree = false;
+  FSPH_TRY(fsph::stage_eos_init(c));  // K := poly_K until eos::init runs
   c->iteration = 0;
   c->dt = c->p.initial_dt;
   c->totaltime = 0;
@@ -515,6 +520,8 @@ int fsph_update_iteration(fsph_ctx *ctx) {
 
 int fsph_eos_init(fsph_ctx *ctx) {
   CTX_OR_FAIL;
+  FSPH_TRY(fsph::stage_eos_init(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
   return FSPH_OK;
 }
 
@@ -682,18 +689,21 @@ int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
   CTX_OR_FAIL;
   FSPH_TRY(fsph::need_tree(c));
   if (!off) return fsph::fail(c, FSPH_E_ARG, "off is required");
-  const int64_t n = c->n, ng = c->ng;
-  std::vector<int32_t> gs(ng), gc(ng), cc(ng);
-  std::vector<int64_t> mo(ng + 1);
+  // read back through the tiles the pair passes walk
+  const int64_t n = c->n, nt = c->nt;
+  std::vector<int32_t> ts(nt), tc(nt), sl(nt), cc(nt);
+  std::vector<int64_t> mo(nt);
   std::vector<uint32_t> mask((size_t)c->mask_words);
   std::vector<int32_t> nngb(n);
-  FSPH_HIP(c, hipMemcpyAsync(gs.data(), c->gstart.p, ng * 4,
+  FSPH_HIP(c, hipMemcpyAsync(ts.data(), c->tstart.p, nt * 4,
                              hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipMemcpyAsync(gc.data(), c->gcount.p, ng * 4,
+  FSPH_HIP(c, hipMemcpyAsync(tc.data(), c->tcount.p, nt * 4,
                              hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipMemcpyAsync(cc.data(), c->ccount.p, ng * 4,
+  FSPH_HIP(c, hipMemcpyAsync(sl.data(), c->tslot.p, nt * 4,
                              hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipMemcpyAsync(mo.data(), c->moff.p, (ng + 1) * 8,
+  FSPH_HIP(c, hipMemcpyAsync(cc.data(), c->tccount.p, nt * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipMemcpyAsync(mo.data(), c->tmoff.p, nt * 8,
                              hipMemcpyDeviceToHost, c->stream));
   FSPH_HIP(c, hipMemcpyAsync(nngb.data(), c->f[fsph::PF_NNGB].p, n * 4,
                              hipMemcpyDeviceToHost, c->stream));
@@ -706,18 +716,18 @@ int fsph_neighbor_lists(fsph_ctx *ctx, int64_t *off, int64_t *ids,
   if (!ids) return FSPH_OK;
   if (cap < off[n]) return fsph::fail(c, FSPH_E_CAP, "ids buffer too small");
   std::vector<int32_t> cand;
-  for (int64_t g = 0; g < ng; ++g) {
-    cand.resize(cc[g]);
-    if (cc[g])
+  for (int64_t t = 0; t < nt; ++t) {
+    cand.resize(cc[t]);
+    if (cc[t])
       FSPH_HIP(c, hipMemcpy(cand.data(),
-                            c->cand.as<int32_t>() + g * c->ccap,
-                            (size_t)cc[g] * 4, hipMemcpyDeviceToHost));
-    const int nw = (cc[g] + 31) / 32;
-    for (int k = 0; k < gc[g]; ++k) {
-      const int64_t i = gs[g] + k;
+                            c->cand.as<int32_t>() + (int64_t)sl[t] * c->ccap,
+                            (size_t)cc[t] * 4, hipMemcpyDeviceToHost));
+    const int nw = (cc[t] + 31) / 32;
+    for (int k = 0; k < tc[t]; ++k) {
+      const int64_t i = ts[t] + k;
       std::vector<int64_t> lst;
       for (int w = 0; w < nw; ++w) {
-        uint32_t bits = mask[mo[g] + (int64_t)w * gc[g] + k];
+        uint32_t bits = mask[mo[t] + (int64_t)w * tc[t] + k];
         while (bits) {
           const int j = __builtin_ctz(bits);
           bits &= bits - 1;

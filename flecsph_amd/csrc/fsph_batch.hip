@@ -271,7 +271,7 @@ int fsph_instance_time(const fsph_ctx *ctx, int inst, int64_t *iteration,
   return FSPH_OK;
 }
 
-int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
+int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *tile_rows,
                   int32_t *chunk) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c) return FSPH_E_ARG;
@@ -279,15 +279,17 @@ int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
   if (steps < 1 || c->n < 2)
     return fsph::fail(c, FSPH_E_ARG, "autotune needs particles and steps >= 1");
   if (!c->initial_done) FSPH_TRY(fsph_step(ctx));  // the initial iteration
-  static const int kGroup[] = {32, 64, 128};
+  // group_max is the reference's grouping and stays as set: it decides the
+  // neighbour lists.  Only the tile width and the chunk are searched.
+  static const int kRows[] = {64, 128};
   static const int kChunks[] = {64, 128, 256};
   double best = 1e300;
-  int bg = c->p.group_max, bc = c->chunk;
+  int br = c->tile_rows, bc = c->chunk;
   Snapshot snap;
   int rc = snap_take(c, snap);
-  for (int gi = 0; gi < 3 && !rc; ++gi)
+  for (int ri = 0; ri < 2 && !rc; ++ri)
     for (int ci = 0; ci < 3 && !rc; ++ci) {
-      c->p.group_max = kGroup[gi];
+      c->tile_rows = kRows[ri];
       c->chunk = kChunks[ci];
       // one untimed step (buffers grow, graphs are captured), then the
       // timed steps from the same state
@@ -304,29 +306,28 @@ int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
                             .count();
       if (!rc && dt < best) {
         best = dt;
-        bg = kGroup[gi];
+        br = kRows[ri];
         bc = kChunks[ci];
       }
     }
-  c->p.group_max = bg;
+  c->tile_rows = br;
   c->chunk = bc;
   if (!rc) rc = snap_restore(c, snap);
   snap_free(snap);
   if (rc) return rc;
-  if (group_max) *group_max = bg;
+  if (tile_rows) *tile_rows = br;
   if (chunk) *chunk = bc;
   return FSPH_OK;
 }
 
-int fsph_set_tuning(fsph_ctx *ctx, int32_t group_max, int32_t chunk) {
+int fsph_set_tuning(fsph_ctx *ctx, int32_t tile_rows, int32_t chunk) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c) return FSPH_E_ARG;
-  if (group_max < 1 || group_max > 128 ||
+  if ((tile_rows != 64 && tile_rows != 128) ||
       (chunk != 64 && chunk != 128 && chunk != 256))
     return fsph::fail(c, FSPH_E_ARG,
-                      "group_max must lie in 1..128 and chunk be 64, 128 "
-                      "or 256");
-  c->p.group_max = group_max;
+                      "tile_rows must be 64 or 128 and chunk 64, 128 or 256");
+  c->tile_rows = tile_rows;
   c->chunk = chunk;
   return FSPH_OK;
 }

@@ -77,6 +77,8 @@ enum PField : int {
   PF_DIVV,              // real: velocity divergence (learned-h feature)
   PF_DLH,               // real: ln(h_new / h_old) of the last h update
   PF_INST,              // int32: instance of the particle (batched mode)
+  PF_KAD,               // double: adiabat K of the polytropic EOS
+                        // (body_u::adiabatic, set by eos::init)
   PF_COUNT
 };
 
@@ -87,6 +89,7 @@ struct Ctx {
   std::string err;
   size_t rsize = 4;  // sizeof(real) for derived fields
 
+  bool have_rhoP = false;  // the last upload gave rho and P
   int64_t n = 0;       // particles held (real + WALL copies)
   int64_t n_real = 0;
   int64_t pcap = 0;    // particle capacity of every per-particle buffer
@@ -118,15 +121,30 @@ struct Ctx {
   int64_t level_off[kMaxDepth + 2] = {0};
   int max_depth = 0;
 
-  // groups and neighbour tiles
+  // groups and neighbour tiles.  A group (tree_topology.h:222-235) decides
+  // which candidates its particles see; a tile is what one pair-kernel
+  // workgroup processes: up to tiles_w rows (particles) of one group, with
+  // the group's candidates that neighbour one of those rows.  A group of
+  // more than tiles_w particles is cut into two tiles.
   int64_t ng = 0;
   int groups_max = 128;      // group_max the current groups were built with
                              // (sizes the kernels that take a group's rows)
   DBuf gstart, gcount, gnode, gflag, gscan;
-  int64_t ccap = 2048;       // candidate capacity per group
+  DBuf gtile;                // first tile of each group (int32)
+  int64_t nt = 0, nsplit = 0;  // tiles; groups cut in two
+  int tile_rows = 64;        // tuning: tile width asked for (64 or 128)
+  int tiles_w = 128;         // tile width the current tiles were built with
+  DBuf tstart, tcount;       // per tile: first particle, rows (int32)
+  DBuf tslot;                // per tile: candidate slot (int32)
+  DBuf tccount, tmoff;       // per tile: candidates (int32), mask offset
+  int64_t ccap = 2048;       // candidate capacity per slot
   bool big_stack = false;    // traversal with the large LDS stack
   int chunk = 128;           // candidates per LDS round of the pair kernels
-  DBuf cand, ccount, moff;   // cand: ng*ccap int32; ccount int32; moff int64
+  // cand: (ng + nsplit) slots of ccap int32 (the traversal fills slot g of
+  // group g; the masks kernel compacts it in place for the group's first
+  // tile and writes the second tile's into slot ng + ...); ccount int32 per
+  // group; moff int64 per group
+  DBuf cand, ccount, moff;
   DBuf mask;                 // uint32 words
   int64_t mask_words = 0;
   bool have_tree = false;
@@ -306,6 +324,9 @@ int stage_gravity(Ctx *c);             // k_grav.hip
 void free_gravity(Ctx *c);             // k_grav.hip
 int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
 int stage_update_h(Ctx *c);            // k_step.hip
+// eos::init (eos.h:89-92): K = P / rho^gamma per particle, or the parameter
+// poly_K where rho and P were not uploaded (k_step.hip)
+int stage_eos_init(Ctx *c);
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
 // device pointer to 2.5 max h (the periodic halo), no host wait
 int stage_halo(Ctx *c, const double **halo);  // k_step.hip

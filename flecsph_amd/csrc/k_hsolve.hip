@@ -277,8 +277,10 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
 template <class T>
 NewtonArgs<T> newton_args(Ctx *c, double kappa, const int32_t *glist) {
   NewtonArgs<T> a{};
-  a.gstart = c->gstart.as<int32_t>();
-  a.gcount = c->gcount.as<int32_t>();
+  // the units of the gather tiles are the pair-kernel tiles (the gather
+  // predicate involves only each particle's own radius)
+  a.gstart = c->tstart.as<int32_t>();
+  a.gcount = c->tcount.as<int32_t>();
   a.cand = c->gcand.as<int32_t>();
   a.ccount = c->gccount.as<int32_t>();
   a.moff = c->gmoff.as<int64_t>();
@@ -320,7 +322,7 @@ template <class T> int newton_impl(Ctx *c, const float *h0) {
   FSPH_TRY(stage_gather_tiles(c, kappa));
   NewtonArgs<T> a = newton_args<T>(c, kappa, nullptr);
   a.h0 = h0;
-  return launch_newton(c, a, c->ng);
+  return launch_newton(c, a, c->nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -570,9 +572,9 @@ template <class T> int learned_impl(Ctx *c) {
     FSPH_TRY(ensure(c, M.Y, (size_t)n * 4));
     FSPH_TRY(ensure(c, c->h_prop, (size_t)n * 4));
     FSPH_TRY(ensure(c, c->hflag, (size_t)n * 4));
-    FSPH_TRY(ensure(c, c->hgflag, (size_t)c->ng * 4));
-    FSPH_TRY(ensure(c, c->hgscan, (size_t)(c->ng + 1) * 8));
-    FSPH_TRY(ensure(c, c->glist, (size_t)c->ng * 4));
+    FSPH_TRY(ensure(c, c->hgflag, (size_t)c->nt * 4));
+    FSPH_TRY(ensure(c, c->hgscan, (size_t)(c->nt + 1) * 8));
+    FSPH_TRY(ensure(c, c->glist, (size_t)c->nt * 4));
     FSPH_TRY(ensure(c, c->hcnt, 2 * 4));
     {
       PhaseTimer tm(c, FSPH_T_H);
@@ -609,7 +611,7 @@ template <class T> int learned_impl(Ctx *c) {
     a.flag = c->hflag.as<int32_t>();
     a.gflag = c->hgflag.as<int32_t>();
     a.cnt = c->hcnt.as<int32_t>();
-    FSPH_TRY(launch_newton(c, a, c->ng));
+    FSPH_TRY(launch_newton(c, a, c->nt));
     int32_t cnt[2] = {0, 0};
     FSPH_HIP(c, hipMemcpyAsync(cnt, c->hcnt.p, 8, hipMemcpyDeviceToHost,
                                c->stream));
@@ -618,12 +620,12 @@ template <class T> int learned_impl(Ctx *c) {
     c->h_clamped = cnt[1];
     if (cnt[0] > 0) {
       FSPH_TRY(scan_i32_to_i64(c, c->hgflag.as<int32_t>(),
-                               c->hgscan.as<int64_t>(), c->ng));
+                               c->hgscan.as<int64_t>(), c->nt));
       int64_t ngl = 0;
-      FSPH_HIP(c, hipMemcpyAsync(&ngl, c->hgscan.as<int64_t>() + c->ng, 8,
+      FSPH_HIP(c, hipMemcpyAsync(&ngl, c->hgscan.as<int64_t>() + c->nt, 8,
                                  hipMemcpyDeviceToHost, c->stream));
-      k_group_list<<<blocks256(c->ng), 256, 0, c->stream>>>(
-          c->hgflag.as<int32_t>(), c->hgscan.as<int64_t>(), c->ng,
+      k_group_list<<<blocks256(c->nt), 256, 0, c->stream>>>(
+          c->hgflag.as<int32_t>(), c->hgscan.as<int64_t>(), c->nt,
           c->glist.as<int32_t>());
       FSPH_HIP(c, hipGetLastError());
       k_restore<<<blocks256(n), 256, 0, c->stream>>>(

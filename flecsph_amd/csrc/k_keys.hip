@@ -529,6 +529,7 @@ int persistent_fields(const Ctx *c, int *out) {
   int nf = 0;
   for (int q = 0; q < kNPersistent; ++q) out[nf++] = kPersistent[q];
   if (c->batch.n_inst) out[nf++] = PF_INST;
+  if (c->p.eos == FSPH_EOS_POLYTROPIC) out[nf++] = PF_KAD;
   return nf;
 }
 

@@ -258,7 +258,7 @@ struct MaskArgs {
   const double *x, *y, *z, *h;
   const double *com, *rad, *lap;
   const int32_t *gstart, *gcount, *gnode;
-  int32_t *cand, *ccount;  // compacted in place
+  int32_t *cand, *ccount;  // traversal output per slot
   const int64_t *moff;
   int64_t ccap;
   uint32_t *mask;
@@ -268,6 +268,12 @@ struct MaskArgs {
   int gather;
   const float *hprop;    // gather radius base instead of h (or null)
   const int32_t *glist;  // tile slot -> group (or null)
+  // SPH tiles (null: the slot is its own tile, compacted in place into
+  // ccount/cand/mask at moff): first tile of each group, candidate slot of
+  // each tile, and the outputs per tile
+  const int32_t *gtile, *tslot;
+  int32_t *tcc;
+  int64_t *tmo;
 };
 
 // Bit compress (the "compress" of Hacker's Delight, 2nd ed., section 7-4):
@@ -312,20 +318,25 @@ __device__ inline uint32_t compress(uint32_t x, uint32_t m, const Compress &c) {
 // distance test.  max(h_k, h_b)^2 is taken as max(h_k^2, h_b^2): squaring
 // and rounding are monotone, so the bound is the same number.
 //
-// Then the candidates that are no neighbour of any of the group's
-// particles are dropped (about a third on Sedov 64^3): each round's column
-// of words is ORed over the rows, the used candidates are written back in
-// order to the front of the tile, and every row's bits are compressed to
-// the same used positions, so the pair passes stage and walk only used
-// candidates in an unchanged order.  ccount becomes the used count.
-template <bool GATHER, int MAXG>
+// Then the candidates that are no neighbour of any particle of a tile are
+// dropped from that tile (about a third on Sedov 64^3): each round's column
+// of words is ORed over the tile's rows, the used candidates are written in
+// order to the tile's candidate slot, and every row's bits are compressed
+// to the same used positions, so the pair passes stage and walk only used
+// candidates in an unchanged order.  NH = 2 cuts a group of more than 64
+// particles into two tiles (rows 0-63 and the rest), each with its own used
+// candidates; the first tile's are compacted in place, ahead of the reads.
+// The neighbour predicate is the group's (its candidates and its
+// centre-of-mass filter) whatever the cut.
+template <bool GATHER, int MAXG, int NH>
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
 #pragma clang fp contract(off)
   constexpr int kR = 128;  // candidates per round (4 mask words)
+  constexpr int kRW = kR / 32;
   __shared__ double cx[kR], cy[kR], cz[kR], ch2[kR];
   __shared__ double px[MAXG], py[MAXG], pz[MAXG], ph2[MAXG];
   __shared__ int cnt[MAXG];
-  __shared__ uint32_t rw[kR / 32][MAXG];
+  __shared__ uint32_t rw[kRW][MAXG];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
@@ -335,6 +346,21 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
   const int cc = a.ccount[gi];
   const int64_t mo = a.moff[gi];
   const int64_t cbase = gi * a.ccap;
+  // tiles of this group: nh of them, rows [64 h, ...) of tile h
+  const int64_t t0 = a.gtile ? a.gtile[g] : gi;
+  const int nh = (NH == 2 && gc > kW) ? 2 : 1;
+  int64_t cout[NH], mout[NH];
+  int rows[NH];
+  {
+    const int nw0 = (cc + 31) >> 5;
+#pragma unroll
+    for (int hh = 0; hh < NH; ++hh) {
+      cout[hh] = (a.gtile ? (int64_t)a.tslot[t0 + (hh < nh ? hh : 0)] : gi) *
+                 a.ccap;
+      rows[hh] = nh == 1 ? gc : (hh == 0 ? kW : gc - kW);
+      mout[hh] = mo + (int64_t)hh * kW * nw0;
+    }
+  }
   for (int k = lane; k < gc; k += kW) {
     px[k] = a.x[gs + k];
     py[k] = a.y[gs + k];
@@ -353,11 +379,11 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     gext_lap = a.lap[gn];
     grad = a.rad[gn];
   }
-  // compaction state: used candidates so far, and per row (k =
-  // lane, lane + 64) the bits not yet written, nb of them, wi words written
-  int64_t used_so_far = 0;
+  // compaction state per tile: used candidates so far, words written and
+  // bits pending (per row k = lane + 64 r of the tile's rows)
+  int64_t used_so_far[NH] = {};
   uint64_t pend[MAXG / kW] = {};
-  int nb = 0, wi = 0;
+  int nb[NH] = {}, wi[NH] = {};
   // The next round's candidates are loaded into registers while the
   // current round's masks are compacted, and stored to LDS after.
   int32_t cidx[kR / kW];
@@ -404,7 +430,7 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
       ch2[i] = bh * bh;
     }
     __syncthreads();
-    const int nwr = min(kR / 32, (cc - r0 + 31) >> 5);
+    const int nwr = min(kRW, (cc - r0 + 31) >> 5);
     for (int q = lane; q < nwr * gc; q += kW) {
       const int w = q / gc, k = q - w * gc;
       const double qx = px[k], qy = py[k], qz = pz[k], qh2 = ph2[k];
@@ -428,71 +454,97 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
     __syncthreads();
     if (r0 + kR < cc) prefetch(r0 + kR);
     {
-      // candidates of the round that neighbour some row
-      uint32_t used[kR / 32];
-      int before[kR / 32], total = 0;
+      // candidates of the round that neighbour some row of each tile
+      uint32_t used[NH][kRW];
+      int before[NH][kRW], total[NH];
 #pragma unroll
-      for (int w = 0; w < kR / 32; ++w) {
-        uint32_t v = 0;
-        if (w < nwr)
-          for (int k = lane; k < gc; k += kW) v |= rw[w][k];
+      for (int hh = 0; hh < NH; ++hh) {
+        total[hh] = 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-        used[w] = __builtin_amdgcn_readfirstlane(v);
-        before[w] = total;
-        total += __popc(used[w]);
+        for (int w = 0; w < kRW; ++w) {
+          uint32_t v = 0;
+          if (w < nwr && hh < nh) {
+            if (nh == 1) {
+              for (int k = lane; k < gc; k += kW) v |= rw[w][k];
+            } else {
+              const int k = lane + hh * kW;
+              if (k < gc) v = rw[w][k];
+            }
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+          used[hh][w] = __builtin_amdgcn_readfirstlane(v);
+          before[hh][w] = total[hh];
+          total[hh] += __popc(used[hh][w]);
+        }
       }
-      Compress cm[kR / 32];
 #pragma unroll
-      for (int w = 0; w < kR / 32; ++w) cm[w] = compress_masks(used[w]);
+      for (int hh = 0; hh < NH; ++hh) {
+        if (hh >= nh) break;
+        Compress cm[kRW];
 #pragma unroll
-      for (int q = 0; q < kR / kW; ++q) {
-        const int i = lane + q * kW, w = i >> 5, bit = i & 31;
-        if (used[w] >> bit & 1u)
-          a.cand[cbase + used_so_far + before[w] +
-                 __popc(used[w] & ((1u << bit) - 1u))] = ccur[q];
-      }
-      // each row's bits at the used positions, appended after the bits of
-      // the earlier rounds (the same positions for every row)
+        for (int w = 0; w < kRW; ++w) cm[w] = compress_masks(used[hh][w]);
 #pragma unroll
-      for (int r = 0; r < MAXG / kW; ++r) {
-        const int k = lane + r * kW;
-        if (k >= gc) continue;
-        uint64_t acc = pend[r];
-        int nbl = nb, wil = wi;
+        for (int q = 0; q < kR / kW; ++q) {
+          const int i = lane + q * kW, w = i >> 5, bit = i & 31;
+          if (used[hh][w] >> bit & 1u)
+            a.cand[cout[hh] + used_so_far[hh] + before[hh][w] +
+                   __popc(used[hh][w] & ((1u << bit) - 1u))] = ccur[q];
+        }
+        // each row's bits at the used positions, appended after the bits of
+        // the earlier rounds (the same positions for every row of the tile)
 #pragma unroll
-        for (int w = 0; w < kR / 32; ++w) {
-          if (w >= nwr) break;
-          acc |= (uint64_t)compress(rw[w][k], used[w], cm[w]) << nbl;
-          nbl += __popc(used[w]);
-          if (nbl >= 32) {
-            a.mask[mo + (int64_t)wil * gc + k] = (uint32_t)acc;
-            acc >>= 32;
-            nbl -= 32;
-            ++wil;
+        for (int r = 0; r < MAXG / kW; ++r) {
+          const int k = lane + r * kW;
+          if (k >= gc || (nh == 2 && r != hh)) continue;
+          const int kk = k - (nh == 2 ? hh * kW : 0);
+          uint64_t acc = pend[r];
+          int nbl = nb[hh], wil = wi[hh];
+#pragma unroll
+          for (int w = 0; w < kRW; ++w) {
+            if (w >= nwr) break;
+            acc |= (uint64_t)compress(rw[w][k], used[hh][w], cm[w]) << nbl;
+            nbl += __popc(used[hh][w]);
+            if (nbl >= 32) {
+              a.mask[mout[hh] + (int64_t)wil * rows[hh] + kk] = (uint32_t)acc;
+              acc >>= 32;
+              nbl -= 32;
+              ++wil;
+            }
+          }
+          pend[r] = acc;
+        }
+        for (int w = 0; w < nwr; ++w) {
+          nb[hh] += __popc(used[hh][w]);
+          if (nb[hh] >= 32) {
+            nb[hh] -= 32;
+            ++wi[hh];
           }
         }
-        pend[r] = acc;
+        used_so_far[hh] += total[hh];
       }
-      for (int w = 0; w < nwr; ++w) {
-        nb += __popc(used[w]);
-        if (nb >= 32) {
-          nb -= 32;
-          ++wi;
-        }
-      }
-      used_so_far += total;
     }
     __syncthreads();
   }
-  {
-    if (nb > 0)
+#pragma unroll
+  for (int hh = 0; hh < NH; ++hh) {
+    if (hh >= nh) break;
+    if (nb[hh] > 0)
 #pragma unroll
       for (int r = 0; r < MAXG / kW; ++r) {
         const int k = lane + r * kW;
-        if (k < gc) a.mask[mo + (int64_t)wi * gc + k] = (uint32_t)pend[r];
+        if (k >= gc || (nh == 2 && r != hh)) continue;
+        const int kk = k - (nh == 2 ? hh * kW : 0);
+        a.mask[mout[hh] + (int64_t)wi[hh] * rows[hh] + kk] = (uint32_t)pend[r];
       }
-    if (lane == 0) a.ccount[gi] = (int32_t)used_so_far;
+    if (lane == 0) {
+      if (a.gtile) {
+        a.tcc[t0 + hh] = (int32_t)used_so_far[hh];
+        a.tmo[t0 + hh] = mout[hh];
+      } else {
+        a.ccount[gi] = (int32_t)used_so_far[hh];
+      }
+    }
   }
   if (a.nngb)
     for (int k = lane; k < gc; k += kW) a.nngb[gs + k] = cnt[k];
@@ -503,21 +555,35 @@ struct TileBufs {
   int64_t &ccap, &words;
 };
 
+// SPH mode: the traversal and the masks run per group (the group decides
+// the candidates, tree_topology.h:213-442) and the masks kernel writes the
+// group's tiles.  Gather mode: the predicate involves only the particle's
+// own radius, so the units are the tiles themselves (nt of them, or the ngl
+// listed in glist).
 int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                 int32_t *nngb, const float *hprop, const int32_t *glist,
                 int64_t ngl) {
-  const int64_t ng = glist ? ngl : c->ng;
-  if (ng <= 0) return 0;
-  FSPH_TRY(ensure(c, tb.ccount, (size_t)ng * 4));
-  FSPH_TRY(ensure(c, tb.moff, (size_t)(ng + 1) * 8));
+  const int64_t nu = glist ? ngl : (gather ? c->nt : c->ng);
+  if (nu <= 0) return 0;
+  const int32_t *ustart = gather ? c->tstart.as<int32_t>()
+                                 : c->gstart.as<int32_t>();
+  const int32_t *ucount = gather ? c->tcount.as<int32_t>()
+                                 : c->gcount.as<int32_t>();
+  // candidate slots: one per unit, plus the second tiles of split groups
+  const int64_t nslots = nu + (gather ? 0 : c->nsplit);
+  // rows of the largest unit (sizes the kernels' LDS arrays)
+  const bool u64 = (gather ? c->tiles_w : c->groups_max) <= kW;
+  FSPH_TRY(ensure(c, tb.ccount, (size_t)nu * 4));
+  FSPH_TRY(ensure(c, tb.moff, (size_t)(nu + 1) * 8));
   for (int attempt = 0;; ++attempt) {
-    FSPH_TRY(ensure(c, tb.cand, (size_t)ng * tb.ccap * 4));
+    FSPH_TRY(ensure(c, tb.cand, (size_t)nslots * tb.ccap * 4));
     FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
     TravArgs t{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
                c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
                c->nbox.as<double>(),    c->nchild.as<int32_t>(),
-               c->gstart.as<int32_t>(), c->gcount.as<int32_t>(),
-               c->gnode.as<int32_t>(),  tb.cand.as<int32_t>(),
+               ustart,                  ucount,
+               gather ? nullptr : c->gnode.as<int32_t>(),
+               tb.cand.as<int32_t>(),
                tb.ccount.as<int32_t>(), tb.ccap,
                c->flags.as<int>(),      scale,
                gather ? 1 : 0,          hprop,
@@ -526,21 +592,20 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
                c->batch.n_inst ? c->batch.root.as<int32_t>() : nullptr};
     {
       PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_TRAVERSE);
-      const bool g64 = c->groups_max <= kW;
       auto go = [&](auto gf) {
         constexpr bool GF = decltype(gf)::value;
-        if (c->big_stack && g64)
+        if (c->big_stack && u64)
           k_traverse<kStackLarge, kW, GF>
-              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+              <<<(unsigned)nu, kW, 0, c->stream>>>(t);
         else if (c->big_stack)
           k_traverse<kStackLarge, kMaxGroup, GF>
-              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
-        else if (g64)
+              <<<(unsigned)nu, kW, 0, c->stream>>>(t);
+        else if (u64)
           k_traverse<kStackSmall, kW, GF>
-              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+              <<<(unsigned)nu, kW, 0, c->stream>>>(t);
         else
           k_traverse<kStackSmall, kMaxGroup, GF>
-              <<<(unsigned)ng, kW, 0, c->stream>>>(t);
+              <<<(unsigned)nu, kW, 0, c->stream>>>(t);
       };
       if (gather)
         go(std::true_type{});
@@ -550,16 +615,17 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
     FSPH_HIP(c, hipGetLastError());
     // mask offsets (an overflowed traversal leaves zero candidates), then
     // the flags and the word count in one read-back
-    k_mask_words<<<(unsigned)((ng + 255) / 256), 256, 0, c->stream>>>(
-        tb.ccount.as<int32_t>(), c->gcount.as<int32_t>(), glist, ng,
-        tb.moff.as<int64_t>());
+    k_mask_words<<<(unsigned)((nu + 255) / 256), 256, 0, c->stream>>>(
+        tb.ccount.as<int32_t>(), ucount, glist, nu, tb.moff.as<int64_t>());
     FSPH_HIP(c, hipGetLastError());
-    FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), ng));
+    FSPH_TRY(scan_i64(c, tb.moff.as<int64_t>(), tb.moff.as<int64_t>(), nu));
     k_tile_mail<<<1, 64, 0, c->stream>>>(c->flags.as<int>(),
-                                         tb.moff.as<int64_t>() + ng,
+                                         tb.moff.as<int64_t>() + nu,
                                          c->mail.as<int64_t>());
     FSPH_TRY(read_mail(c, 2));
     const int fl = (int)c->mail_h[0];
+    if (fl & kFlagScanStall)
+      return fail(c, FSPH_E_HIP, "scan: a tile's look-back did not complete");
     if (fl & kFlagStack) {
       if (c->big_stack)
         return fail(c, FSPH_E_CAP, "neighbour traversal stack overflow");
@@ -578,27 +644,38 @@ int build_tiles(Ctx *c, TileBufs tb, bool gather, double scale,
   const int64_t words = c->mail_h[1];
   tb.words = words;
   FSPH_TRY(ensure(c, tb.mask, (size_t)(words > 0 ? words : 1) * 4));
+  const bool tiled = !gather;
+  if (tiled) {
+    FSPH_TRY(ensure(c, c->tccount, (size_t)c->nt * 4));
+    FSPH_TRY(ensure(c, c->tmoff, (size_t)c->nt * 8));
+  }
   MaskArgs m{c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
              c->f[PF_Z].as<double>(), c->f[PF_H].as<double>(),
              c->ncom.as<double>(),    c->nrad.as<double>(),
-             c->nlap.as<double>(),    c->gstart.as<int32_t>(),
-             c->gcount.as<int32_t>(), c->gnode.as<int32_t>(),
+             c->nlap.as<double>(),    ustart,
+             ucount,                  gather ? nullptr : c->gnode.as<int32_t>(),
              tb.cand.as<int32_t>(),   tb.ccount.as<int32_t>(),
              tb.moff.as<int64_t>(),   tb.ccap,
              tb.mask.as<uint32_t>(),  nngb,
              scale,                   gather ? 1 : 0,
-             hprop,                   glist};
+             hprop,                   glist,
+             tiled ? c->gtile.as<int32_t>() : nullptr,
+             tiled ? c->tslot.as<int32_t>() : nullptr,
+             tiled ? c->tccount.as<int32_t>() : nullptr,
+             tiled ? c->tmoff.as<int64_t>() : nullptr};
   {
     PhaseTimer tm(c, gather ? FSPH_T_H : FSPH_T_MASKS);
-    const bool g64 = c->groups_max <= kW;
-    if (gather && g64)
-      k_masks<true, kW><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+    const bool split = tiled && c->nsplit > 0;
+    if (gather && u64)
+      k_masks<true, kW, 1><<<(unsigned)nu, kW, 0, c->stream>>>(m);
     else if (gather)
-      k_masks<true, kMaxGroup><<<(unsigned)ng, kW, 0, c->stream>>>(m);
-    else if (g64)
-      k_masks<false, kW><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+      k_masks<true, kMaxGroup, 1><<<(unsigned)nu, kW, 0, c->stream>>>(m);
+    else if (u64)
+      k_masks<false, kW, 1><<<(unsigned)nu, kW, 0, c->stream>>>(m);
+    else if (split)
+      k_masks<false, kMaxGroup, 2><<<(unsigned)nu, kW, 0, c->stream>>>(m);
     else
-      k_masks<false, kMaxGroup><<<(unsigned)ng, kW, 0, c->stream>>>(m);
+      k_masks<false, kMaxGroup, 1><<<(unsigned)nu, kW, 0, c->stream>>>(m);
   }
   return hip_check(c, hipGetLastError(), "k_masks");
 }

@@ -1,14 +1,15 @@
 // k_sph.hip -- the SPH pair passes over the neighbour tiles.
 //
-// One 128-thread workgroup per group.  Candidates and the chunk's mask
-// words are staged in LDS in chunks of CH candidates (SoA, coalesced
-// loads); the group's own particles are staged once.  The chunk's
+// One 128-thread workgroup per tile (up to 128 particles of one SPH group),
+// or one 64-thread wavefront per tile of up to 64.  Candidates and the
+// chunk's mask words are staged in LDS in chunks of CH candidates (SoA,
+// coalesced loads); the tile's own particles are staged once.  The chunk's
 // neighbour pairs are then dealt to the threads in equal runs (the balanced
 // schedule below), so a thread's loop count is the chunk's pair count over
 // 128 whatever the particles' neighbour counts, and the partial sums are
 // added up per particle in a fixed order.  Pair arithmetic runs in T (float
 // by default, double for FSPH_PREC_FP64) with compensated (Kahan) sums; FP32
-// kernels stage positions relative to the group's first particle (rounded
+// kernels stage positions relative to the tile's first particle (rounded
 // once from an FP64 difference), FP64 kernels stage them as they are.
 #include <cfloat>
 #include <cstring>
@@ -58,8 +59,9 @@ template <class T> __device__ inline double pos_origin(const double *x,
 
 struct Phys {
   int kernel;
-  double gamma, K, alpha, beta, eps;
+  double gamma, alpha, beta, eps;
   int eos;
+  const double *kad;  // per-particle adiabat (polytropic EOS)
 };
 
 // kernels.h: cubic spline (106-153) and Wendland C6 3D (502-566); h is the
@@ -102,25 +104,29 @@ template <class T> __device__ inline T kernel_F(int kind, T r, T ir, T ih) {
   return T(1365.0 / (64.0 * M_PI)) * ih4 * dW * ir;
 }
 
-__device__ inline void eos_pc(const Phys &ph, double rho, double u, double *P,
-                              double *c) {
+// K: the particle's adiabat (polytropic EOS only)
+__device__ inline void eos_pc(const Phys &ph, double rho, double u, double K,
+                              double *P, double *c) {
   if (ph.eos == FSPH_EOS_POLYTROPIC) {  // eos.h:102-119
-    *P = ph.K * pow(rho, ph.gamma);
-    *c = sqrt(ph.K * ph.gamma * pow(rho, ph.gamma - 1.));
+    *P = K * pow(rho, ph.gamma);
+    *c = sqrt(K * ph.gamma * pow(rho, ph.gamma - 1.));
   } else {  // eos.h:166-183
     *P = (ph.gamma - 1.) * rho * u;
     *c = sqrt(ph.gamma * (ph.gamma - 1.) * u);
   }
 }
 
+// Tile tables: per tile its first particle and rows, its candidate slot
+// and count, and the offset of its mask words.
 struct Tiles {
-  const int32_t *gstart, *gcount, *cand, *ccount;
+  const int32_t *tstart, *tcount, *tslot, *cand, *ccount;
   const int64_t *moff;
   const uint32_t *mask;
   int64_t ccap;
 };
 
-// Per-workgroup view of one group's tile.
+// Per-workgroup view of one tile (the names say group: the rows of a tile
+// are particles of one group).
 struct Grp {
   int64_t gs, mo, cb;
   int gc, cc;
@@ -128,11 +134,11 @@ struct Grp {
 
 __device__ inline Grp grp_of(const Tiles &t, int64_t g) {
   Grp c;
-  c.gs = t.gstart[g];
-  c.gc = t.gcount[g];
+  c.gs = t.tstart[g];
+  c.gc = t.tcount[g];
   c.cc = t.ccount[g];
   c.mo = t.moff[g];
-  c.cb = g * t.ccap;
+  c.cb = (int64_t)t.tslot[g] * t.ccap;
   return c;
 }
 
@@ -211,7 +217,7 @@ __device__ inline void density_group(const DensArgs<T> &a, int64_t g) {
     const int64_t i = c.gs + t;
     const double rr = (double)(rho.s - rho.c);
     double P, cs;
-    eos_pc(a.ph, rr, a.u[i], &P, &cs);
+    eos_pc(a.ph, rr, a.u[i], a.ph.kad[i], &P, &cs);
     a.rho[i] = (T)rr;
     a.P[i] = (T)P;
     a.c[i] = (T)cs;
@@ -567,7 +573,7 @@ __device__ inline void force_group(const ForceArgs<T, DUDT> &a, int64_t g) {
     const double dti = a.dtv[a.inst ? a.inst[i] : 0];
     const double uh = a.u[i] + 0.5 * dti * (double)du;
     double P, cs;
-    eos_pc(a.ph, (double)prho, uh, &P, &cs);
+    eos_pc(a.ph, (double)prho, uh, a.ph.kad[i], &P, &cs);
     a.P_out[i] = (T)P;
     a.c_out[i] = (T)cs;
   }
@@ -633,12 +639,12 @@ __global__ __launch_bounds__(NT) void k_force_p(ForceArgs<T, DUDT> a,
 }
 
 // Calls f(CH, NT) (as std::integral_constant values) for the context's
-// chunk and workgroup width: one wavefront when no group exceeds 64
-// particles, two otherwise.
+// chunk and workgroup width: one wavefront for tiles of at most 64 rows,
+// two otherwise.
 template <int V> using IC = std::integral_constant<int, V>;
-// (by the group limit the groups were built with: a group_max changed since
+// (by the width the tiles were built with: a tile width changed since
 // applies from the next update_iteration)
-inline bool narrow(const Ctx *c) { return c->groups_max <= 64; }
+inline bool narrow(const Ctx *c) { return c->tiles_w <= 64; }
 template <class F> void dispatch(const Ctx *c, F &&f) {
   const bool nr = narrow(c);
   switch (c->chunk) {
@@ -657,7 +663,7 @@ Phys phys_of(const Ctx *c) {
   Phys p;
   p.kernel = c->p.kernel;
   p.gamma = c->p.gamma;
-  p.K = c->p.poly_K;
+  p.kad = c->f[PF_KAD].as<double>();
   p.alpha = c->p.visc_alpha;
   p.beta = c->p.visc_beta;
   p.eps = c->p.visc_eps;
@@ -667,11 +673,12 @@ Phys phys_of(const Ctx *c) {
 
 Tiles tiles_of(const Ctx *c) {
   Tiles t;
-  t.gstart = c->gstart.as<int32_t>();
-  t.gcount = c->gcount.as<int32_t>();
+  t.tstart = c->tstart.as<int32_t>();
+  t.tcount = c->tcount.as<int32_t>();
+  t.tslot = c->tslot.as<int32_t>();
   t.cand = c->cand.as<int32_t>();
-  t.ccount = c->ccount.as<int32_t>();
-  t.moff = c->moff.as<int64_t>();
+  t.ccount = c->tccount.as<int32_t>();
+  t.moff = c->tmoff.as<int64_t>();
   t.mask = c->mask.as<uint32_t>();
   t.ccap = c->ccap;
   return t;
@@ -680,7 +687,7 @@ Tiles tiles_of(const Ctx *c) {
 template <class T, bool DIV> void vsig_launch(Ctx *c, const VsigArgs<T> &v) {
   dispatch(c, [&](auto ch, auto nt) {
     k_vsig<T, DIV, decltype(ch)::value, decltype(nt)::value>
-        <<<(unsigned)c->ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(v);
+        <<<(unsigned)c->nt, (unsigned)decltype(nt)::value, 0, c->stream>>>(v);
   });
 }
 
@@ -754,7 +761,7 @@ template <class T> int density_impl(Ctx *c, bool with_vsig) {
     PhaseTimer tm(c, FSPH_T_DENSITY);
     dispatch(c, [&](auto ch, auto nt) {
       k_density<T, decltype(ch)::value, decltype(nt)::value>
-          <<<(unsigned)c->ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(a);
+          <<<(unsigned)c->nt, (unsigned)decltype(nt)::value, 0, c->stream>>>(a);
     });
   }
   FSPH_HIP(c, hipGetLastError());
@@ -774,7 +781,7 @@ template <class T, bool DUDT, int VS = 0> int force_impl(Ctx *c) {
   const ForceArgs<T, DUDT> a = force_args<T, DUDT>(c);
   {
     PhaseTimer tm(c, DUDT ? FSPH_T_DUDT : FSPH_T_ACCEL);
-    const unsigned ng = (unsigned)c->ng;
+    const unsigned ng = (unsigned)c->nt;
     dispatch(c, [&](auto ch, auto nt) {
       k_force<T, DUDT, decltype(ch)::value, decltype(nt)::value, VS>
           <<<ng, (unsigned)decltype(nt)::value, 0, c->stream>>>(a);
@@ -845,7 +852,7 @@ const void *sph_graph_func(const Ctx *c, int pass) {
 unsigned sph_graph_block(const Ctx *c) { return narrow(c) ? 64u : 128u; }
 
 void sph_graph_work(Ctx *c, int pass, void *work) {
-  const Work w{c->graph.counters.as<unsigned long long>() + pass, c->ng};
+  const Work w{c->graph.counters.as<unsigned long long>() + pass, c->nt};
   std::memcpy(work, &w, sizeof(w));
 }
 

@@ -511,6 +511,34 @@ int stage_update_h(Ctx *c) {
   return hip_check(c, hipGetLastError(), "k_update_h");
 }
 
+// eos::init (eos.h:76-92): the adiabat K = P / rho^gamma of each particle
+// from its current density and pressure; with no density uploaded, the
+// parameter poly_K.
+template <class T>
+__global__ __launch_bounds__(kT) void k_eos_init(const T *__restrict__ rho,
+                                                 const T *__restrict__ P,
+                                                 int64_t n, double gamma,
+                                                 double K0, int from_state,
+                                                 double *__restrict__ K) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  K[i] = from_state ? (double)P[i] / pow((double)rho[i], gamma) : K0;
+}
+
+int stage_eos_init(Ctx *c) {
+  if (c->p.eos != FSPH_EOS_POLYTROPIC || c->n <= 0) return 0;
+  const int fs = c->have_rhoP ? 1 : 0;
+  if (c->rsize == 4)
+    k_eos_init<float><<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_RHO].as<float>(), c->f[PF_P].as<float>(), c->n, c->p.gamma,
+        c->p.poly_K, fs, c->f[PF_KAD].as<double>());
+  else
+    k_eos_init<double><<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_RHO].as<double>(), c->f[PF_P].as<double>(), c->n,
+        c->p.gamma, c->p.poly_K, fs, c->f[PF_KAD].as<double>());
+  return hip_check(c, hipGetLastError(), "k_eos_init");
+}
+
 template <class T> void step_graph_args_t(Ctx *c, char *slot) {
   static_assert(sizeof(StepSlot<T>) <= kGraphSlot, "slot");
   StepSlot<T> s{};

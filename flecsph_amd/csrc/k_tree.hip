@@ -371,35 +371,62 @@ __global__ __launch_bounds__(kT) void k_cofm(CofmArgs a,
 
 // Groups (tree_topology.h:222-235): nodes with at most gmax particles whose
 // parent has more (or the root), and single leaves whose parent has more.
+// The flag of a group also says how many pair-kernel tiles it makes: a
+// group of more than tw (the tile width) particles is cut into two tiles of
+// rows.  Flags are packed as groups (low 32 bits) + tiles beyond the first
+// (high 32 bits), so one exclusive scan numbers both.
 __global__ __launch_bounds__(kT) void k_group_flags(
     const int32_t *__restrict__ count, const int32_t *__restrict__ parent,
     int64_t nn, const int32_t *__restrict__ leaf_parent, int64_t n, int gmax,
-    int32_t *__restrict__ flag) {
+    int tw, int64_t *__restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i < nn) {
     const int32_t p = parent[i];
-    flag[i] = count[i] <= gmax && (p < 0 || count[p] > gmax);
+    const bool grp = count[i] <= gmax && (p < 0 || count[p] > gmax);
+    flag[i] = grp ? 1 + ((int64_t)(count[i] > tw) << 32) : 0;
   } else if (i < nn + n) {
     flag[i] = count[leaf_parent[i - nn]] > gmax;
   }
 }
 
+// Writes group g (its start, count and node) and its tiles: tile t0 =
+// g + (split groups before g) holds the first min(count, tw) rows and
+// reads candidate slot g; a split group's second tile t0 + 1 holds the rest
+// and reads slot ng + (split groups before g).
 __global__ __launch_bounds__(kT) void k_group_write(
-    const int32_t *__restrict__ flag, const int64_t *__restrict__ pos,
+    const int64_t *__restrict__ flag, const int64_t *__restrict__ pos,
     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-    int64_t nn, int64_t n, int32_t *__restrict__ gstart,
-    int32_t *__restrict__ gcount, int32_t *__restrict__ gnode) {
+    int64_t nn, int64_t n, int64_t ng, int tw, int32_t *__restrict__ gstart,
+    int32_t *__restrict__ gcount, int32_t *__restrict__ gnode,
+    int32_t *__restrict__ gtile, int32_t *__restrict__ tstart,
+    int32_t *__restrict__ tcount, int32_t *__restrict__ tslot) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= nn + n || !flag[i]) return;
-  const int64_t g = pos[i];
+  const int64_t pk = pos[i];
+  const int64_t g = pk & 0xffffffffll, s = pk >> 32;
+  const int64_t t0 = g + s;
+  int32_t st, cnt;
   if (i < nn) {
-    gstart[g] = start[i];
-    gcount[g] = count[i];
+    st = start[i];
+    cnt = count[i];
     gnode[g] = (int32_t)i;
   } else {
-    gstart[g] = (int32_t)(i - nn);
-    gcount[g] = 1;
+    st = (int32_t)(i - nn);
+    cnt = 1;
     gnode[g] = -1;
+  }
+  gstart[g] = st;
+  gcount[g] = cnt;
+  gtile[g] = (int32_t)t0;
+  tstart[t0] = st;
+  tslot[t0] = (int32_t)g;
+  if (cnt > tw) {
+    tcount[t0] = tw;
+    tstart[t0 + 1] = st + tw;
+    tcount[t0 + 1] = cnt - tw;
+    tslot[t0 + 1] = (int32_t)(ng + s);
+  } else {
+    tcount[t0] = cnt;
   }
 }
 
@@ -504,14 +531,16 @@ int group_flags(Ctx *c) {
   const int64_t n = c->n, nn = c->nn;
   const int64_t tot = nn + n;
   c->groups_max = c->p.group_max;
-  FSPH_TRY(ensure(c, c->gflag, (size_t)tot * 4));
+  // tile width: rows of one pair-kernel workgroup
+  c->tiles_w = c->groups_max <= 64 ? 64 : c->tile_rows;
+  FSPH_TRY(ensure(c, c->gflag, (size_t)tot * 8));
   FSPH_TRY(ensure(c, c->gscan, (size_t)(tot + 1) * 8));
   k_group_flags<<<blocks_for(tot), kT, 0, c->stream>>>(
       c->ncount.as<int32_t>(), c->nparent.as<int32_t>(), nn,
-      c->leaf_parent.as<int32_t>(), n, c->p.group_max, c->gflag.as<int32_t>());
+      c->leaf_parent.as<int32_t>(), n, c->p.group_max, c->tiles_w,
+      c->gflag.as<int64_t>());
   FSPH_HIP(c, hipGetLastError());
-  FSPH_TRY(scan_i32_to_i64(c, c->gflag.as<int32_t>(), c->gscan.as<int64_t>(),
-                           tot));
+  FSPH_TRY(scan_i64(c, c->gflag.as<int64_t>(), c->gscan.as<int64_t>(), tot));
   FSPH_HIP(c, hipMemcpyAsync(c->mail_h + (kMail - 1),
                              c->gscan.as<int64_t>() + tot, 8,
                              hipMemcpyDeviceToHost, c->stream));
@@ -523,15 +552,25 @@ int stage_groups(Ctx *c) {
   const int64_t n = c->n, nn = c->nn;
   const int64_t tot = nn + n;
   FSPH_HIP(c, hipEventSynchronize(c->ev_groups));
-  const int64_t ng = c->mail_h[kMail - 1];
+  const int64_t packed = c->mail_h[kMail - 1];
+  const int64_t ng = packed & 0xffffffffll, nsplit = packed >> 32;
   c->ng = ng;
+  c->nsplit = nsplit;
+  c->nt = ng + nsplit;
   FSPH_TRY(ensure(c, c->gstart, (size_t)ng * 4));
   FSPH_TRY(ensure(c, c->gcount, (size_t)ng * 4));
   FSPH_TRY(ensure(c, c->gnode, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, c->gtile, (size_t)ng * 4));
+  FSPH_TRY(ensure(c, c->tstart, (size_t)c->nt * 4));
+  FSPH_TRY(ensure(c, c->tcount, (size_t)c->nt * 4));
+  FSPH_TRY(ensure(c, c->tslot, (size_t)c->nt * 4));
   k_group_write<<<blocks_for(tot), kT, 0, c->stream>>>(
-      c->gflag.as<int32_t>(), c->gscan.as<int64_t>(), c->nstart.as<int32_t>(),
-      c->ncount.as<int32_t>(), nn, n, c->gstart.as<int32_t>(),
-      c->gcount.as<int32_t>(), c->gnode.as<int32_t>());
+      c->gflag.as<int64_t>(), c->gscan.as<int64_t>(), c->nstart.as<int32_t>(),
+      c->ncount.as<int32_t>(), nn, n, ng, c->tiles_w,
+      c->gstart.as<int32_t>(), c->gcount.as<int32_t>(),
+      c->gnode.as<int32_t>(), c->gtile.as<int32_t>(),
+      c->tstart.as<int32_t>(), c->tcount.as<int32_t>(),
+      c->tslot.as<int32_t>());
   return hip_check(c, hipGetLastError(), "k_group_write");
 }
 

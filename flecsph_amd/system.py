@@ -346,14 +346,17 @@ class BatchSystem(BodySystem):
         return it.value, dt.value, t.value
 
 
-def set_tuning(system, group_max, chunk):
-    """Applies (group_max, chunk), e.g. autotune's result, to a system."""
-    system._call("set_tuning", int(group_max), int(chunk))
+def set_tuning(system, tile_rows, chunk):
+    """Applies (tile_rows, chunk), e.g. autotune's result, to a system:
+    tile_rows (64 or 128) rows of one group per pair-kernel workgroup, chunk
+    candidates staged in LDS per round.  The neighbour lists do not depend on
+    either (they follow the system's group_max, the reference's grouping)."""
+    system._call("set_tuning", int(tile_rows), int(chunk))
 
 
 def autotune(system, steps=2):
     """Runs the library's autotuner on a BodySystem or BatchSystem; returns
-    (group_max, chunk)."""
+    (tile_rows, chunk)."""
     g, ch = C.c_int32(), C.c_int32()
     system._call("autotune", int(steps), C.byref(g), C.byref(ch))
     return g.value, ch.value

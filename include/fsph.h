@@ -362,25 +362,30 @@ int fsph_instance_time(const fsph_ctx *ctx, int inst, int64_t *iteration,
 int fsph_enable_graph(fsph_ctx *ctx, int on);
 
 /* Autotuner (new): times `steps` calls of fsph_step for each candidate
-   leaf-group width (group_max 32, 64, 128) and candidate tile chunk of the
-   pair kernels (64, 128, 256 candidates staged in LDS per round), keeps the
-   fastest and returns it.  Every candidate is timed from the same state: the
-   particle fields, clocks and instance clocks are saved first (one device
-   copy of every field) and restored before each candidate and at the end,
-   so the call leaves the simulation where it was (after the initial
-   iteration, which it runs if it has not run).  The learned predictor of
-   FSPH_H_LEARNED keeps what it learns during the timed steps.  The physics
-   does not depend on the choice (neighbour sets are the same; sums are only
-   re-associated). */
-int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *group_max,
+   tile width of the pair kernels (64 or 128 rows per workgroup: a group of
+   more than 64 particles is processed as two tiles of its rows, each with
+   the group's candidates that neighbour one of its rows) and candidate
+   chunk (64, 128, 256 candidates staged in LDS per round), keeps the
+   fastest and returns it.  group_max, the reference's grouping
+   (tree_topology::sub_entities_, tree_topology.h:222-235,2121), is not
+   searched: the traversal's node test uses the whole group
+   (tree_topology.h:323-355), so the neighbour lists depend on it.  The
+   tile width and the chunk only cut and re-associate the sums: the
+   neighbour lists are those of the configured group_max for every choice.
+   Every candidate is timed from the same state: the particle fields,
+   clocks and instance clocks are saved first (one device copy of every
+   field) and restored before each candidate and at the end, so the call
+   leaves the simulation where it was (after the initial iteration, which it
+   runs if it has not run).  The learned predictor of FSPH_H_LEARNED keeps
+   what it learns during the timed steps. */
+int fsph_autotune(fsph_ctx *ctx, int steps, int32_t *tile_rows,
                   int32_t *chunk);
 
-/* Sets the two tuning choices the autotuner searches (new): group_max
-   (1..128, the reference's sub_entities_ limit of a leaf group) and chunk
-   (64, 128 or 256).  group_max takes effect at the next
-   fsph_update_iteration (the current groups keep the limit they were built
+/* Sets the two tuning choices the autotuner searches (new): tile_rows (64
+   or 128) and chunk (64, 128 or 256).  tile_rows takes effect at the next
+   fsph_update_iteration (the current tiles keep the width they were built
    with); chunk at the next pair pass. */
-int fsph_set_tuning(fsph_ctx *ctx, int32_t group_max, int32_t chunk);
+int fsph_set_tuning(fsph_ctx *ctx, int32_t tile_rows, int32_t chunk);
 
 #ifdef __cplusplus
 }

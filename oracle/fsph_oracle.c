@@ -44,6 +44,8 @@ struct fo_sys {
   int64_t n, n_real, cap;
   double *x, *y, *z, *vx, *vy, *vz, *vhx, *vhy, *vhz, *u, *h, *m;
   double *rho, *P, *c, *vsig, *ax, *ay, *az, *dudt, *gx, *gy, *gz, *phi;
+  double *kad; /* adiabat of the polytropic EOS (body_u::adiabatic) */
+  int have_rhoP; /* the last upload gave rho and P */
   int64_t *id;
   uint64_t *key;
   int32_t *nngb;
@@ -81,12 +83,13 @@ static double kernel_width(const fsph_params *p) {
 /* ------------------------------------------------------------------ */
 /* allocation                                                          */
 /* ------------------------------------------------------------------ */
-#define NDBL 24
+#define NDBL 25
 static double **dbl_fields(fo_sys *s, double **out[NDBL]) {
   double **f[NDBL] = {&s->x,   &s->y,    &s->z,   &s->vx,  &s->vy,  &s->vz,
                       &s->vhx, &s->vhy,  &s->vhz, &s->u,   &s->h,   &s->m,
                       &s->rho, &s->P,    &s->c,   &s->vsig, &s->ax, &s->ay,
-                      &s->az,  &s->dudt, &s->gx,  &s->gy,  &s->gz,  &s->phi};
+                      &s->az,  &s->dudt, &s->gx,  &s->gy,  &s->gz,  &s->phi,
+                      &s->kad};
   memcpy(out, f, sizeof(f));
   return NULL;
 }
@@ -164,6 +167,8 @@ int fo_upload(fo_sys *s, const fsph_particles *in) {
     s->gx[i] = s->gy[i] = s->gz[i] = s->phi[i] = 0;
     s->key[i] = 0; s->nngb[i] = 0; s->hngb[i] = 0; s->hiter[i] = 0;
   }
+  s->have_rhoP = in->rho && in->P;
+  fo_eos_init(s);
   free_tree(s);
   s->iteration = 0;
   s->dt = s->p.initial_dt;
@@ -822,19 +827,25 @@ void fo_kernel_gradient(int kernel, const double v[3], double h,
 /* ------------------------------------------------------------------ */
 /* A8: density + EOS + signal speed, default_physics.h:194-320         */
 /* ------------------------------------------------------------------ */
+/* K: the particle's adiabat (polytropic EOS) */
 static void eos_pressure_soundspeed(const fsph_params *p, double rho, double u,
-                                    double *P, double *c) {
+                                    double K, double *P, double *c) {
   if (p->eos == FSPH_EOS_POLYTROPIC) { /* eos.h:102-119 */
-    *P = p->poly_K * pow(rho, p->gamma);
-    *c = sqrt(p->poly_K * p->gamma * pow(rho, p->gamma - 1.));
+    *P = K * pow(rho, p->gamma);
+    *c = sqrt(K * p->gamma * pow(rho, p->gamma - 1.));
   } else { /* eos.h:166-183 */
     *P = (p->gamma - 1.) * rho * u;
     *c = sqrt(p->gamma * (p->gamma - 1.) * u);
   }
 }
 
+/* eos::init (eos.h:76-92, 152-159): the adiabat K = P / rho^gamma of each
+   particle from its current density and pressure (read by the polytropic
+   EOS only).  Without uploaded rho and P it is the parameter poly_K. */
 int fo_eos_init(fo_sys *s) {
-  (void)s; /* polytropic K is a parameter here; ideal EOS reads none */
+  for (int64_t i = 0; i < s->n; ++i)
+    s->kad[i] = s->have_rhoP ? s->P[i] / pow(s->rho[i], s->p.gamma)
+                             : s->p.poly_K;
   return 0;
 }
 
@@ -865,7 +876,7 @@ int fo_density_eos(fo_sys *s) {
       rho_a += s->m[b] * Wab;
     }
     s->rho[a] = rho_a;
-    eos_pressure_soundspeed(p, rho_a, s->u[a], &s->P[a], &s->c[a]);
+    eos_pressure_soundspeed(p, rho_a, s->u[a], s->kad[a], &s->P[a], &s->c[a]);
   }
 #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
   for (int64_t a = 0; a < n; ++a) {
@@ -987,7 +998,7 @@ int fo_dudt(fo_sys *s) {
   /* recompute_pressure_soundspeed: P, c at u + dt/2 du/dt */
   for (int64_t a = 0; a < n; ++a) {
     double uh = s->u[a] + 0.5 * s->dt * s->dudt[a];
-    eos_pressure_soundspeed(p, s->rho[a], uh, &s->P[a], &s->c[a]);
+    eos_pressure_soundspeed(p, s->rho[a], uh, s->kad[a], &s->P[a], &s->c[a]);
   }
   return 0;
 }

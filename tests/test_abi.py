@@ -47,8 +47,10 @@ def test_default_params_match_reference_defaults():
         else:
             assert a == b, name
     # params.h: initial_dt 0.001, sph_viscosity_epsilon 0.01,
-    # gravitational_constant 1, sub_entities_ 128 (tree_topology.h:2121)
+    # gravitational_constant 1, sph_eta 1.5 (params.h:212),
+    # sub_entities_ 128 (tree_topology.h:2121)
     assert p.initial_dt == 0.001 and p.visc_eps == 0.01 and p.G == 1.0
+    assert p.eta == 1.5
     assert p.group_max == 128
 
 

@@ -105,7 +105,7 @@ def test_autotune_keeps_the_physics():
     before = {f: s.get(f).copy() for f in ("x", "vx", "u", "h")}
     t_before = s.time()
     g, ch = autotune(s, steps=1)
-    assert g in (32, 64, 128) and ch in (64, 128, 256)
+    assert g in (64, 128) and ch in (64, 128, 256)
     assert s.time() == t_before
     for f, v in before.items():
         assert np.array_equal(s.get(f), v), f
@@ -121,35 +121,53 @@ def test_autotune_keeps_the_physics():
     s.close()
 
 
-def test_set_tuning_keeps_neighbour_sets():
-    """group_max and chunk change the leaf groups and the LDS rounds, not the
-    neighbour sets: the centre-of-mass filter of a group is conservative."""
+@pytest.mark.parametrize("case", ["random_varh", "polytrope50k"])
+def test_set_tuning_keeps_neighbour_sets(oracle_mod, case):
+    """tile_rows and chunk cut the groups' rows into tiles and the
+    candidates into LDS rounds; the groups stay the reference's (group_max
+    128), so the neighbour lists and the signal speed are the oracle's at
+    group_max 128 for every choice.  Variable-h inputs: with uniform h a
+    different grouping would give the same lists and prove nothing."""
     from flecsph_amd import set_tuning
-    parts, prm = generators.sedov(n_side=16)
-    lists = []
-    for gm, ch in ((128, 128), (64, 64), (32, 256)):
-        s = BodySystem(_abi.default_params(**prm))
+    if case == "random_varh":
+        parts, prm = generators.random_box(6000, seed=101, h_spread=1.5)
+    else:
+        parts, prm = generators.polytrope(n=50_000)
+        prm = dict(prm, gravity=_abi.GRAV_NONE)
+    prm = dict(prm, precision=_abi.PREC_FP64)
+    p = _abi.default_params(**prm)
+    assert p.group_max == 128
+    o = oracle_mod.OracleSystem(p, threads=8)
+    o.upload(parts)
+    o.update_iteration()
+    o.density_eos()
+    oo, io = o.neighbor_lists()
+    io_sorted = np.concatenate([np.sort(io[oo[i]:oo[i + 1]])
+                                for i in range(len(oo) - 1)])
+    vs_o = o.get("vsig")
+    for tr, ch in ((64, 64), (128, 128), (64, 256)):
+        s = BodySystem(p)
         s.upload(parts)
-        set_tuning(s, gm, ch)
+        set_tuning(s, tr, ch)
         s.update_iteration()
+        s.density_eos()
         off, idx = s.neighbor_lists()
-        lists.append((off, np.concatenate(
-            [np.sort(idx[off[i]:off[i + 1]]) for i in range(len(off) - 1)])))
+        assert np.array_equal(off, oo), (tr, ch)
+        assert np.array_equal(idx, io_sorted), (tr, ch)
+        vs = s.get("vsig")
+        assert np.abs(vs - vs_o).max() <= 1e-12 * np.abs(vs_o).max(), (tr, ch)
         with pytest.raises(FsphError):
-            set_tuning(s, 129, 128)
+            set_tuning(s, 96, 128)
         with pytest.raises(FsphError):
             set_tuning(s, 64, 100)
         s.close()
-    for off, idx in lists[1:]:
-        assert np.array_equal(off, lists[0][0])
-        assert np.array_equal(idx, lists[0][1])
 
 
-def test_group_max_change_applies_from_the_next_update():
-    """A group_max lowered between update_iteration and the pair passes
+def test_tile_width_change_applies_from_the_next_update():
+    """A tile width changed between update_iteration and the pair passes
     takes effect at the next update_iteration: the passes keep sizing their
-    work for the groups they were given (up to 128 particles here), and give
-    the results of a context that never changed it."""
+    work for the tiles they were given (up to 128 rows here), and give the
+    results of a context that never changed it."""
     from flecsph_amd import set_tuning
     parts, prm = generators.random_box(6000, seed=101, h_spread=1.5)
     out = []
@@ -175,7 +193,7 @@ def test_group_max_change_applies_from_the_next_update():
 def test_graph_replay_matches_eager_every_step(precision, tuning):
     """The graph-replayed step tail gives the eager batched results, step
     after step (each replay follows eager tree building and host syncs), with
-    two-wavefront (group_max 128) and one-wavefront (group_max 64) pair
+    two-wavefront (tile width 128) and one-wavefront (tile width 64) pair
     kernels."""
     from flecsph_amd import set_tuning
     sets, prm = [], None

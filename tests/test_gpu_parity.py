@@ -100,11 +100,17 @@ def test_tree_and_neighbors_bit_exact_past_one_workgroup_scan(oracle_mod):
     assert np.array_equal(ig, io_sorted)
 
 
+@pytest.mark.parametrize("tile_rows", [64, 128])
 @pytest.mark.parametrize("name", CASES)
-def test_neighbor_lists_bit_exact(oracle_mod, name):
+def test_neighbor_lists_bit_exact(oracle_mod, name, tile_rows):
+    """Lists read back through the tiles the pair passes walk, at both tile
+    widths (64: groups of more than 64 particles are cut into two tiles),
+    equal the oracle's at the reference's grouping (group_max 128)."""
+    from flecsph_amd import set_tuning
     parts, prm = get_case(name, _abi.PREC_FP32)
     g, o = both(oracle_mod, parts, prm)
-    print(f"case {name}", flush=True)
+    print(f"case {name} tile_rows {tile_rows}", flush=True)
+    set_tuning(g, tile_rows, 128)
     g.update_iteration()
     o.update_iteration()
     og, ig = g.neighbor_lists()
@@ -163,20 +169,25 @@ def test_sph_passes(oracle_mod, name, precision):
         assert e <= tol[f], (f, e)
 
 
+@pytest.mark.parametrize("tuning", [(64, 64), (128, 256), (64, 128)])
 @pytest.mark.parametrize("precision", [_abi.PREC_FP64, _abi.PREC_FP32])
 @pytest.mark.parametrize("name", ["random_varh", "sedov16_periodic"])
-def test_sph_passes_one_wavefront(oracle_mod, name, precision):
-    """group_max 64 selects the one-wavefront pair kernels (and chunk 64 the
-    smallest LDS rounds): same bars as the two-wavefront kernels."""
+def test_sph_passes_tile_widths(oracle_mod, name, precision, tuning):
+    """Tile width 64 selects the one-wavefront pair kernels over tiles of up
+    to 64 rows (groups of up to 128 particles cut in two), 128 the
+    two-wavefront kernels over whole groups; chunk 64/128/256 the LDS
+    rounds.  Same bars for every choice, against the oracle at the
+    reference's grouping (group_max 128)."""
     from flecsph_amd import set_tuning
     parts, prm = get_case(name, precision)
-    prm = dict(prm, group_max=64)
     g, o = both(oracle_mod, parts, prm)
-    set_tuning(g, 64, 64)
+    set_tuning(g, *tuning)
     tol = TOL[precision]
     for s in (g, o):
         s.update_iteration()
         s.density_eos()
+    if name == "random_varh":
+        assert g.tree()["group_count"].max() > 64  # some groups are cut
     for f in ("rho", "P", "c", "vsig"):
         assert rel_err(g.get(f), o.get(f)) <= tol[f], f
     for s in (g, o):
@@ -189,6 +200,31 @@ def test_sph_passes_one_wavefront(oracle_mod, name, precision):
         s.integrate(_abi.KICK_V)
         s.dudt()
     assert rel_err(g.get("dudt"), o.get("dudt")) <= tol["dudt"]
+
+
+@pytest.mark.parametrize("tile_rows", [64, 128])
+def test_polytrope_lists_and_vsig_at_reference_grouping(oracle_mod,
+                                                        tile_rows):
+    """A 50k-particle polytrope (strongly variable h, groups of 1 to 128
+    particles): lists bit-equal to the oracle's at group_max 128 for both
+    tile widths, and the signal speed, an unweighted maximum over the list
+    (so sensitive to any list difference), to FP64 rounding."""
+    from flecsph_amd import set_tuning
+    parts, prm = generators.polytrope(n=50_000)
+    prm = dict(prm, precision=_abi.PREC_FP64, gravity=_abi.GRAV_NONE)
+    g, o = both(oracle_mod, parts, prm)
+    set_tuning(g, tile_rows, 64)
+    for s in (g, o):
+        s.update_iteration()
+        s.density_eos()
+    og, ig = g.neighbor_lists()
+    oo, io = o.neighbor_lists()
+    assert np.array_equal(og, oo)
+    io_sorted = np.concatenate([np.sort(io[oo[i]:oo[i + 1]])
+                                for i in range(len(oo) - 1)])
+    assert np.array_equal(ig, io_sorted)
+    for f in ("rho", "vsig"):
+        assert rel_err(g.get(f), o.get(f)) <= 1e-12, f
 
 
 @pytest.mark.parametrize("name", ["random_varh", "sedov16_periodic",

@@ -304,6 +304,24 @@ def test_variable_h_lists_equal_pairwise_replay(oracle_mod):
     assert got == want
 
 
+def test_grouping_changes_variable_h_lists(oracle_mod):
+    """The reference's lists depend on its grouping: the node test takes the
+    group's box and its members' spheres (tree_topology.h:323-355) while a
+    reached leaf is kept by max(h_k, h_b) (:372-379).  So the GPU tests that
+    hold the tile width apart from group_max need an input where
+    group_max 64 and 128 give different lists: this is one (random_varh)."""
+    parts, prm = generators.random_box(6000, seed=101, h_spread=1.5)
+    lists = []
+    for gm in (128, 64):
+        s = make_sys(oracle_mod, parts, **prm, group_max=gm)
+        s.update_iteration()
+        off, idx = s.neighbor_lists()
+        lists.append([frozenset(idx[off[i]:off[i + 1]].tolist())
+                      for i in range(len(off) - 1)])
+    differ = sum(a != b for a, b in zip(*lists))
+    assert differ > 100, differ
+
+
 def test_variable_h_lists_sandwich(oracle_mod):
     """{d <= (h_a+h_b)/2} within the list within {d <= max(h_a,h_b)}."""
     parts, prm = generators.random_box(3000, seed=22, h_spread=2.0)
