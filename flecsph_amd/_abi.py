@@ -142,6 +142,8 @@ PROTOTYPES = {
     "timestep": (C.c_int, [_P, _PD]),
     "advance_time": (C.c_int, [_P]),
     "step": (C.c_int, [_P]),
+    "check_state": (C.c_int, [_P]),
+    "set_checks": (C.c_int, [_P, C.c_int]),
     "time": (C.c_int, [_P, _PI64, _PD, _PD]),
     "set_dt": (C.c_int, [_P, C.c_double]),
     "reductions": (C.c_int, [_P, _PD]),

@@ -85,6 +85,29 @@ int read_mail(Ctx *c, int words) {
   return 0;
 }
 
+namespace {
+__global__ void k_total_mail(const int64_t *__restrict__ total,
+                             const int *__restrict__ scan_word,
+                             int64_t *__restrict__ mail) {
+  if (threadIdx.x == 0) {
+    mail[0] = *total;
+    mail[1] = *scan_word;
+  }
+}
+}  // namespace
+
+int read_scan_total(Ctx *c, const int64_t *dev_total, int64_t *out) {
+  k_total_mail<<<1, 64, 0, c->stream>>>(dev_total,
+                                        c->flags.as<int>() + kScanWord,
+                                        c->mail.as<int64_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(read_mail(c, 2));
+  if (c->mail_h[1] & kFlagScanStall)
+    return fail(c, FSPH_E_HIP, "scan: a tile's look-back did not complete");
+  *out = c->mail_h[0];
+  return 0;
+}
+
 int read_flags(Ctx *c, int *flags) {
   FSPH_HIP(c, hipMemcpyAsync(flags, c->flags.p, sizeof(int),
                              hipMemcpyDeviceToHost, c->stream));
@@ -96,7 +119,7 @@ namespace {
 
 int do_update_iteration(Ctx *c) {
   c->have_tree = false;
-  FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, sizeof(int), c->stream));
+  FSPH_HIP(c, hipMemsetAsync(c->flags.p, 0, 16 * sizeof(int), c->stream));
   {
     PhaseTimer t(c, FSPH_T_BOUNDARY);
     FSPH_TRY(stage_boundary(c));
@@ -221,6 +244,7 @@ int do_step(Ctx *c) {
       // the rest of the step up to the dt minimum as one graph replay
       FSPH_TRY(graph_step_tail(c));
       c->batch.dt_ready = c->p.adaptive_dt != 0;
+      if (c->checks) FSPH_TRY(stage_check_state(c));
       {
         PhaseTimer t(c, FSPH_T_DT);
         FSPH_TRY(do_timestep(c));
@@ -244,6 +268,9 @@ int do_step(Ctx *c) {
     }
   }
   FSPH_TRY(do_update_h(c));  // hydro/main_driver.cc:247-256
+  // check_nans, check_negativity (hydro/main_driver.cc:263-265); the result
+  // is read once the step's work has completed
+  if (c->checks) FSPH_TRY(stage_check_state(c));
   {
     PhaseTimer t(c, FSPH_T_DT);
     FSPH_TRY(do_timestep(c));  // hydro/main_driver.cc:267-273
@@ -254,6 +281,22 @@ int do_step(Ctx *c) {
 int do_gravity(Ctx *c) { return stage_gravity(c); }
 
 }  // namespace
+
+int check_result(Ctx *c) {
+  int w[2];
+  std::memcpy(w, c->mail_h + kMailCheck, sizeof(w));
+  if (!w[0]) return 0;
+  int64_t id = -1;
+  if (w[1] >= 0 && w[1] < c->n)
+    FSPH_HIP(c, hipMemcpy(&id, c->f[PF_ID].as<int64_t>() + w[1], 8,
+                          hipMemcpyDeviceToHost));
+  if (w[0] & kFlagNaN)
+    return fail(c, FSPH_E_NAN, "NaN in the state of particle id %lld "
+                "(check_nans, default_physics.h:918-945)", (long long)id);
+  return fail(c, FSPH_E_NEG, "negative density, pressure or internal "
+              "energy of particle id %lld (check_negativity, "
+              "default_physics.h:950-979)", (long long)id);
+}
 }  // namespace fsph
 
 using fsph::Ctx;
@@ -591,7 +634,15 @@ int fsph_step(fsph_ctx *ctx) {
   CTX_OR_FAIL;
   if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
   FSPH_TRY(fsph::do_step(c));
+  c->mail_h[fsph::kMailScan] = 0;
+  FSPH_HIP(c, hipMemcpyAsync(c->mail_h + fsph::kMailScan,
+                             c->flags.as<int>() + fsph::kScanWord, 4,
+                             hipMemcpyDeviceToHost, c->stream));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->mail_h[fsph::kMailScan] & fsph::kFlagScanStall)
+    return fsph::fail(c, FSPH_E_HIP,
+                      "scan: a tile's look-back did not complete");
+  if (c->checks) FSPH_TRY(fsph::check_result(c));
   for (int i = 0; i < FSPH_NTIMERS; ++i) c->phase_ms[i] = 0;
   for (int k = 0; k < c->ev_used; ++k) {
     float ms = 0;
@@ -599,6 +650,21 @@ int fsph_step(fsph_ctx *ctx) {
       c->phase_ms[c->ev_slot[k]] += ms;
   }
   c->ev_used = 0;
+  return FSPH_OK;
+}
+
+int fsph_check_state(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
+  FSPH_TRY(fsph::stage_check_state(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return fsph::check_result(c);
+}
+
+int fsph_set_checks(fsph_ctx *ctx, int on) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  c->checks = on != 0;
   return FSPH_OK;
 }
 

@@ -57,6 +57,17 @@ enum : int {
   kFlagScanStall = 32,
 };
 
+// Words of ctx->flags (an int array of 16): [0] the error bits above;
+// [kCheckWord], [kCheckWord + 1]: the state check's bits and lowest index;
+// [kScanWord]: kFlagScanStall of any scan since the last update_iteration
+// (sticky: every read-back that depends on a scan checks it).
+constexpr int kCheckWord = 4;
+constexpr int kScanWord = 6;
+// mail_h slot that receives the state check's two words, and the one that
+// receives the scan word at the end of a step / with the group count
+constexpr int kMailCheck = kMail - 3;
+constexpr int kMailScan = kMail - 2;
+
 struct Ctx;
 
 // graph argument slots (bytes each) and their order
@@ -90,6 +101,7 @@ struct Ctx {
   size_t rsize = 4;  // sizeof(real) for derived fields
 
   bool have_rhoP = false;  // the last upload gave rho and P
+  bool checks = true;      // fsph_step checks the state for NaN/negativity
   int64_t n = 0;       // particles held (real + WALL copies)
   int64_t n_real = 0;
   int64_t pcap = 0;    // particle capacity of every per-particle buffer
@@ -327,10 +339,18 @@ int stage_update_h(Ctx *c);            // k_step.hip
 // eos::init (eos.h:89-92): K = P / rho^gamma per particle, or the parameter
 // poly_K where rho and P were not uploaded (k_step.hip)
 int stage_eos_init(Ctx *c);
+// check_nans + check_negativity (default_physics.h:918-979): enqueues the
+// check and the copy of its result to mail_h[kMailCheck]; check_result
+// (fsph_api.hip) reads it after the stream has been synchronised
+int stage_check_state(Ctx *c);
+int check_result(Ctx *c);
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
 // device pointer to 2.5 max h (the periodic halo), no host wait
 int stage_halo(Ctx *c, const double **halo);  // k_step.hip
 int read_mail(Ctx *c, int words);             // fsph_api.hip
+// Reads the int64 *dev_total (a scan's total) to *out, failing with
+// FSPH_E_HIP if a scan has stalled (fsph_api.hip).
+int read_scan_total(Ctx *c, const int64_t *dev_total, int64_t *out);
 // device clock of a single simulation (k_step.hip)
 int put_time(Ctx *c);      // tstate := host dt, totaltime, totaltime_next
 int sync_time(Ctx *c);     // host copies := tstate (waits) if time_dirty

@@ -428,9 +428,12 @@ __global__ __launch_bounds__(kT) void k_count_sinks(const int2 *__restrict__ l,
   const SinkRun r = sink_run(sp.valid, sp.sink);
   if (sp.valid && r.len) atomicAdd(&cnt[r.sink], r.len);
 }
+// (does nothing after a stalled scan: the cursors may be incomplete; the
+// stall is reported when the step completes)
 __global__ __launch_bounds__(kT) void k_scatter_sinks(
-    const int2 *__restrict__ l, int64_t n,
+    const int2 *__restrict__ l, int64_t n, const int *__restrict__ scan_word,
     unsigned long long *__restrict__ cursor, int32_t *__restrict__ src) {
+  if (*scan_word & kFlagScanStall) return;
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   const bool valid = i < n;
   const int2 pr = valid ? l[i] : make_int2(-1, 0);
@@ -745,7 +748,8 @@ int bucket_sort(Ctx *c, const int2 *list, int64_t n, int64_t nc, DBuf &off,
                              hipMemcpyDeviceToDevice, c->stream));
   if (n)
     k_scatter_sinks<<<blocks_for(n), kT, 0, c->stream>>>(
-        list, n, G.cursor.as<unsigned long long>(), src.as<int32_t>());
+        list, n, c->flags.as<int>() + kScanWord,
+        G.cursor.as<unsigned long long>(), src.as<int32_t>());
   return hip_check(c, hipGetLastError(), "k_scatter_sinks");
 }
 
@@ -766,9 +770,7 @@ template <class T> int gravity_impl(Ctx *c) {
   FSPH_HIP(c, hipGetLastError());
   FSPH_TRY(scan_i32_to_i64(c, G.flag.as<int32_t>(), G.pos.as<int64_t>(), tot));
   int64_t nc = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&nc, G.pos.as<int64_t>() + tot, 8,
-                             hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  FSPH_TRY(read_scan_total(c, G.pos.as<int64_t>() + tot, &nc));
   G.nc = nc;
   if (nc > G.cap_cells) {
     const int64_t cap = nc + nc / 4 + 1024;

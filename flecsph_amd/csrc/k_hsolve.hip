@@ -622,8 +622,7 @@ template <class T> int learned_impl(Ctx *c) {
       FSPH_TRY(scan_i32_to_i64(c, c->hgflag.as<int32_t>(),
                                c->hgscan.as<int64_t>(), c->nt));
       int64_t ngl = 0;
-      FSPH_HIP(c, hipMemcpyAsync(&ngl, c->hgscan.as<int64_t>() + c->nt, 8,
-                                 hipMemcpyDeviceToHost, c->stream));
+      FSPH_TRY(read_scan_total(c, c->hgscan.as<int64_t>() + c->nt, &ngl));
       k_group_list<<<blocks256(c->nt), 256, 0, c->stream>>>(
           c->hgflag.as<int32_t>(), c->hgscan.as<int64_t>(), c->nt,
           c->glist.as<int32_t>());

@@ -59,10 +59,13 @@ __global__ __launch_bounds__(kT) void k_keep_real(
   if (i < n) keep[i] = id[i] < (inst ? nreal[inst[i]] : n_real);
 }
 
+// (does nothing after a stalled scan: off may be incomplete; the stall is
+// reported at the step's next read-back)
 __global__ __launch_bounds__(kT) void k_compact_index(
     const int32_t *__restrict__ keep, const int64_t *__restrict__ off,
-    int64_t n, uint32_t *__restrict__ perm) {
+    int64_t n, const int *__restrict__ scan_word, uint32_t *__restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (*scan_word & kFlagScanStall) return;
   if (i < n && keep[i]) perm[off[i]] = (uint32_t)i;
 }
 
@@ -565,7 +568,8 @@ int stage_boundary(Ctx *c) {
         c->batch.nreal.as<int64_t>(), keep);
     FSPH_TRY(scan_i32_to_i64(c, keep, off, c->n));
     k_compact_index<<<blocks_for(c->n), kT, 0, c->stream>>>(
-        keep, off, c->n, c->perm.as<uint32_t>());
+        keep, off, c->n, c->flags.as<int>() + kScanWord,
+        c->perm.as<uint32_t>());
     FSPH_HIP(c, hipGetLastError());
     FSPH_TRY(permute_particles(c, c->perm.as<uint32_t>(), c->n_real));
     c->n = c->n_real;
@@ -607,9 +611,7 @@ int stage_boundary(Ctx *c) {
   int64_t *off = c->pair_off.as<int64_t>();
   FSPH_TRY(scan_i32_to_i64(c, cnt, off, n));
   int64_t extra = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&extra, off + n, sizeof(int64_t),
-                             hipMemcpyDeviceToHost, c->stream));
-  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  FSPH_TRY(read_scan_total(c, off + n, &extra));
   if (n + extra > c->pcap) {
     // keep the offsets: they live in pair_off, which ensure_particles
     // does not touch.

@@ -249,7 +249,7 @@ __global__ void k_tile_mail(const int *__restrict__ flags,
                             const int64_t *__restrict__ words,
                             int64_t *__restrict__ mail) {
   if (threadIdx.x == 0) {
-    mail[0] = *flags;
+    mail[0] = flags[0] | flags[kScanWord];
     mail[1] = *words;
   }
 }

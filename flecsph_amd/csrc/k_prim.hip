@@ -211,7 +211,7 @@ int scan_impl(Ctx *c, const In *in, int64_t *out, int64_t n) {
   unsigned *ctr = reinterpret_cast<unsigned *>(st + 2 * c->scan_cap);
   const ScanState S{st + p * c->scan_cap, st + (1 - p) * c->scan_cap,
                     c->scan_cap, ctr + 2 * p, ctr + 2 * (1 - p),
-                    c->flags.as<int>()};
+                    c->flags.as<int>() + kScanWord};
   k_scan_one<In><<<(unsigned)nt, kScanThreads, 0, c->stream>>>(in, n, out, S);
   return hip_check(c, hipGetLastError(), "scan");
 }

@@ -511,6 +511,62 @@ int stage_update_h(Ctx *c) {
   return hip_check(c, hipGetLastError(), "k_update_h");
 }
 
+// check_nans and check_negativity (default_physics.h:918-979) over every
+// particle: a NaN in any field the reference checks sets kFlagNaN, a
+// negative density, pressure or internal energy kFlagNeg, and the lowest
+// offending index is kept (out[0] flags, out[1] index).
+template <class T> struct CheckArgs {
+  const double *d[12];  // x y z vx vy vz vhx vhy vhz m u, adiabat (or null)
+  const T *r[11];       // ax ay az gx gy gz phi rho P dudt vsig
+  int *out;
+};
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_check_state(CheckArgs<T> a,
+                                                    int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  bool nan = false;
+#pragma unroll
+  for (int q = 0; q < 11; ++q) nan |= a.d[q][i] != a.d[q][i];
+  if (a.d[11]) nan |= a.d[11][i] != a.d[11][i];
+#pragma unroll
+  for (int q = 0; q < 11; ++q) nan |= a.r[q][i] != a.r[q][i];
+  const bool neg = a.r[7][i] < T(0) || a.r[8][i] < T(0) || a.d[10][i] < 0.0;
+  if (nan || neg) {
+    atomicOr(&a.out[0], (nan ? kFlagNaN : 0) | (neg ? kFlagNeg : 0));
+    atomicMin(&a.out[1], (int)i);
+  }
+}
+
+template <class T> int check_impl(Ctx *c, int *out) {
+  CheckArgs<T> a{};
+  const int df[11] = {PF_X,   PF_Y,   PF_Z,   PF_VX, PF_VY, PF_VZ,
+                      PF_VHX, PF_VHY, PF_VHZ, PF_M,  PF_U};
+  for (int q = 0; q < 11; ++q) a.d[q] = c->f[df[q]].as<double>();
+  a.d[11] = c->p.eos == FSPH_EOS_POLYTROPIC ? c->f[PF_KAD].as<double>()
+                                            : nullptr;
+  const int rf[11] = {PF_AX,  PF_AY,  PF_AZ, PF_GX,   PF_GY,  PF_GZ,
+                      PF_PHI, PF_RHO, PF_P,  PF_DUDT, PF_VSIG};
+  for (int q = 0; q < 11; ++q) a.r[q] = c->f[rf[q]].as<T>();
+  a.out = out;
+  k_check_state<T><<<blocks_for(c->n), kT, 0, c->stream>>>(a, c->n);
+  return hip_check(c, hipGetLastError(), "k_check_state");
+}
+
+int stage_check_state(Ctx *c) {
+  if (c->n <= 0) return 0;
+  int *out = c->flags.as<int>() + kCheckWord;
+  FSPH_HIP(c, hipMemsetAsync(out, 0, 4, c->stream));
+  FSPH_HIP(c, hipMemsetAsync(out + 1, 0x7f, 4, c->stream));
+  FSPH_TRY(c->rsize == 4 ? check_impl<float>(c, out)
+                         : check_impl<double>(c, out));
+  // into the mailbox's host copy; read after the caller's synchronisation
+  FSPH_HIP(c, hipMemcpyAsync(c->mail_h + kMailCheck, out, 8,
+                             hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
 // eos::init (eos.h:76-92): the adiabat K = P / rho^gamma of each particle
 // from its current density and pressure; with no density uploaded, the
 // parameter poly_K.

@@ -165,7 +165,7 @@ __global__ void k_tree_mail(const int64_t *__restrict__ nn,
                             int64_t *__restrict__ mail) {
   if (threadIdx.x != 0) return;
   mail[0] = *nn;
-  mail[1] = *flags;
+  mail[1] = flags[0] | flags[kScanWord];
   unsigned long long acc = 0;
   for (int d = 0; d <= kMaxDepth; ++d) {
     mail[2 + d] = (int64_t)lvl[d];
@@ -544,6 +544,10 @@ int group_flags(Ctx *c) {
   FSPH_HIP(c, hipMemcpyAsync(c->mail_h + (kMail - 1),
                              c->gscan.as<int64_t>() + tot, 8,
                              hipMemcpyDeviceToHost, c->stream));
+  c->mail_h[kMailScan] = 0;
+  FSPH_HIP(c, hipMemcpyAsync(c->mail_h + kMailScan,
+                             c->flags.as<int>() + kScanWord, 4,
+                             hipMemcpyDeviceToHost, c->stream));
   FSPH_HIP(c, hipEventRecord(c->ev_groups, c->stream));
   return 0;
 }
@@ -552,6 +556,8 @@ int stage_groups(Ctx *c) {
   const int64_t n = c->n, nn = c->nn;
   const int64_t tot = nn + n;
   FSPH_HIP(c, hipEventSynchronize(c->ev_groups));
+  if (c->mail_h[kMailScan] & kFlagScanStall)
+    return fail(c, FSPH_E_HIP, "scan: a tile's look-back did not complete");
   const int64_t packed = c->mail_h[kMail - 1];
   const int64_t ng = packed & 0xffffffffll, nsplit = packed >> 32;
   c->ng = ng;

@@ -142,6 +142,13 @@ class _System:
     def step(self):
         self._call("step")
 
+    def check_state(self):
+        """check_nans + check_negativity (default_physics.h:918-979)."""
+        self._call("check_state")
+
+    def set_checks(self, on=True):
+        self._call("set_checks", int(bool(on)))
+
     def time(self):
         it, dt, tt = C.c_int64(), C.c_double(), C.c_double()
         self._call("time", C.byref(it), C.byref(dt), C.byref(tt))

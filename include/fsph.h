@@ -229,6 +229,16 @@ int fsph_advance_time(fsph_ctx *ctx);
    iteration (no integration). */
 int fsph_step(fsph_ctx *ctx);
 
+/* apply_all(physics::check_nans) and apply_all(physics::check_negativity)
+   (default_physics.h:918-979) over every particle: FSPH_E_NAN if a field
+   the reference checks holds a NaN, FSPH_E_NEG for a negative density,
+   pressure or internal energy (the message names the particle's id).
+   fsph_step runs the same check after the smoothing-length update, as the
+   driver does (hydro/main_driver.cc:263-265), and returns its code. */
+int fsph_check_state(fsph_ctx *ctx);
+/* Turns fsph_step's state check off or on (default on). */
+int fsph_set_checks(fsph_ctx *ctx, int on);
+
 /* Current time-step state: iteration, dt, totaltime
    (physics::iteration/dt/totaltime, default_physics.h:31-40). */
 int fsph_time(const fsph_ctx *ctx, int64_t *iteration, double *dt,

@@ -1205,6 +1205,25 @@ int fo_time(const fo_sys *s, int64_t *it, double *dt, double *tt) {
   return 0;
 }
 
+/* check_nans + check_negativity (default_physics.h:918-979) over every
+   particle: FSPH_E_NAN before FSPH_E_NEG. */
+int fo_check_state(fo_sys *s) {
+  int nan = 0, neg = 0;
+  double **f[NDBL];
+  dbl_fields(s, f);
+  for (int64_t i = 0; i < s->n; ++i) {
+    for (int k = 0; k < NDBL; ++k) {
+      double *a = *f[k];
+      if (a == s->c || a == s->h ||
+          (a == s->kad && s->p.eos != FSPH_EOS_POLYTROPIC))
+        continue; /* the reference checks neither c nor h */
+      if (a[i] != a[i]) nan = 1;
+    }
+    if (s->rho[i] < 0 || s->P[i] < 0 || s->u[i] < 0) neg = 1;
+  }
+  return nan ? FSPH_E_NAN : (neg ? FSPH_E_NEG : 0);
+}
+
 int fo_set_dt(fo_sys *s, double dt) {
   s->dt = dt;
   return 0;

@@ -37,6 +37,7 @@ int fo_update_h(fo_sys *s);
 int fo_timestep(fo_sys *s, double *dt);
 int fo_advance_time(fo_sys *s);
 int fo_step(fo_sys *s);
+int fo_check_state(fo_sys *s);
 int fo_time(const fo_sys *s, int64_t *iteration, double *dt, double *totaltime);
 int fo_set_dt(fo_sys *s, double dt);
 int fo_reductions(fo_sys *s, double out[10]);
