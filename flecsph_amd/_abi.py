@@ -40,15 +40,20 @@ KICK_V = 1
 KICK_U = 2
 SAVE_VHALF = 4
 DRIFT = 8
+KICK_E = 16
+
+VISC_CONSTANT = 0
+VISC_CULLEN = 1
 
 FIELDS = ["x", "y", "z", "vx", "vy", "vz", "vhx", "vhy", "vhz", "u", "h", "m",
           "rho", "P", "c", "vsig", "ax", "ay", "az", "dudt", "gx", "gy", "gz",
-          "phi", "id", "key", "nngb", "hngb", "hiter"]
+          "phi", "id", "key", "nngb", "hngb", "hiter", "e", "dedt", "alpha",
+          "divv"]
 FIELD_INDEX = {name: i for i, name in enumerate(FIELDS)}
 
 TIMERS = ["integrate", "boundary", "range_keys", "sort", "tree", "traverse",
           "masks", "density", "vsig", "accel", "gravity", "dudt", "h", "dt",
-          "graph"]
+          "graph", "grav_pairs", "m2l", "p2p"]
 
 
 class FsphParams(C.Structure):
@@ -66,6 +71,10 @@ class FsphParams(C.Structure):
         ("h_mode", C.c_int32), ("newton_maxit", C.c_int32),
         ("newton_tol", C.c_double), ("h_search", C.c_double),
         ("learn_steps", C.c_int32), ("n_tol", C.c_int32),
+        ("thermokinetic", C.c_int32), ("viscosity", C.c_int32),
+        ("visc_alpha_max", C.c_double), ("visc_l", C.c_double),
+        ("visc_delta", C.c_double),
+        ("evolve_internal_energy", C.c_int32),
     ]
 
 
@@ -106,6 +115,12 @@ def default_params(**kw):
     p.h_search = 1.3
     p.learn_steps = 50
     p.n_tol = 5
+    p.thermokinetic = 1
+    p.viscosity = VISC_CONSTANT
+    p.visc_alpha_max = 1.0
+    p.visc_l = 0.05
+    p.visc_delta = 1.0
+    p.evolve_internal_energy = 1
     for k in range(3):
         p.periodic[k] = 0
         p.box[k] = 1.0
@@ -147,6 +162,17 @@ PROTOTYPES = {
     "time": (C.c_int, [_P, _PI64, _PD, _PD]),
     "set_dt": (C.c_int, [_P, C.c_double]),
     "reductions": (C.c_int, [_P, _PD]),
+    "dedt": (C.c_int, [_P]),
+    "h_gather": (C.c_int, [_P, _PD, C.POINTER(C.c_int64)]),
+    "instance_set_periodic": (C.c_int, [_P, C.c_int, C.c_int]),
+    "write_bodies": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "read_bodies": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "compute_alpha": (C.c_int, [_P]),
+    "initialize_alpha": (C.c_int, [_P]),
+    "set_field": (C.c_int, [_P, C.c_int, _PD]),
+    "set_total_energy": (C.c_int, [_P]),
+    "gravity_stats": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "set_fmm_capacity": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int64]),
     "get_range": (C.c_int, [_P, _PD, _PD]),
     "get_tree": (C.c_int, [_P, _PI64, _PU64, _PI32, _PI32, _PI32, _PI64,
                            _PI32, _PI32]),

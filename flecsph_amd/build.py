@@ -51,18 +51,22 @@ def build(verbose=False):
 
 
 DRIVER = os.path.join(HERE, "fsph_hydro_driver")
+FUNCTOR_CHECK = os.path.join(HERE, "fsph_functor_check")
 
 
 def build_driver():
-    """The C++ hydro driver over host/body_system.hpp, linked to
-    libfsph.so (host compiler only; the device code is in the library)."""
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(HERE, "host"),
-           os.path.join(HERE, "host", "hydro_driver.cpp"), "-L", HERE,
-           "-lfsph", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed:\n{r.stderr[-4000:]}")
+    """The C++ programs over host/body_system.hpp (the hydro driver loop
+    and the host-functor example), linked to libfsph.so (host compiler
+    only; the device code is in the library)."""
+    for src, out in (("hydro_driver.cpp", DRIVER),
+                     ("functor_check.cpp", FUNCTOR_CHECK)):
+        cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+               os.path.join(ROOT, "include"), "-I",
+               os.path.join(HERE, "host"), os.path.join(HERE, "host", src),
+               "-L", HERE, "-lfsph", "-Wl,-rpath,$ORIGIN", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{src} build failed:\n{r.stderr[-4000:]}")
     return DRIVER
 
 

@@ -60,8 +60,20 @@ int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep) {
   return 0;
 }
 
+// fsph_field -> PField (-1 for the keys, which are not a particle field)
+int download_map(int field) {
+  static const int map[FSPH_NFIELDS] = {
+      PF_X,    PF_Y,    PF_Z,    PF_VX,   PF_VY,    PF_VZ,   PF_VHX,
+      PF_VHY,  PF_VHZ,  PF_U,    PF_H,    PF_M,     PF_RHO,  PF_P,
+      PF_C,    PF_VSIG, PF_AX,   PF_AY,   PF_AZ,    PF_DUDT, PF_GX,
+      PF_GY,   PF_GZ,   PF_PHI,  PF_ID,   -1,       PF_NNGB, PF_HNGB,
+      PF_HITER, PF_E,   PF_DEDT, PF_ALPHA, PF_DIVV};
+  static_assert(sizeof(map) / sizeof(map[0]) == FSPH_NFIELDS, "field map");
+  return map[field];
+}
+
 size_t field_size(const Ctx *c, int f) {
-  if (f < PF_NDBL || f == PF_ID || f == PF_KAD) return 8;
+  if (f < PF_NDBL || f == PF_ID || f == PF_KAD || f == PF_E) return 8;
   if (f == PF_NNGB || f == PF_HNGB || f == PF_HITER || f == PF_INST) return 4;
   return c->rsize;
 }
@@ -197,6 +209,8 @@ int do_gravity(Ctx *c);
 // The end-of-step smoothing-length update: closed form (the reference's
 // get_all(compute_smoothinglength)) or Newton-h over fresh gather tiles.
 int do_update_h(Ctx *c) {
+  c->h_gather_words = 0;
+  c->h_kappa = 0;
   if (c->p.h_mode == FSPH_H_NEWTON) {
     FSPH_TRY(need_tree(c));
     return stage_newton_h(c, nullptr);
@@ -215,11 +229,16 @@ int do_step(Ctx *c) {
     // hydro/main_driver.cc:102-173 (iteration 0: no integration)
     FSPH_TRY(do_update_iteration(c));
     FSPH_TRY(stage_eos_init(c));  // hydro/main_driver.cc:106
+    // hydro/main_driver.cc:108-115
+    if (c->p.thermokinetic) FSPH_TRY(stage_set_total_energy(c));
+    const bool cullen = c->p.viscosity == FSPH_VISC_CULLEN;
+    if (cullen) FSPH_TRY(stage_initialize_alpha(c));
     FSPH_TRY(stage_density_only(c));
     {
       PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(c, FSPH_SAVE_VHALF));
     }
+    if (cullen) FSPH_TRY(stage_compute_alpha(c));  // main_driver.cc:119-122
     FSPH_TRY(stage_acceleration_vsig(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
@@ -229,14 +248,21 @@ int do_step(Ctx *c) {
       PhaseTimer t(c, FSPH_T_DT);
       FSPH_TRY(do_timestep(c));
     }
-    FSPH_TRY(stage_dudt(c));
+    if (c->p.evolve_internal_energy)  // hydro/main_driver.cc:145-172
+      FSPH_TRY(c->p.thermokinetic ? stage_dedt(c) : stage_dudt(c));
     c->initial_done = true;
   } else {
-    // hydro/main_driver.cc:174-245
+    // hydro/main_driver.cc:174-245; the thermokinetic formulation kicks e
+    // where the u formulation kicks u (hydro/main_driver.cc:176-180,
+    // 216-229), neither without evolve_internal_energy
+    const uint32_t kick_energy =
+        !c->p.evolve_internal_energy ? 0u
+        : c->p.thermokinetic        ? FSPH_KICK_E
+                                    : FSPH_KICK_U;
     {
       PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(
-          c, FSPH_KICK_U | FSPH_KICK_V | FSPH_SAVE_VHALF | FSPH_DRIFT));
+          c, kick_energy | FSPH_KICK_V | FSPH_SAVE_VHALF | FSPH_DRIFT));
     }
     FSPH_TRY(do_update_iteration(c));
     if (c->graph.on && c->batch.n_inst && c->p.h_mode == FSPH_H_CLOSED &&
@@ -252,6 +278,8 @@ int do_step(Ctx *c) {
       return do_advance(c);
     }
     FSPH_TRY(stage_density_only(c));
+    // main_driver.cc:189-192
+    if (c->p.viscosity == FSPH_VISC_CULLEN) FSPH_TRY(stage_compute_alpha(c));
     FSPH_TRY(stage_acceleration_vsig(c));
     if (c->p.gravity) {
       PhaseTimer t(c, FSPH_T_GRAVITY);
@@ -261,10 +289,10 @@ int do_step(Ctx *c) {
       PhaseTimer t(c, FSPH_T_INTEGRATE);
       FSPH_TRY(stage_integrate(c, FSPH_KICK_V));
     }
-    FSPH_TRY(stage_dudt(c));
-    {
+    if (c->p.evolve_internal_energy) {
+      FSPH_TRY(c->p.thermokinetic ? stage_dedt(c) : stage_dudt(c));
       PhaseTimer t(c, FSPH_T_INTEGRATE);
-      FSPH_TRY(stage_integrate(c, FSPH_KICK_U));
+      FSPH_TRY(stage_integrate(c, kick_energy));
     }
   }
   FSPH_TRY(do_update_h(c));  // hydro/main_driver.cc:247-256
@@ -331,6 +359,12 @@ void fsph_default_params(fsph_params *p) {
   p->h_search = 1.3;
   p->learn_steps = 50;
   p->n_tol = 5;
+  p->thermokinetic = 1;  // params.h:590-592
+  p->viscosity = FSPH_VISC_CONSTANT;
+  p->visc_alpha_max = 1.0;  // params.h:515-530
+  p->visc_l = 0.05;
+  p->visc_delta = 1.0;
+  p->evolve_internal_energy = 1;  // params.h:583-585
 }
 
 int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
@@ -395,7 +429,7 @@ void fsph_destroy(fsph_ctx *ctx) {
   fsph::DBuf *more[] = {&c->graph.counters,
                         &c->batch.nreal, &c->batch.segr, &c->batch.seg,
                         &c->batch.bounds, &c->batch.halo, &c->batch.root,
-                        &c->batch.dt};
+                        &c->batch.dt, &c->batch.open};
   for (auto *b : more)
     if (b->p) hipFree(b->p);
   for (int f = 0; f < fsph::PF_COUNT; ++f) {
@@ -481,8 +515,9 @@ int fsph::upload_particles(Ctx *c, const fsph_particles *in) {
   const int zero_f[] = {fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY, fsph::PF_AZ,
                         fsph::PF_DUDT, fsph::PF_GX, fsph::PF_GY, fsph::PF_GZ,
                         fsph::PF_PHI, fsph::PF_COLD, fsph::PF_NNGB,
-                        fsph::PF_HNGB, fsph::PF_HITER, fsph::PF_DIVV,
-                        fsph::PF_DLH};
+                        fsph::PF_HNGB, fsph::PF_HITER, fsph::PF_LDIV,
+                        fsph::PF_DLH, fsph::PF_E, fsph::PF_DEDT,
+                        fsph::PF_DIVV, fsph::PF_DDIVV};
   for (int f : zero_f)
     FSPH_HIP(c, hipMemsetAsync(c->f[f].p, 0, (size_t)n * fsph::field_size(c, f),
                                c->stream));
@@ -491,6 +526,7 @@ int fsph::upload_particles(Ctx *c, const fsph_particles *in) {
   c->n = c->n_real = n;
   c->have_tree = false;
   FSPH_TRY(fsph::stage_eos_init(c));  // K := poly_K until eos::init runs
+  FSPH_TRY(fsph::stage_initialize_alpha(c));
   c->iteration = 0;
   c->dt = c->p.initial_dt;
   c->totaltime = 0;
@@ -516,13 +552,9 @@ int fsph_download(fsph_ctx *ctx, int field, void *dst) {
     return fsph::fail(c, FSPH_E_ARG, "bad field %d", field);
   const int64_t n = c->n;
   if (n == 0) return FSPH_OK;
-  static const int map[FSPH_NFIELDS] = {
-      fsph::PF_X,   fsph::PF_Y,   fsph::PF_Z,    fsph::PF_VX,   fsph::PF_VY,
-      fsph::PF_VZ,  fsph::PF_VHX, fsph::PF_VHY,  fsph::PF_VHZ,  fsph::PF_U,
-      fsph::PF_H,   fsph::PF_M,   fsph::PF_RHO,  fsph::PF_P,    fsph::PF_C,
-      fsph::PF_VSIG, fsph::PF_AX, fsph::PF_AY,   fsph::PF_AZ,   fsph::PF_DUDT,
-      fsph::PF_GX,  fsph::PF_GY,  fsph::PF_GZ,   fsph::PF_PHI,  fsph::PF_ID,
-      -1 /* key */, fsph::PF_NNGB, fsph::PF_HNGB, fsph::PF_HITER};
+  struct {
+    int operator[](int f) const { return fsph::download_map(f); }
+  } map;
   if (field == FSPH_F_KEY) {
     if (!c->key.p) return fsph::fail(c, FSPH_E_ARG, "no keys yet");
     FSPH_HIP(c, hipMemcpyAsync(dst, c->key.p, (size_t)n * 8,
@@ -593,6 +625,18 @@ int fsph_gravity(fsph_ctx *ctx) {
   return FSPH_OK;
 }
 
+int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[6]) {
+  CTX_OR_FAIL;
+  if (!out) return FSPH_E_ARG;
+  return fsph::gravity_stats(c, out);
+}
+
+int fsph_set_fmm_capacity(fsph_ctx *ctx, int64_t cap_q, int64_t cap_m2l,
+                          int64_t cap_p2p) {
+  CTX_OR_FAIL;
+  return fsph::set_fmm_capacity(c, cap_q, cap_m2l, cap_p2p);
+}
+
 int fsph_dudt(fsph_ctx *ctx) {
   CTX_OR_FAIL;
   FSPH_TRY(fsph::need_tree(c));
@@ -601,9 +645,77 @@ int fsph_dudt(fsph_ctx *ctx) {
   return FSPH_OK;
 }
 
+int fsph_set_field(fsph_ctx *ctx, int field, const double *src) {
+  CTX_OR_FAIL;
+  if (!src || field < 0 || field >= FSPH_NFIELDS || field == FSPH_F_ID ||
+      field == FSPH_F_KEY || field == FSPH_F_NNGB || field == FSPH_F_HNGB ||
+      field == FSPH_F_HITER)
+    return fsph::fail(c, FSPH_E_ARG, "field %d cannot be set", field);
+  const int64_t n = c->n;
+  if (n == 0) return FSPH_OK;
+  const int f = fsph::download_map(field);
+  const size_t es = fsph::field_size(c, f);
+  if (es == 8) {
+    FSPH_HIP(c, hipMemcpyAsync(c->f[f].p, src, (size_t)n * 8,
+                               hipMemcpyHostToDevice, c->stream));
+  } else {
+    FSPH_TRY(fsph::ensure(c, c->red_tmp, (size_t)n * 8));
+    FSPH_HIP(c, hipMemcpyAsync(c->red_tmp.p, src, (size_t)n * 8,
+                               hipMemcpyHostToDevice, c->stream));
+    FSPH_TRY(fsph::convert_from_double(c, c->red_tmp.as<double>(),
+                                       c->f[f].as<float>(), n));
+  }
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  // positions and smoothing lengths decide the tree and the tiles
+  if (field <= FSPH_F_Z || field == FSPH_F_H) c->have_tree = false;
+  return FSPH_OK;
+}
+
+int fsph_dedt(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (!c->p.thermokinetic)
+    return fsph::fail(c, FSPH_E_ARG,
+                      "de/dt needs the thermokinetic formulation");
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::stage_dedt(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_compute_alpha(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (c->p.viscosity != FSPH_VISC_CULLEN)
+    return fsph::fail(c, FSPH_E_ARG,
+                      "compute_alpha needs the Cullen-Dehnen viscosity");
+  FSPH_TRY(fsph::need_tree(c));
+  FSPH_TRY(fsph::stage_compute_alpha(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_initialize_alpha(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  FSPH_TRY(fsph::stage_initialize_alpha(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+int fsph_set_total_energy(fsph_ctx *ctx) {
+  CTX_OR_FAIL;
+  if (!c->p.thermokinetic)
+    return fsph::fail(c, FSPH_E_ARG,
+                      "the total energy needs the thermokinetic formulation");
+  FSPH_TRY(fsph::stage_set_total_energy(c));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
 int fsph_integrate(fsph_ctx *ctx, uint32_t what) {
   CTX_OR_FAIL;
   if (c->n < 2) return fsph::fail(c, FSPH_E_ARG, "no particles uploaded");
+  if ((what & FSPH_KICK_E) && !c->p.thermokinetic)
+    return fsph::fail(c, FSPH_E_ARG,
+                      "kick_e needs the thermokinetic formulation");
   FSPH_TRY(fsph::stage_integrate(c, what));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
   return FSPH_OK;
@@ -820,9 +932,12 @@ int fsph_enable_graph(fsph_ctx *ctx, int on) {
   CTX_OR_FAIL;
   if (on && !c->batch.n_inst)
     return fsph::fail(c, FSPH_E_ARG, "graph mode needs a batched context");
-  if (on && (c->p.h_mode != FSPH_H_CLOSED || c->p.gravity))
+  if (on && (c->p.h_mode != FSPH_H_CLOSED || c->p.gravity ||
+             c->p.viscosity != FSPH_VISC_CONSTANT ||
+             !c->p.evolve_internal_energy))
     return fsph::fail(c, FSPH_E_ARG,
-                      "graph mode needs the closed-form h and no gravity");
+                      "graph mode needs the closed-form h, no gravity, "
+                      "the constant viscosity and an evolved energy");
   c->graph.on = on != 0;
   return FSPH_OK;
 }
@@ -955,6 +1070,14 @@ int fsph_h_stats(fsph_ctx *ctx, int64_t *fallback, int64_t *clamped,
   if (clamped) *clamped = c->h_clamped;
   if (trained_steps) *trained_steps = c->mlp.trained_steps;
   if (last_loss) *last_loss = c->mlp.last_loss;
+  return FSPH_OK;
+}
+
+int fsph_h_gather(fsph_ctx *ctx, double *kappa, int64_t *mask_words) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return FSPH_E_ARG;
+  if (kappa) *kappa = c->h_kappa;
+  if (mask_words) *mask_words = c->h_gather_words;
   return FSPH_OK;
 }
 

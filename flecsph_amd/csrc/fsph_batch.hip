@@ -37,6 +37,10 @@ int batch_alloc(Ctx *c, int ni) {
   FSPH_TRY(fsph::ensure(c, B.halo, (size_t)ni * 8));
   FSPH_TRY(fsph::ensure(c, B.root, (size_t)ni * 4));
   FSPH_TRY(fsph::ensure(c, B.dt, (size_t)ni * 8));
+  FSPH_TRY(fsph::ensure(c, B.open, (size_t)ni * 4));
+  B.open_h.assign(ni, 0);
+  FSPH_HIP(c, hipMemcpy(B.open.p, B.open_h.data(), (size_t)ni * 4,
+                        hipMemcpyHostToDevice));
   B.n_real.assign(ni, 0);
   B.dt_h.assign(ni, c->p.initial_dt);
   B.t_h.assign(ni, 0.0);
@@ -257,6 +261,18 @@ int fsph_instance_download(fsph_ctx *ctx, int inst, int field, void *dst) {
   FSPH_TRY(fsph_download(ctx, field, all.data()));
   const int64_t a = c->batch.seg_h[inst], b = c->batch.seg_h[inst + 1];
   std::memcpy(dst, all.data() + a, (size_t)(b - a) * 8);
+  return FSPH_OK;
+}
+
+int fsph_instance_set_periodic(fsph_ctx *ctx, int inst, int periodic) {
+  BATCH_OR_FAIL;
+  auto &B = c->batch;
+  if (inst < 0 || inst >= B.n_inst)
+    return fsph::fail(c, FSPH_E_ARG, "bad instance %d", inst);
+  B.open_h[inst] = periodic ? 0 : 1;
+  FSPH_HIP(c, hipMemcpyAsync(B.open.as<int32_t>() + inst, &B.open_h[inst], 4,
+                             hipMemcpyHostToDevice, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
   return FSPH_OK;
 }
 

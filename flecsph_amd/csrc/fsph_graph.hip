@@ -145,6 +145,7 @@ int graph_capture(Ctx *c, const NodeSpec *sp, int ns) {
   G.key[1] = (int)c->rsize;
   G.key[2] = c->p.adaptive_dt;
   G.key[3] = c->batch.n_inst;
+  G.key[4] = c->p.thermokinetic;
   return 0;
 }
 
@@ -164,10 +165,12 @@ int graph_step_tail(Ctx *c) {
   FSPH_TRY(graph_setup(c));
   NodeSpec sp[Ctx::Graph::kNodes];
   const int ns = step_specs(c, sp);
-  const int key[4] = {c->chunk * 2 + (int)(sph_graph_block(c) == 64),
-                      (int)c->rsize, c->p.adaptive_dt, c->batch.n_inst};
-  if (!G.exec || key[0] != G.key[0] || key[1] != G.key[1] ||
-      key[2] != G.key[2] || key[3] != G.key[3])
+  const int key[5] = {c->chunk * 2 + (int)(sph_graph_block(c) == 64),
+                      (int)c->rsize, c->p.adaptive_dt, c->batch.n_inst,
+                      c->p.thermokinetic};
+  bool same = G.exec != nullptr;
+  for (int k = 0; k < 5; ++k) same = same && key[k] == G.key[k];
+  if (!same)
     FSPH_TRY(graph_capture(c, sp, ns));
   // this step's arguments into the kernel nodes
   for (int i = 0; i < ns; ++i) {

@@ -85,11 +85,16 @@ enum PField : int {
   PF_GY, PF_GZ, PF_PHI, // real-typed fields
   PF_NNGB,              // int32
   PF_HNGB, PF_HITER,    // int32: smoothing-length solver outputs
-  PF_DIVV,              // real: velocity divergence (learned-h feature)
+  PF_LDIV,              // real: velocity divergence (learned-h feature)
   PF_DLH,               // real: ln(h_new / h_old) of the last h update
   PF_INST,              // int32: instance of the particle (batched mode)
   PF_KAD,               // double: adiabat K of the polytropic EOS
                         // (body_u::adiabatic, set by eos::init)
+  PF_E,                 // double: thermokinetic energy (body_u::totalenergy)
+  PF_DEDT,              // real: de/dt
+  PF_ALPHA,             // real: viscosity alpha (body_u::alpha)
+  PF_DIVV,              // real: compute_divv's div v (body_u::divergenceV)
+  PF_DDIVV,             // real: d(div v)/dt (body_u::ddivvdt)
   PF_COUNT
 };
 
@@ -203,6 +208,8 @@ struct Ctx {
     DBuf dt;                          // double [n_inst]
     std::vector<double> dt_h, t_h, tnext_h;
     std::vector<int64_t> seg_h;       // host copy of seg after the last sort
+    DBuf open;                        // int32 [n_inst]: 1 = open boundaries
+    std::vector<int32_t> open_h;
     bool dt_ready = false;            // bounds[.][7] already reduced
   } batch;
 
@@ -214,7 +221,8 @@ struct Ctx {
     bool on = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int key[4] = {0, 0, 0, 0};   // chunk, rsize, adaptive_dt, n_inst
+    // chunk and width, rsize, adaptive_dt, n_inst, thermokinetic
+    int key[5] = {0, 0, 0, 0, 0};
     DBuf counters;               // work counters of the pair passes
     std::vector<char> host;      // host staging of the kernel arguments
     int cus = 0;
@@ -246,6 +254,12 @@ struct Ctx {
   DBuf tstate;
   bool time_dirty = false;
   bool initial_done = false;
+  int64_t io_step = 0;  // outputs written (fsph_write_bodies)
+  // gather set of the last smoothing-length update: mask words of the
+  // gather tiles built (rows x 32-candidate words, every build of the
+  // update) and the gather radius factor of its first pass
+  int64_t h_gather_words = 0;
+  double h_kappa = 0;
 
   // timing: event pairs handed out per timed region of a step
   hipEvent_t ev_groups = nullptr;  // group count copied to mail_h[kMail-1]
@@ -269,9 +283,12 @@ struct PhaseTimer {
       hipEventRecord(c->ev[2 * k], c->stream);
     }
   }
-  ~PhaseTimer() {
+  // Records the end event now (once); the destructor then does nothing.
+  void stop() {
     if (k >= 0) hipEventRecord(c->ev[2 * k + 1], c->stream);
+    k = -1;
   }
+  ~PhaseTimer() { stop(); }
 };
 
 // ---- helpers (fsph_api.hip) ---------------------------------------------
@@ -291,6 +308,7 @@ int hip_check(Ctx *c, hipError_t e, const char *what);
 int ensure(Ctx *c, DBuf &b, size_t bytes, bool keep = false);
 int ensure_particles(Ctx *c, int64_t cap, bool keep);
 size_t field_size(const Ctx *c, int f);
+int download_map(int field);
 int read_flags(Ctx *c, int *flags);
 
 // ---- primitives (k_prim.hip) --------------------------------------------
@@ -332,8 +350,14 @@ int stage_acceleration(Ctx *c);        // k_sph.hip
 int stage_density_only(Ctx *c);        // k_sph.hip
 int stage_acceleration_vsig(Ctx *c);   // k_sph.hip
 int stage_dudt(Ctx *c);                // k_sph.hip
+int stage_dedt(Ctx *c);                // k_sph.hip
+int stage_compute_alpha(Ctx *c);       // k_sph.hip
+int stage_initialize_alpha(Ctx *c);    // k_step.hip
 int stage_gravity(Ctx *c);             // k_grav.hip
 void free_gravity(Ctx *c);             // k_grav.hip
+int gravity_stats(Ctx *c, int64_t out[6]);  // k_grav.hip
+int set_fmm_capacity(Ctx *c, int64_t cap_q, int64_t cap_m2l,
+                     int64_t cap_p2p);      // k_grav.hip
 int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip
 int stage_update_h(Ctx *c);            // k_step.hip
 // eos::init (eos.h:89-92): K = P / rho^gamma per particle, or the parameter
@@ -345,6 +369,8 @@ int stage_eos_init(Ctx *c);
 int stage_check_state(Ctx *c);
 int check_result(Ctx *c);
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
+// set_total_energy of the thermokinetic formulation (k_step.hip)
+int stage_set_total_energy(Ctx *c);
 // device pointer to 2.5 max h (the periodic halo), no host wait
 int stage_halo(Ctx *c, const double **halo);  // k_step.hip
 int read_mail(Ctx *c, int words);             // fsph_api.hip
@@ -356,6 +382,7 @@ int put_time(Ctx *c);      // tstate := host dt, totaltime, totaltime_next
 int sync_time(Ctx *c);     // host copies := tstate (waits) if time_dirty
 int stage_dt_rule(Ctx *c); // dt minimum and set_adaptive_timestep's rule
 int stage_advance(Ctx *c); // totaltime := totaltime_next (or += dt)
+int convert_from_double(Ctx *c, const double *src, float *dst, int64_t n);
 int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
                       int64_t n);              // k_step.hip
 // Batched mode (k_batch.hip): per-instance reductions over the particles

@@ -631,12 +631,24 @@ template <class T> using EPos = typename std::conditional<sizeof(T) == 4,
 // bucket of c particles gets LP = floor(64/c) lanes per particle, which
 // split the staged sources; the lanes' partial sums are added in lane order
 // through LDS.
+//
+// FP32 evaluations stage each source as one float4 (position relative to
+// the sink bucket's centre of mass, and G m) read with one LDS load, and a
+// lane sums its terms four at a time in FP32 (fused multiply-adds) before
+// adding each group of four to FP64 accumulators: one conversion and one
+// FP64 add per four terms instead of per term, at the cost of at most three
+// more FP32 roundings per term.  The particle itself (and any coincident
+// source) is the pair at distance zero, which contributes nothing.  FP64
+// evaluations keep the per-term form.
 template <class T>
 __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   using P = EPos<T>;
-  __shared__ P sx[kStage], sy[kStage], sz[kStage];
-  __shared__ T sm[kStage];
-  __shared__ int32_t sidx[kStage];
+  constexpr bool F32 = sizeof(T) == 4;
+  __shared__ float4 s4[F32 ? kStage : 1];
+  __shared__ P sx[F32 ? 1 : kStage], sy[F32 ? 1 : kStage],
+      sz[F32 ? 1 : kStage];
+  __shared__ T sm[F32 ? 1 : kStage];
+  __shared__ int32_t sidx[F32 ? 1 : kStage];
   __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
   __shared__ T part[4][kEvalB];
   const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
@@ -650,8 +662,7 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   const int64_t p = st + k;
   const double cx = a.c.com[3 * cell], cy = a.c.com[3 * cell + 1],
                cz = a.c.com[3 * cell + 2];
-  const double ox = sizeof(T) == 4 ? cx : 0.0, oy = sizeof(T) == 4 ? cy : 0.0,
-               oz = sizeof(T) == 4 ? cz : 0.0;
+  const double ox = F32 ? cx : 0.0, oy = F32 ? cy : 0.0, oz = F32 ? cz : 0.0;
   P px = 0, py = 0, pz = 0;
   KS<T> f0, f1, f2, pp;
   if (act) {
@@ -691,25 +702,67 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
       int i = 0;
       while (boff[i + 1] <= q) ++i;
       const int64_t pi = bst[i] + (q - boff[i]);
-      sx[q] = (P)(a.x[pi] - ox); sy[q] = (P)(a.y[pi] - oy);
-      sz[q] = (P)(a.z[pi] - oz);
-      sm[q] = (T)a.m[pi];
-      sidx[q] = (int32_t)pi;
+      if constexpr (F32) {
+        s4[q] = make_float4((float)(a.x[pi] - ox), (float)(a.y[pi] - oy),
+                            (float)(a.z[pi] - oz), (float)(a.G * a.m[pi]));
+      } else {
+        sx[q] = (P)(a.x[pi] - ox); sy[q] = (P)(a.y[pi] - oy);
+        sz[q] = (P)(a.z[pi] - oz);
+        sm[q] = (T)a.m[pi];
+        sidx[q] = (int32_t)pi;
+      }
     }
     __syncthreads();
-    if (act)
-      for (int q = r; q < nst; q += LP) {
-        if (sidx[q] == (int32_t)p) continue;
-        const T d0 = (T)(px - sx[q]), d1 = (T)(py - sy[q]),
-                d2 = (T)(pz - sz[q]);
-        const T ir = fast_rsq(d0 * d0 + d1 * d1 + d2 * d2);
-        const T gm = a.G * sm[q] * ir;
-        pp.add(-gm);
-        const T fr = -gm * ir * ir;
-        f0.add(fr * d0);
-        f1.add(fr * d1);
-        f2.add(fr * d2);
+    if (act) {
+      if constexpr (F32) {
+        // four terms per group: partial sums in FP32, then into FP64
+        int q = r;
+        for (; q + 3 * LP < nst; q += 4 * LP) {
+          float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float4 s = s4[q + u * LP];
+            const float d0 = px - s.x, d1 = py - s.y, d2 = pz - s.z;
+            const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            const float ir = r2 > 0.f ? fast_rsq(r2) : 0.f;
+            const float gm = s.w * ir;
+            const float fr = gm * (ir * ir);
+            ph -= gm;
+            g0 -= fr * d0;
+            g1 -= fr * d1;
+            g2 -= fr * d2;
+          }
+          f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
+        }
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
+        for (; q < nst; q += LP) {
+          const float4 s = s4[q];
+          const float d0 = px - s.x, d1 = py - s.y, d2 = pz - s.z;
+          const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          const float ir = r2 > 0.f ? fast_rsq(r2) : 0.f;
+          const float gm = s.w * ir;
+          const float fr = gm * (ir * ir);
+          ph -= gm;
+          g0 -= fr * d0;
+          g1 -= fr * d1;
+          g2 -= fr * d2;
+        }
+        f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
+      } else {
+        for (int q = r; q < nst; q += LP) {
+          if (sidx[q] == (int32_t)p) continue;
+          const T d0 = (T)(px - sx[q]), d1 = (T)(py - sy[q]),
+                  d2 = (T)(pz - sz[q]);
+          const T ir = fast_rsq(d0 * d0 + d1 * d1 + d2 * d2);
+          const T gm = a.G * sm[q] * ir;
+          pp.add(-gm);
+          const T fr = -gm * ir * ir;
+          f0.add(fr * d0);
+          f1.add(fr * d1);
+          f2.add(fr * d2);
+        }
       }
+    }
     __syncthreads();
   }
   part[0][t] = f0.get();
@@ -731,6 +784,24 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
     a.gz[p] = g2;
     a.phi[p] = ph;
   }
+}
+
+// Particle-particle interactions of a P2P list of (sink, source) bucket
+// pairs: count(sink) * count(source), less the self pairs of a bucket with
+// itself.
+__global__ __launch_bounds__(kT) void k_p2p_work(
+    const int2 *__restrict__ l, int64_t n, const int32_t *__restrict__ count,
+    unsigned long long *__restrict__ out) {
+  unsigned long long s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kT) {
+    const int2 pr = l[i];
+    const unsigned long long a = (unsigned)count[pr.x],
+                             b = (unsigned)count[pr.y];
+    s += a * b - (pr.x == pr.y ? a : 0ull);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
 }
 
 int bucket_sort(Ctx *c, const int2 *list, int64_t n, int64_t nc, DBuf &off,
@@ -826,8 +897,12 @@ template <class T> int gravity_impl(Ctx *c) {
     G.cap_m2l = std::max<int64_t>(1 << 16, 256 * nc);
     G.cap_p2p = std::max<int64_t>(1 << 16, 64 * nc);
   }
+  if (G.cap_q < 1) G.cap_q = 1;  // capacities given by set_fmm_capacity
+  if (G.cap_m2l < 1) G.cap_m2l = 1;
+  if (G.cap_p2p < 1) G.cap_p2p = 1;
   constexpr int kMaxLevels = 200;
   FSPH_TRY(ensure(c, G.ctr, (size_t)(2 + kMaxLevels + 2) * 8));
+  PhaseTimer t_pairs(c, FSPH_T_GRAV_PAIRS);
   for (int attempt = 0;; ++attempt) {
     FSPH_TRY(ensure(c, G.qa, (size_t)G.cap_q * 8));
     FSPH_TRY(ensure(c, G.qb, (size_t)G.cap_q * 8));
@@ -876,7 +951,7 @@ template <class T> int gravity_impl(Ctx *c) {
       G.n_p2p = h2;
       break;
     }
-    if (attempt >= 8) return fail(c, FSPH_E_CAP, "FMM pair lists overflow");
+    if (attempt >= 30) return fail(c, FSPH_E_CAP, "FMM pair lists overflow");
     if (fl & 1) G.cap_q *= 2;
     if (fl & 2) G.cap_m2l = std::max<int64_t>(2 * G.cap_m2l, h1 + 1);
     if (fl & 4) G.cap_p2p = std::max<int64_t>(2 * G.cap_p2p, h2 + 1);
@@ -886,11 +961,15 @@ template <class T> int gravity_impl(Ctx *c) {
                        G.m2l_src));
   FSPH_TRY(bucket_sort(c, G.p2p.as<int2>(), G.n_p2p, nc, G.p2p_off,
                        G.p2p_src));
+  t_pairs.stop();
   // ---- M2L, L2L
   FSPH_HIP(c, hipMemsetAsync(Lc, 0, (size_t)nc * 10 * sizeof(T), c->stream));
   M2LArgs<T> ma{cells, G.m2l_off.as<int64_t>(), G.m2l_src.as<int32_t>(), Q, Lc,
                 (T)c->p.G, order};
-  k_m2l<T><<<(unsigned)nc, 64, 0, c->stream>>>(ma);
+  {
+    PhaseTimer t(c, FSPH_T_M2L);
+    k_m2l<T><<<(unsigned)nc, 64, 0, c->stream>>>(ma);
+  }
   FSPH_HIP(c, hipGetLastError());
   for (int d = 1; d <= c->max_depth; ++d) {
     const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
@@ -909,7 +988,10 @@ template <class T> int gravity_impl(Ctx *c) {
                  c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(),
                  c->f[PF_GX].as<T>(), c->f[PF_GY].as<T>(), c->f[PF_GZ].as<T>(),
                  c->f[PF_PHI].as<T>(), (T)c->p.G};
-  k_eval<T><<<(unsigned)nc, kEvalB, 0, c->stream>>>(ea);
+  {
+    PhaseTimer t(c, FSPH_T_P2P);
+    k_eval<T><<<(unsigned)nc, kEvalB, 0, c->stream>>>(ea);
+  }
   return hip_check(c, hipGetLastError(), "k_eval");
 }
 
@@ -921,6 +1003,40 @@ int stage_gravity(Ctx *c) {
   if (!(c->p.theta > 0))
     return fail(c, FSPH_E_ARG, "fmm_macangle (theta) must be positive");
   return c->rsize == 4 ? gravity_impl<float>(c) : gravity_impl<double>(c);
+}
+
+int gravity_stats(Ctx *c, int64_t out[6]) {
+  auto &G = c->grav;
+  for (int i = 0; i < 6; ++i) out[i] = 0;
+  if (G.nc == 0) return FSPH_OK;  // no gravity evaluation yet
+  out[0] = G.nc;
+  out[1] = G.nc - G.n_node_cells;
+  out[2] = G.n_m2l;
+  out[3] = G.n_p2p;
+  out[5] = G.levels;
+  if (G.n_p2p == 0) return FSPH_OK;
+  FSPH_TRY(ensure(c, c->red_tmp, 8));
+  unsigned long long *w = c->red_tmp.as<unsigned long long>();
+  FSPH_HIP(c, hipMemsetAsync(w, 0, 8, c->stream));
+  const unsigned nb = (unsigned)std::min<int64_t>(2048, blocks_for(G.n_p2p));
+  k_p2p_work<<<nb, kT, 0, c->stream>>>(G.p2p.as<int2>(), G.n_p2p,
+                                       G.ccount.as<int32_t>(), w);
+  FSPH_HIP(c, hipGetLastError());
+  unsigned long long v = 0;
+  FSPH_HIP(c, hipMemcpyAsync(&v, w, 8, hipMemcpyDeviceToHost, c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  out[4] = (int64_t)v;
+  return FSPH_OK;
+}
+
+int set_fmm_capacity(Ctx *c, int64_t cap_q, int64_t cap_m2l,
+                     int64_t cap_p2p) {
+  if (cap_q < 1 || cap_m2l < 1 || cap_p2p < 1)
+    return fail(c, FSPH_E_ARG, "FMM list capacities must be positive");
+  c->grav.cap_q = cap_q;
+  c->grav.cap_m2l = cap_m2l;
+  c->grav.cap_p2p = cap_p2p;
+  return FSPH_OK;
 }
 
 void free_gravity(Ctx *c) {

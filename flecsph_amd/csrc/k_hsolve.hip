@@ -320,6 +320,8 @@ double search_factor(const Ctx *c) {
 template <class T> int newton_impl(Ctx *c, const float *h0) {
   const double kappa = search_factor(c);
   FSPH_TRY(stage_gather_tiles(c, kappa));
+  c->h_gather_words += c->gmask_words;
+  if (c->h_kappa == 0) c->h_kappa = kappa;
   NewtonArgs<T> a = newton_args<T>(c, kappa, nullptr);
   a.h0 = h0;
   return launch_newton(c, a, c->nt);
@@ -452,7 +454,7 @@ template <class T> FeatArgs<T> feat_args(Ctx *c) {
   a.h = c->f[PF_H].as<double>();
   a.m = c->f[PF_M].as<double>();
   a.rho = c->f[PF_RHO].as<T>();
-  a.divv = c->f[PF_DIVV].as<T>();
+  a.divv = c->f[PF_LDIV].as<T>();
   a.vsig = c->f[PF_VSIG].as<T>();
   a.ax = c->f[PF_AX].as<T>();
   a.ay = c->f[PF_AY].as<T>();
@@ -604,6 +606,8 @@ template <class T> int learned_impl(Ctx *c) {
                                         std::max(0.03, 6.0 * rms)));
     M.kappa = (float)kl;
     FSPH_TRY(stage_gather_tiles(c, kl, c->h_prop.as<float>()));
+    c->h_gather_words += c->gmask_words;
+    c->h_kappa = kl;
     FSPH_HIP(c, hipMemsetAsync(c->hcnt.p, 0, 8, c->stream));
     NewtonArgs<T> a = newton_args<T>(c, kl, nullptr);
     a.h0 = c->h_prop.as<float>();
@@ -634,6 +638,7 @@ template <class T> int learned_impl(Ctx *c) {
       FSPH_HIP(c, hipStreamSynchronize(c->stream));
       FSPH_TRY(stage_gather_tiles(c, kappa, nullptr, c->glist.as<int32_t>(),
                                   ngl));
+      c->h_gather_words += c->gmask_words;
       NewtonArgs<T> b = newton_args<T>(c, kappa, c->glist.as<int32_t>());
       b.only = c->hflag.as<int32_t>();
       FSPH_TRY(launch_newton(c, b, ngl));

@@ -15,7 +15,7 @@ inline unsigned blocks_for(int64_t n, int t = kT) {
 // ---------------------------------------------------------------------------
 // Particle permutation: dst[f][i] = src[f][perm[i]] for every persistent field
 // ---------------------------------------------------------------------------
-constexpr int kMaxPermFields = 20;
+constexpr int kMaxPermFields = 24;
 struct PermArgs {
   const void *src[kMaxPermFields];
   void *dst[kMaxPermFields];
@@ -74,12 +74,29 @@ struct Box {
   int per[3];
 };
 
+// halo of the particle's instance (batched mode) or the one halo (device
+// memory, written by the h_max reduction); in batched mode an instance may
+// have open boundaries (no teleport, no copies)
+struct Halo {
+  const double *h;
+  const double *per_inst;
+  const int32_t *inst;
+  const int32_t *open;
+  __device__ double of(int64_t i) const {
+    return inst ? per_inst[inst[i]] : *h;
+  }
+  __device__ bool is_open(int64_t i) const {
+    return inst && open && open[inst[i]];
+  }
+};
+
 // Step 1 of pboundary_generate (boundary.h:151-167): teleport into the box.
 __global__ __launch_bounds__(kT) void k_teleport(double *x, double *y,
-                                                 double *z, int64_t n, Box b) {
+                                                 double *z, int64_t n, Box b,
+                                                 Halo hl) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || hl.is_open(i)) return;
   double *X[3] = {x, y, z};
   for (int d = 0; d < 3; ++d) {
     if (!b.per[d]) continue;
@@ -107,17 +124,6 @@ __device__ inline int copies_of(int on) {
   return a + bb + cc + (a & bb) + (a & cc) + (cc & bb) + (a & bb & cc);
 }
 
-// halo of the particle's instance (batched mode) or the one halo (device
-// memory, written by the h_max reduction)
-struct Halo {
-  const double *h;
-  const double *per_inst;
-  const int32_t *inst;
-  __device__ double of(int64_t i) const {
-    return inst ? per_inst[inst[i]] : *h;
-  }
-};
-
 __global__ __launch_bounds__(kT) void k_count_copies(
     const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, int64_t n, Box b, Halo hl,
@@ -125,7 +131,7 @@ __global__ __launch_bounds__(kT) void k_count_copies(
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {x[i], y[i], z[i]};
-  cnt[i] = copies_of(on_edge_mask(p, b, hl.of(i)));
+  cnt[i] = hl.is_open(i) ? 0 : copies_of(on_edge_mask(p, b, hl.of(i)));
 }
 
 // add_corner_XX/XXX (boundary.h:72-119): mirrored coordinate of a corner
@@ -156,7 +162,7 @@ __global__ __launch_bounds__(kT) void k_copy_map(
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   const double p[3] = {x[i], y[i], z[i]};
-  const int on = on_edge_mask(p, b, hl.of(i));
+  const int on = hl.is_open(i) ? 0 : on_edge_mask(p, b, hl.of(i));
   if (!on) return;
   int64_t w = off[i];
   const uint64_t src = (uint64_t)i << 3;
@@ -533,6 +539,11 @@ int persistent_fields(const Ctx *c, int *out) {
   for (int q = 0; q < kNPersistent; ++q) out[nf++] = kPersistent[q];
   if (c->batch.n_inst) out[nf++] = PF_INST;
   if (c->p.eos == FSPH_EOS_POLYTROPIC) out[nf++] = PF_KAD;
+  if (c->p.thermokinetic) out[nf++] = PF_E;
+  if (c->p.viscosity == FSPH_VISC_CULLEN) {  // alpha decays; d(div v)/dt
+    out[nf++] = PF_ALPHA;                    // differences div v in time
+    out[nf++] = PF_DIVV;
+  }
   return nf;
 }
 
@@ -576,7 +587,7 @@ int stage_boundary(Ctx *c) {
   }
   // getSmoothinglength (bodies_system.h:112-124): halo = 2.5 h_max (of
   // each instance in batched mode).
-  Halo hl{nullptr, nullptr, nullptr};
+  Halo hl{nullptr, nullptr, nullptr, nullptr};
   if (c->batch.n_inst) {
     FSPH_TRY(batch_reduce(c, 0));
     const int ni = c->batch.n_inst;
@@ -589,6 +600,7 @@ int stage_boundary(Ctx *c) {
                                hipMemcpyHostToDevice, c->stream));
     hl.per_inst = c->batch.halo.as<double>();
     hl.inst = c->f[PF_INST].as<int32_t>();
+    hl.open = c->batch.open.as<int32_t>();
   } else {
     FSPH_TRY(stage_halo(c, &hl.h));
   }
@@ -601,7 +613,7 @@ int stage_boundary(Ctx *c) {
   const int64_t n = c->n;
   k_teleport<<<blocks_for(n), kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), n, b);
+      c->f[PF_Z].as<double>(), n, b, hl);
   int32_t *cnt = c->f[PF_NNGB].as<int32_t>();
   k_count_copies<<<blocks_for(n), kT, 0, c->stream>>>(
       c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),

@@ -19,6 +19,8 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 template <class T> struct IntArgs {
   double *x, *y, *z, *vx, *vy, *vz, *vhx, *vhy, *vhz, *u;
   const T *ax, *ay, *az, *gx, *gy, *gz, *dudt;
+  double *e;        // thermokinetic energy (FSPH_KICK_E)
+  const T *dedt;
 };
 
 template <class T>
@@ -26,6 +28,7 @@ __device__ inline void integrate_one(const IntArgs<T> &a, int64_t i,
                                      uint32_t what, double dt) {
 #pragma clang fp contract(off)
   if (what & FSPH_KICK_U) a.u[i] = a.u[i] + 0.5 * dt * (double)a.dudt[i];
+  if (what & FSPH_KICK_E) a.e[i] = a.e[i] + 0.5 * dt * (double)a.dedt[i];
   double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
   if (what & FSPH_KICK_V) {
     const double f = 0.5 * dt;
@@ -243,6 +246,12 @@ __global__ __launch_bounds__(kT) void k_to_double(const T *src, double *dst,
   if (i < n) dst[i] = (double)src[i];
 }
 
+__global__ __launch_bounds__(kT) void k_from_double(const double *src,
+                                                    float *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
 // ---- batched mode: per-instance reductions -------------------------------
 // Min/max of doubles with the native FP64 atomics (global_atomic_min_f64 /
 // max_f64); exact, so the order of the updates does not matter.
@@ -446,7 +455,8 @@ template <class T> int integrate_impl(Ctx *c, uint32_t what) {
                c->f[PF_AX].as<T>(),       c->f[PF_AY].as<T>(),
                c->f[PF_AZ].as<T>(),       c->f[PF_GX].as<T>(),
                c->f[PF_GY].as<T>(),       c->f[PF_GZ].as<T>(),
-               c->f[PF_DUDT].as<T>()};
+               c->f[PF_DUDT].as<T>(),     c->f[PF_E].as<double>(),
+               c->f[PF_DEDT].as<T>()};
   const bool b = c->batch.n_inst > 0;
   k_integrate<T><<<blocks_for(c->n), kT, 0, c->stream>>>(
       a, c->n, what, 0.0,
@@ -472,10 +482,16 @@ template <class T> int dt_impl(Ctx *c) {
   return hip_check(c, hipGetLastError(), "k_dt_rule");
 }
 
-template <class T> int red_impl(Ctx *c, double out[10]) {
-  const int nb = (int)std::min<int64_t>(512, (c->n + kT - 1) / kT);
-  FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 10 * 8));
-  double *part = c->red_tmp.as<double>();
+// Workgroups of the reduction: at most kRedRows, each writes one row of 10
+// partial sums; the final row follows them.
+constexpr int kRedRows = 512;
+inline int red_blocks(int64_t n) {
+  return (int)std::min<int64_t>(kRedRows, (n + kT - 1) / kT);
+}
+
+// Sums into part[0 .. (nb + 1) * 10) and copies the final row to out.
+template <class T>
+int red_run(Ctx *c, double *part, int nb, double out[10]) {
   RedArgs<T> a{c->f[PF_X].as<double>(),  c->f[PF_Y].as<double>(),
                c->f[PF_Z].as<double>(),  c->f[PF_VX].as<double>(),
                c->f[PF_VY].as<double>(), c->f[PF_VZ].as<double>(),
@@ -489,6 +505,12 @@ template <class T> int red_impl(Ctx *c, double out[10]) {
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
   out[6] *= 0.5;
   return 0;
+}
+
+template <class T> int red_impl(Ctx *c, double out[10]) {
+  const int nb = red_blocks(c->n);
+  FSPH_TRY(ensure(c, c->red_tmp, (size_t)(nb + 1) * 10 * 8));
+  return red_run<T>(c, c->red_tmp.as<double>(), nb, out);
 }
 
 }  // namespace
@@ -518,6 +540,13 @@ int stage_update_h(Ctx *c) {
 template <class T> struct CheckArgs {
   const double *d[12];  // x y z vx vy vz vhx vhy vhz m u, adiabat (or null)
   const T *r[11];       // ax ay az gx gy gz phi rho P dudt vsig
+  const double *e;      // thermokinetic energy and de/dt (or null)
+  const T *dedt;
+  int evolve_u;         // evolve_internal_energy: u is checked
+  // the energy rate the configuration computes (du/dt, de/dt or neither):
+  // the other is 0 in every body_u of the reference, and here is never
+  // written (WALL copies would read stale memory), so it is not checked
+  int check_dudt, check_dedt;
   int *out;
 };
 
@@ -528,11 +557,20 @@ __global__ __launch_bounds__(kT) void k_check_state(CheckArgs<T> a,
   if (i >= n) return;
   bool nan = false;
 #pragma unroll
-  for (int q = 0; q < 11; ++q) nan |= a.d[q][i] != a.d[q][i];
+  for (int q = 0; q < 10; ++q) nan |= a.d[q][i] != a.d[q][i];
+  if (a.evolve_u) nan |= a.d[10][i] != a.d[10][i];
   if (a.d[11]) nan |= a.d[11][i] != a.d[11][i];
 #pragma unroll
-  for (int q = 0; q < 11; ++q) nan |= a.r[q][i] != a.r[q][i];
-  const bool neg = a.r[7][i] < T(0) || a.r[8][i] < T(0) || a.d[10][i] < 0.0;
+  for (int q = 0; q < 11; ++q)
+    if (q != 9 || a.check_dudt) nan |= a.r[q][i] != a.r[q][i];
+  bool neg = a.r[7][i] < T(0) || a.r[8][i] < T(0) ||
+             (a.evolve_u && a.d[10][i] < 0.0);
+  if (a.e) {
+    nan |= a.e[i] != a.e[i];
+    if (a.check_dedt) nan |= a.dedt[i] != a.dedt[i];
+    // recover_internal_energy stops at u <= 0 (default_physics.h:134-143)
+    if (a.evolve_u) neg |= !(a.d[10][i] > 0.0);
+  }
   if (nan || neg) {
     atomicOr(&a.out[0], (nan ? kFlagNaN : 0) | (neg ? kFlagNeg : 0));
     atomicMin(&a.out[1], (int)i);
@@ -549,6 +587,13 @@ template <class T> int check_impl(Ctx *c, int *out) {
   const int rf[11] = {PF_AX,  PF_AY,  PF_AZ, PF_GX,   PF_GY,  PF_GZ,
                       PF_PHI, PF_RHO, PF_P,  PF_DUDT, PF_VSIG};
   for (int q = 0; q < 11; ++q) a.r[q] = c->f[rf[q]].as<T>();
+  if (c->p.thermokinetic) {
+    a.e = c->f[PF_E].as<double>();
+    a.dedt = c->f[PF_DEDT].as<T>();
+  }
+  a.evolve_u = c->p.evolve_internal_energy;
+  a.check_dudt = c->p.evolve_internal_energy && !c->p.thermokinetic;
+  a.check_dedt = c->p.evolve_internal_energy && c->p.thermokinetic;
   a.out = out;
   k_check_state<T><<<blocks_for(c->n), kT, 0, c->stream>>>(a, c->n);
   return hip_check(c, hipGetLastError(), "k_check_state");
@@ -565,6 +610,53 @@ int stage_check_state(Ctx *c) {
   FSPH_HIP(c, hipMemcpyAsync(c->mail_h + kMailCheck, out, 8,
                              hipMemcpyDeviceToHost, c->stream));
   return 0;
+}
+
+// initialize_alpha (viscosity.h:164-171): 0 for the Cullen-Dehnen switch,
+// sph_viscosity_alpha for the constant viscosity.
+template <class T>
+__global__ __launch_bounds__(kT) void k_fill(T *__restrict__ a, int64_t n,
+                                             T v) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+int stage_initialize_alpha(Ctx *c) {
+  if (c->n <= 0) return 0;
+  const double v =
+      c->p.viscosity == FSPH_VISC_CULLEN ? 0.0 : c->p.visc_alpha;
+  if (c->rsize == 4)
+    k_fill<float><<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_ALPHA].as<float>(), c->n, (float)v);
+  else
+    k_fill<double><<<blocks_for(c->n), kT, 0, c->stream>>>(
+        c->f[PF_ALPHA].as<double>(), c->n, v);
+  return hip_check(c, hipGetLastError(), "k_fill");
+}
+
+// set_total_energy (default_physics.h:328-335) with no external potential:
+// e = (ekin + u) + 0, ekin = .5 (v . v) summed in component order.
+__global__ __launch_bounds__(kT) void k_set_energy(
+    const double *__restrict__ vx, const double *__restrict__ vy,
+    const double *__restrict__ vz, const double *__restrict__ u, int64_t n,
+    double *__restrict__ e) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  double vv = 0;
+  vv += vx[i] * vx[i];
+  vv += vy[i] * vy[i];
+  vv += vz[i] * vz[i];
+  e[i] = .5 * vv + u[i] + 0.0;
+}
+
+int stage_set_total_energy(Ctx *c) {
+  if (!c->p.thermokinetic || c->n <= 0) return 0;
+  k_set_energy<<<blocks_for(c->n), kT, 0, c->stream>>>(
+      c->f[PF_VX].as<double>(), c->f[PF_VY].as<double>(),
+      c->f[PF_VZ].as<double>(), c->f[PF_U].as<double>(), c->n,
+      c->f[PF_E].as<double>());
+  return hip_check(c, hipGetLastError(), "k_set_energy");
 }
 
 // eos::init (eos.h:76-92): the adiabat K = P / rho^gamma of each particle
@@ -606,7 +698,8 @@ template <class T> void step_graph_args_t(Ctx *c, char *slot) {
                     c->f[PF_AX].as<T>(),       c->f[PF_AY].as<T>(),
                     c->f[PF_AZ].as<T>(),       c->f[PF_GX].as<T>(),
                     c->f[PF_GY].as<T>(),       c->f[PF_GZ].as<T>(),
-                    c->f[PF_DUDT].as<T>()};
+                    c->f[PF_DUDT].as<T>(),     c->f[PF_E].as<double>(),
+                    c->f[PF_DEDT].as<T>()};
   s.da = dt_args<T>(c);
   s.h = c->f[PF_H].as<double>();
   s.m = c->f[PF_M].as<double>();
@@ -619,10 +712,12 @@ template <class T> void step_graph_args_t(Ctx *c, char *slot) {
   std::memcpy(slot, &s, sizeof(s));
 }
 
-template <class T> const void *step_graph_func_t(int what) {
+template <class T> const void *step_graph_func_t(int what, bool tk) {
   switch (what) {
     case 0: return reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_V>);
-    case 1: return reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_U>);
+    case 1:
+      return tk ? reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_E>)
+                : reinterpret_cast<const void *>(k_integrate_p<T, FSPH_KICK_U>);
     case 2: return reinterpret_cast<const void *>(k_update_h_p<T>);
     default: return reinterpret_cast<const void *>(k_seg_dt_p<T>);
   }
@@ -637,8 +732,9 @@ void step_graph_args(Ctx *c, char *slots) {
 }
 
 const void *step_graph_func(const Ctx *c, int what) {
-  return c->rsize == 4 ? step_graph_func_t<float>(what)
-                       : step_graph_func_t<double>(what);
+  const bool tk = c->p.thermokinetic != 0;
+  return c->rsize == 4 ? step_graph_func_t<float>(what, tk)
+                       : step_graph_func_t<double>(what, tk);
 }
 
 int step_graph_seg_init(Ctx *c) {
@@ -746,6 +842,11 @@ int convert_to_double(Ctx *c, const void *src, size_t esize, double *dst,
     k_to_double<double><<<blocks_for(n), kT, 0, c->stream>>>(
         static_cast<const double *>(src), dst, n);
   return hip_check(c, hipGetLastError(), "k_to_double");
+}
+
+int convert_from_double(Ctx *c, const double *src, float *dst, int64_t n) {
+  k_from_double<<<blocks_for(n), kT, 0, c->stream>>>(src, dst, n);
+  return hip_check(c, hipGetLastError(), "k_from_double");
 }
 
 }  // namespace fsph

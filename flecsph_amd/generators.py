@@ -68,7 +68,7 @@ def sod(scale=1):
     h = eta * 2.0 * (m / rho) ** (1.0 / 3.0)
     c = np.sqrt(gamma * (gamma - 1.0) * u)
     zero = np.zeros(len(x))
-    params = dict(kernel=_abi.KERNEL_CUBIC_SPLINE, eos=_abi.EOS_IDEAL,
+    params = dict(thermokinetic=0, kernel=_abi.KERNEL_CUBIC_SPLINE, eos=_abi.EOS_IDEAL,
                   precision=_abi.PREC_FP64, adaptive_dt=1, gamma=gamma,
                   eta=eta, visc_alpha=1.0, visc_beta=2.0, visc_eps=0.01,
                   cfl=0.25, initial_dt=1e-4, n_target=60)
@@ -98,7 +98,7 @@ def sedov(n_side=64, seed=SEED_BASE):
     P = (gamma - 1.0) * rho * u
     c = np.sqrt(gamma * (gamma - 1.0) * u)
     zero = np.zeros(n)
-    params = dict(kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_IDEAL,
+    params = dict(thermokinetic=0, kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_IDEAL,
                   precision=_abi.PREC_FP32, adaptive_dt=1, gamma=gamma,
                   eta=eta, visc_alpha=1.0, visc_beta=2.0, visc_eps=0.01,
                   cfl=0.25, initial_dt=1e-6, periodic=(1, 1, 1),
@@ -119,6 +119,59 @@ def sedov_batch(n_inst=64, lo=8_000, hi=200_000, seed=SEED_BASE + 7):
         parts, prm = sedov(n_side=int(ns))
         sets.append(parts)
     return sets, prm
+
+
+def sod_tube(j=6, gamma=5.0 / 3.0, kernel=_abi.KERNEL_WENDLAND_C6,
+             n_ngb=100):
+    """A Sod tube for a batch of the Sedov parameter set: [0,1]x[0,0.2]^2,
+    interface at x = 0.5, left rho=1, p=1 on a 10j x 4j x 4j cell-centred
+    lattice (dx = 0.05/j), right rho=0.125, p=0.1 on 5j x 2j x 2j at 2dx
+    (equal masses), v = 0; 180 j^3 particles, open boundaries.  Returns the
+    particle set only (the batch holds the parameters)."""
+    dx = 0.05 / j
+    xl, yl, zl = _lattice(10 * j, 4 * j, 4 * j, dx, 0.0, 0.0, 0.0)
+    xr, yr, zr = _lattice(5 * j, 2 * j, 2 * j, 2 * dx, 0.5, 0.0, 0.0)
+    x = np.concatenate([xl, xr]); y = np.concatenate([yl, yr])
+    z = np.concatenate([zl, zr])
+    left = x < 0.5
+    rho = np.where(left, 1.0, 0.125)
+    P = np.where(left, 1.0, 0.1)
+    u = P / ((gamma - 1.0) * rho)
+    m = np.full(len(x), dx ** 3)
+    h = _eta_for(n_ngb) * 2.0 * (m / rho) ** (1.0 / 3.0)
+    c = np.sqrt(gamma * (gamma - 1.0) * u)
+    zero = np.zeros(len(x))
+    return _pack(x, y, z, zero, zero, zero, u, h, m, rho, P, c)
+
+
+def mixed_batch(n_inst=64, lo=8_000, hi=200_000, seed=SEED_BASE + 7):
+    """Batched workload of Sedov blasts and Sod tubes: per instance a seeded
+    coin picks the kind and a seeded log-uniform draw in [lo, hi] the
+    particle count, rounded to the nearest lattice of that kind (n_side^3
+    for a blast, 180 j^3 for a tube).  All share the Sedov parameter set
+    (Wendland C6, gamma 5/3, 100 neighbours, FP32); blasts are periodic in
+    the unit box, tubes have open boundaries.  Returns (list of particle
+    sets, common parameters, per-instance periodic flags)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n_inst))
+    kinds = rng.random(n_inst) < 0.5  # True: Sod tube
+    sets, periodic, prm = [], [], None
+    for cnt, tube in zip(counts, kinds):
+        if tube:
+            j = int(np.clip(np.rint(np.cbrt(cnt / 180.0)),
+                            np.ceil(np.cbrt(lo / 180.0)),
+                            np.floor(np.cbrt(hi / 180.0))))
+            sets.append(sod_tube(j))
+            periodic.append(False)
+        else:
+            ns = int(np.clip(np.rint(np.cbrt(cnt)), np.ceil(np.cbrt(lo)),
+                             np.floor(np.cbrt(hi))))
+            parts, prm = sedov(n_side=ns)
+            sets.append(parts)
+            periodic.append(True)
+    if prm is None:
+        prm = sedov(n_side=8)[1]
+    return sets, prm, periodic
 
 
 def _polytrope_mass_inverse(q):
@@ -160,7 +213,11 @@ def polytrope(n=1_000_000, seed=SEED_BASE + 3):
     P = K * rho_floor ** gamma
     c = np.sqrt(K * gamma * rho_floor ** (gamma - 1.0))
     zero = np.zeros(n)
-    params = dict(kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_POLYTROPIC,
+    # the internal energy is not evolved: with a polytropic EOS it does not
+    # enter the pressure, and the reference's polytropic-star setups turn it
+    # off (data/mesa_nx20.par:52, data/implosion_nx20.par:25)
+    params = dict(thermokinetic=0, evolve_internal_energy=0,
+                  kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_POLYTROPIC,
                   precision=_abi.PREC_FP32, adaptive_dt=1, gamma=gamma,
                   poly_K=K, eta=eta, visc_alpha=1.0, visc_beta=2.0,
                   visc_eps=0.01, cfl=0.25, initial_dt=1e-4,
@@ -205,7 +262,7 @@ def _cold_sphere(pos):
     P = (gamma - 1.0) * rho * u
     c = np.sqrt(gamma * (gamma - 1.0) * u)
     zero = np.zeros(n)
-    params = dict(kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_IDEAL,
+    params = dict(thermokinetic=0, kernel=_abi.KERNEL_WENDLAND_C6, eos=_abi.EOS_IDEAL,
                   precision=_abi.PREC_FP32, adaptive_dt=1, gamma=gamma,
                   eta=eta, visc_alpha=1.0, visc_beta=2.0, visc_eps=0.01,
                   cfl=0.25, initial_dt=1e-4, gravity=_abi.GRAV_FMM, G=1.0,
@@ -214,14 +271,22 @@ def _cold_sphere(pos):
                  rho, P, c), params
 
 
-def collapse(n=4_000_000, seed=SEED_BASE + 5):
+def collapse(n=4_000_000, seed=SEED_BASE + 5, jitter=0.15):
     """Cold collapse of a uniform sphere: about n points of a seeded FCC
     lattice in the unit ball (the lattice of data/collapse_nx10.par; random
     points would leave close pairs that unsoftened gravity merges within a
-    few steps), rho = 1, u = 0.05, gamma = 5/3, G = 1, theta = 0.5,
-    Wendland C6 with 100 neighbours."""
+    few steps), each moved by a seeded uniform offset of up to `jitter`
+    nearest-neighbour spacings per axis (so hard neighbour counts are not
+    locked to lattice shells, while no two points come closer than
+    1 - 2 sqrt(3) jitter spacings), rho = 1, u = 0.05, gamma = 5/3, G = 1,
+    theta = 0.5, Wendland C6 with 100 neighbours."""
     rng = np.random.default_rng(seed)
-    return _cold_sphere(_fcc_ball(n, rng))
+    pos = _fcc_ball(n, rng)
+    if jitter:
+        a = (4.0 * (4.0 * math.pi / 3.0) / n) ** (1.0 / 3.0)
+        d_nn = a / math.sqrt(2.0)
+        pos = pos + (rng.random(pos.shape) * 2.0 - 1.0) * (jitter * d_nn)
+    return _cold_sphere(pos)
 
 
 def random_box(n=2000, seed=1, h_spread=0.5, n_ngb=40, v_scale=0.1,
@@ -240,7 +305,7 @@ def random_box(n=2000, seed=1, h_spread=0.5, n_ngb=40, v_scale=0.1,
     u = 1.0 + 0.5 * rng.random(n)
     gamma = 1.4
     c = np.sqrt(gamma * (gamma - 1.0) * u)
-    params = dict(kernel=kernel, eos=_abi.EOS_IDEAL, gamma=gamma, eta=eta,
+    params = dict(thermokinetic=0, kernel=kernel, eos=_abi.EOS_IDEAL, gamma=gamma, eta=eta,
                   precision=_abi.PREC_FP64, adaptive_dt=1, initial_dt=1e-3,
                   n_target=n_ngb)
     return _pack(x, y, z, vx, vy, vz, u, h, m, None, None, c), params

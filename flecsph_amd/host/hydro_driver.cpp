@@ -5,7 +5,7 @@
 // It shows the call sequence a FleCSPH driver keeps when the pipeline moves
 // to the GPU, and prints the conserved sums as one JSON line.
 //
-// usage: fsph_hydro_driver [steps] [precision 0|1]
+// usage: fsph_hydro_driver [steps] [precision 0|1] [thermokinetic 0|1]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +16,7 @@
 int main(int argc, char **argv) {
   const int steps = argc > 1 ? std::atoi(argv[1]) : 5;
   const int prec = argc > 2 ? std::atoi(argv[2]) : FSPH_PREC_FP64;
+  const int tk = argc > 3 ? std::atoi(argv[3]) : 1;  // params.h:590-592
   // two cell-centred cubic lattices with equal particle masses
   const int nl = 30;
   const double dx = 0.5 / nl;
@@ -55,6 +56,7 @@ int main(int argc, char **argv) {
   prm.eta = eta;
   prm.initial_dt = 1e-4;
   prm.cfl = 0.25;
+  prm.thermokinetic = tk;
 
   using namespace fsph;
   try {
@@ -68,15 +70,20 @@ int main(int argc, char **argv) {
       if (it == 0) {  // main_driver.cc:102-173
         bs.update_iteration();
         bs.apply_all(eos::init);
+        if (tk) bs.apply_all(physics::set_total_energy);
         bs.apply_in_smoothinglength(physics::compute_density_pressure_soundspeed);
         bs.apply_all(integration::save_velocityhalf);
         bs.reset_ghosts();
         bs.apply_in_smoothinglength(physics::compute_acceleration);
         bs.apply_all(physics::compute_dt);
         bs.get_all(physics::set_adaptive_timestep);
-        bs.apply_in_smoothinglength(physics::compute_dudt);
+        if (tk)
+          bs.apply_in_smoothinglength(physics::compute_dedt);
+        else
+          bs.apply_in_smoothinglength(physics::compute_dudt);
       } else {  // main_driver.cc:174-245
-        bs.apply_all(integration::leapfrog_kick_u);
+        bs.apply_all(tk ? integration::leapfrog_kick_e
+                        : integration::leapfrog_kick_u);
         bs.apply_all(integration::leapfrog_kick_v);
         bs.apply_all(integration::save_velocityhalf);
         bs.apply_all(integration::leapfrog_drift);
@@ -86,8 +93,13 @@ int main(int argc, char **argv) {
         bs.apply_in_smoothinglength(physics::compute_acceleration);
         bs.apply_all(integration::leapfrog_kick_v);
         bs.reset_ghosts();
-        bs.apply_in_smoothinglength(physics::compute_dudt);
-        bs.apply_all(integration::leapfrog_kick_u);
+        if (tk) {
+          bs.apply_in_smoothinglength(physics::compute_dedt);
+          bs.apply_all(integration::leapfrog_kick_e);
+        } else {
+          bs.apply_in_smoothinglength(physics::compute_dudt);
+          bs.apply_all(integration::leapfrog_kick_u);
+        }
       }
       bs.get_all(physics::compute_smoothinglength);  // main_driver.cc:247
       bs.apply_all(physics::compute_dt);              // main_driver.cc:267

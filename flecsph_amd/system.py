@@ -102,6 +102,14 @@ class _System:
         self._call("download", idx, out.ctypes.data_as(C.c_void_p))
         return out
 
+    def set(self, field, values):
+        """Overwrites one field of every particle (current order)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        n, _ = self.count()
+        if v.shape != (n,):
+            raise ValueError(f"{field}: expected {n} values")
+        self._call("set_field", _abi.FIELD_INDEX[field], _dptr(v))
+
     def get_all(self, fields=None):
         fields = fields or [f for f in _abi.FIELDS]
         return {f: self.get(f) for f in fields}
@@ -124,6 +132,28 @@ class _System:
 
     def dudt(self):
         self._call("dudt")
+
+    def dedt(self):
+        self._call("dedt")
+
+    def write_bodies(self, prefix, separate_iterations=False):
+        """body_system::write_bodies: appends a Step#k to <prefix>.h5part
+        (a directory of raw arrays with JSON attributes)."""
+        self._call("write_bodies", str(prefix).encode(),
+                   int(bool(separate_iterations)))
+
+    def read_bodies(self, prefix, start_iteration=0):
+        """body_system::read_bodies from <prefix>.h5part."""
+        self._call("read_bodies", str(prefix).encode(), int(start_iteration))
+
+    def compute_alpha(self):
+        self._call("compute_alpha")
+
+    def initialize_alpha(self):
+        self._call("initialize_alpha")
+
+    def set_total_energy(self):
+        self._call("set_total_energy")
 
     def integrate(self, what):
         self._call("integrate", int(what))
@@ -161,6 +191,17 @@ class _System:
         out = np.zeros(10)
         self._call("reductions", _dptr(out))
         return out
+
+    def gravity_stats(self):
+        out = np.zeros(6, np.int64)
+        self._call("gravity_stats",
+                   out.ctypes.data_as(C.POINTER(C.c_int64)))
+        keys = ("cells", "buckets", "m2l_pairs", "p2p_pairs",
+                "p2p_interactions", "pair_levels")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def set_fmm_capacity(self, cap_q, cap_m2l, cap_p2p):
+        self._call("set_fmm_capacity", int(cap_q), int(cap_m2l), int(cap_p2p))
 
     def range(self):
         lo, hi = np.zeros(3), np.zeros(3)
@@ -272,6 +313,12 @@ class BodySystem(_System):
                    float(lr), C.byref(loss))
         return float(loss.value)
 
+    def h_gather(self):
+        """(gather radius factor, gather mask words) of the last h update."""
+        k, w = C.c_double(), C.c_int64()
+        self._call("h_gather", C.byref(k), C.byref(w))
+        return k.value, w.value
+
     def h_stats(self):
         fb, cl = C.c_int64(), C.c_int64()
         ts, loss = C.c_int32(), C.c_float()
@@ -345,6 +392,9 @@ class BatchSystem(BodySystem):
 
     def enable_graph(self, on=True):
         self._call("enable_graph", int(bool(on)))
+
+    def instance_set_periodic(self, i, periodic):
+        self._call("instance_set_periodic", int(i), int(bool(periodic)))
 
     def instance_time(self, i):
         it, dt, t = C.c_int64(), C.c_double(), C.c_double()

@@ -100,7 +100,26 @@ typedef struct fsph_params {
   int32_t n_tol;       /* FSPH_H_LEARNED: a proposal whose kernel-weighted
                           neighbour count is off the target by more than
                           this falls back to plain Newton-h (5)             */
+  int32_t thermokinetic; /* thermokinetic_formulation (params.h:590-592):
+                          1 evolves the specific thermokinetic energy
+                          e = u + v^2/2 with compute_dedt and leapfrog_kick_e
+                          and recovers u from it (default_physics.h:127-
+                          145, 503-566; integration.h:95-98); 0 evolves u
+                          with compute_dudt                                 */
+  int32_t viscosity;   /* sph_viscosity: FSPH_VISC_CONSTANT or
+                          FSPH_VISC_CULLEN (viscosity.h:128-155)            */
+  double visc_alpha_max; /* sph_viscosity_alpha_max (Cullen-Dehnen)         */
+  double visc_l;       /* sph_viscosity_l (Cullen-Dehnen decay length)      */
+  double visc_delta;   /* sph_viscosity_delta (Cullen-Dehnen)               */
+  int32_t evolve_internal_energy; /* evolve_internal_energy (params.h:583-
+                          585): 0 leaves out the energy pass and the energy
+                          kicks (hydro/main_driver.cc:145,175,214) and the
+                          checks of u, as the reference's polytropic-star
+                          configurations do (data/mesa_nx20.par:52)        */
 } fsph_params;
+
+#define FSPH_VISC_CONSTANT 0 /* Monaghan alpha, beta (viscosity.h:128-137) */
+#define FSPH_VISC_CULLEN 1   /* Cullen & Dehnen 2010 (viscosity.h:139-363) */
 
 #define FSPH_H_CLOSED 0  /* h = eta kw (m/rho)^(1/3), default_physics.h:826 */
 #define FSPH_H_NEWTON 1  /* Newton on rho~(h) h^3 = m (eta kw)^3            */
@@ -139,6 +158,10 @@ enum fsph_field {
   FSPH_F_HNGB,   /* #{b : r_ab < h_a} after the last Newton/learned update */
   FSPH_F_HITER,  /* Newton iterations of the last update (0: accepted
                     learned proposal)                                      */
+  FSPH_F_E,      /* thermokinetic energy e (body_u::getTotalenergy)        */
+  FSPH_F_DEDT,   /* de/dt (body_u::getDedt)                                */
+  FSPH_F_ALPHA,  /* viscosity alpha of the particle (body_u::getAlpha)     */
+  FSPH_F_DIVV,   /* velocity divergence (body_u::getDivergenceV)           */
   FSPH_NFIELDS
 };
 
@@ -155,12 +178,45 @@ const char *fsph_last_error(const fsph_ctx *ctx);
    bodies_system.h:92-98, minus the HDF5 reader).  Resets time and dt. */
 int fsph_upload(fsph_ctx *ctx, const fsph_particles *parts);
 
+/* body_system::write_bodies (bodies_system.h:100-102 -> io.h:873-1081)
+   and read_bodies (bodies_system.h:92-98 -> io.h:632-871) in the H5part
+   step layout of doc/H5_STRUCTURE.md, stored without HDF5: <prefix>.h5part
+   is a directory holding header.json ({"format": "fsph-h5raw-1",
+   "dimension": 3}) and one Step#<k> directory per output with
+   attributes.json ({"time", "iteration", "timestep", "nparticles"}) and
+   one raw little-endian array per dataset, named as the reference names
+   them: x y z vx vy vz ax ay az h rho u P m alpha divergenceV dDivVdt e
+   (.f8, double), id (.i8, int64), type (.i4, int32: 0 NORMAL, 1 WALL).
+   write_bodies appends a step holding every particle (WALL copies
+   included, as the reference writes tree entities) with the current
+   time, iteration and dt; with separate_iterations each output goes to
+   <prefix>_<%05d output>.h5part as Step#0.  read_bodies uploads the
+   particles of Step#0 (start_iteration 0) or of the step whose iteration
+   attribute is start_iteration (in <prefix>.h5part, else in
+   <prefix>_<%05d start_iteration>.h5part), reading x y z vx vy vz m h id
+   and, when present, u, rho and P; for start_iteration != 0 the context's
+   dt, time and iteration become the step's, and the next fsph_step is the
+   driver's initial iteration at that iteration (main_driver.cc:101). */
+int fsph_write_bodies(fsph_ctx *ctx, const char *prefix,
+                      int separate_iterations);
+int fsph_read_bodies(fsph_ctx *ctx, const char *prefix,
+                     int64_t start_iteration);
+
 /* Number of particles currently held: *n_total includes periodic WALL
    copies, *n_real does not (body_system::getNLocalBodies). */
 int fsph_count(const fsph_ctx *ctx, int64_t *n_total, int64_t *n_real);
 
 /* Copies one field of all n_total particles to host memory dst. */
 int fsph_download(fsph_ctx *ctx, int field, void *dst);
+
+/* Overwrites one field of all n_total particles from host doubles src (in
+   the context's current order), the write-back half of running a functor
+   on the host: the reference's functors read and write body_u fields
+   directly (tree_topology.h:406).  Ids, keys and neighbour counts cannot
+   be set; setting x, y, z or h invalidates the tree until the next
+   fsph_update_iteration.  Derived fields are rounded to the context's
+   precision. */
+int fsph_set_field(fsph_ctx *ctx, int field, const double *src);
 
 /* ---- the per-step pipeline (app/drivers/hydro/main_driver.cc:98-279) -- */
 
@@ -198,10 +254,50 @@ int fsph_acceleration(fsph_ctx *ctx);
    does (default_physics.h:406-407). */
 int fsph_gravity(fsph_ctx *ctx);
 
+/* Sizes of the last fsph_gravity (introspection for measurement; no
+   reference counterpart): out[0] FMM cells (internal nodes with more than
+   leaf_size particles, and buckets), out[1] buckets, out[2] M2L cell
+   pairs, out[3] P2P bucket pairs, out[4] particle-particle interactions of
+   those pairs (self pairs excluded), out[5] levels of the dual-tree pair
+   recursion.  All zero before the first gravity evaluation. */
+int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[6]);
+
+/* Capacities (entries) of the FMM pair queue, M2L list and P2P list that
+   the next fsph_gravity starts from (no reference counterpart).  The lists
+   grow by doubling whenever a level overflows them, so any positive value
+   gives the same result; small values make the overflow-and-regrow path
+   run, for testing. */
+int fsph_set_fmm_capacity(fsph_ctx *ctx, int64_t cap_q, int64_t cap_m2l,
+                          int64_t cap_p2p);
+
 /* apply_in_smoothinglength(physics::compute_dudt) (default_physics.h:
    422-487) followed by apply_all(recompute_pressure_soundspeed)
    (default_physics.h:154-161) with the context's current dt. */
 int fsph_dudt(fsph_ctx *ctx);
+
+/* apply_in_smoothinglength(physics::compute_dedt) (default_physics.h:
+   503-566) followed by apply_all(recompute_pressure_soundspeed_
+   thermokinetic) (default_physics.h:170-180) with the context's current
+   dt: de/dt, u recovered from e and v, P and c at the half step.  Needs
+   thermokinetic = 1. */
+int fsph_dedt(fsph_ctx *ctx);
+
+/* apply_in_smoothinglength(viscosity::compute_alpha) (viscosity.h:
+   318-356): the Cullen-Dehnen switch sets each particle's alpha from the
+   trigger -d(div v)/dt, the limiter xi and the signal speed (compute_divv,
+   default_physics.h:274-300, runs inside fsph_density_eos with this
+   viscosity).  Needs viscosity = FSPH_VISC_CULLEN. */
+int fsph_compute_alpha(fsph_ctx *ctx);
+
+/* apply_all(viscosity::initialize_alpha) (viscosity.h:164-171): alpha = 0
+   for the Cullen-Dehnen viscosity, sph_viscosity_alpha otherwise. */
+int fsph_initialize_alpha(fsph_ctx *ctx);
+
+/* apply_all(physics::set_total_energy) (default_physics.h:328-335):
+   e = v^2/2 + u (no external potential).  The driver calls it once, after
+   eos::init (hydro/main_driver.cc:108-111); later density passes recover
+   u from e (default_physics.h:313-314).  Needs thermokinetic = 1. */
+int fsph_set_total_energy(fsph_ctx *ctx);
 
 /* Leapfrog pieces (include/physics/integration.h:53-110), with the
    context's current dt.  `what` is a bitmask of FSPH_KICK_*. */
@@ -209,6 +305,7 @@ int fsph_dudt(fsph_ctx *ctx);
 #define FSPH_KICK_U 2u      /* leapfrog_kick_u   integration.h:81-84     */
 #define FSPH_SAVE_VHALF 4u  /* save_velocityhalf integration.h:53-55     */
 #define FSPH_DRIFT 8u       /* leapfrog_drift    integration.h:107-110   */
+#define FSPH_KICK_E 16u     /* leapfrog_kick_e   integration.h:95-98     */
 int fsph_integrate(fsph_ctx *ctx, uint32_t what);
 
 /* get_all(physics::compute_smoothinglength) (default_physics.h:826-848). */
@@ -308,6 +405,14 @@ int fsph_mlp_train(fsph_ctx *ctx, const float *x, const float *t, int64_t B,
 int fsph_h_stats(fsph_ctx *ctx, int64_t *fallback, int64_t *clamped,
                  int32_t *trained_steps, float *last_loss);
 
+/* Gather set of the last smoothing-length update (Newton-h or learned-h):
+   *kappa is the gather radius of its first pass in units of the particle's
+   h (h_search for Newton-h, the narrowed radius around the proposal for
+   learned-h), *mask_words the neighbour-mask words of every gather tile
+   built in the update (rows times 32-candidate words: the pair tests the
+   solve walks).  Both 0 after a closed-form update. */
+int fsph_h_gather(fsph_ctx *ctx, double *kappa, int64_t *mask_words);
+
 /* Milliseconds spent in each phase of the last fsph_step, measured with
    HIP events on the context's stream around the phase's kernels (summed
    when a phase runs more than once in a step); see FSPH_T_* for the slots.
@@ -329,6 +434,13 @@ enum fsph_timer {
   FSPH_T_DT,            /* time-step reduction                           */
   FSPH_T_GRAPH,         /* graph replay of a batched step's tail (pair
                            passes, kicks, h, dt minimum)                 */
+  FSPH_T_GRAV_PAIRS,    /* FMM interaction lists: dual-tree pair levels
+                           and their bucketing by sink (inside
+                           FSPH_T_GRAVITY)                               */
+  FSPH_T_M2L,           /* multipole-to-local (1 kernel, inside
+                           FSPH_T_GRAVITY)                               */
+  FSPH_T_P2P,           /* local-expansion evaluation and direct sums
+                           (1 kernel, inside FSPH_T_GRAVITY)             */
   FSPH_NTIMERS
 };
 int fsph_phase_times(fsph_ctx *ctx, float ms[FSPH_NTIMERS]);
@@ -361,14 +473,21 @@ int fsph_instance_count(const fsph_ctx *ctx, int inst, int64_t *n_total,
 /* One field (as fsph_download) of instance inst's particles, in the
    instance's current (sorted) order. */
 int fsph_instance_download(fsph_ctx *ctx, int inst, int field, void *dst);
+/* Boundary of instance inst (new): periodic = 1 (the default) gives the
+   context's periodic boundaries and box (periodic[], box[]); 0 gives open
+   boundaries (no wrapping, no WALL copies), so one batch can hold periodic
+   Sedov blasts and open Sod tubes.  Takes effect at the next
+   fsph_update_iteration. */
+int fsph_instance_set_periodic(fsph_ctx *ctx, int inst, int periodic);
 int fsph_instance_time(const fsph_ctx *ctx, int inst, int64_t *iteration,
                        double *dt, double *totaltime);
 
 /* Graph mode of a batched context (new): the sync-free tail of every step
    after the neighbour tiles (density, signal speed, acceleration, kick v,
    du/dt, kick u, closed-form h, per-instance dt minimum) runs as one
-   replayed hipGraph of persistent kernels.  Needs h_mode FSPH_H_CLOSED;
-   results are those of the eager path. */
+   replayed hipGraph of persistent kernels.  Needs h_mode FSPH_H_CLOSED,
+   no gravity and the constant viscosity; results are those of the eager
+   path. */
 int fsph_enable_graph(fsph_ctx *ctx, int on);
 
 /* Autotuner (new): times `steps` calls of fsph_step for each candidate

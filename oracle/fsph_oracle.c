@@ -45,6 +45,8 @@ struct fo_sys {
   double *x, *y, *z, *vx, *vy, *vz, *vhx, *vhy, *vhz, *u, *h, *m;
   double *rho, *P, *c, *vsig, *ax, *ay, *az, *dudt, *gx, *gy, *gz, *phi;
   double *kad; /* adiabat of the polytropic EOS (body_u::adiabatic) */
+  double *e, *dedt; /* thermokinetic energy and its rate (body_u) */
+  double *alpha, *divv, *ddivvdt; /* viscosity alpha, div v, d(div v)/dt */
   int have_rhoP; /* the last upload gave rho and P */
   int64_t *id;
   uint64_t *key;
@@ -83,13 +85,14 @@ static double kernel_width(const fsph_params *p) {
 /* ------------------------------------------------------------------ */
 /* allocation                                                          */
 /* ------------------------------------------------------------------ */
-#define NDBL 25
+#define NDBL 30
 static double **dbl_fields(fo_sys *s, double **out[NDBL]) {
   double **f[NDBL] = {&s->x,   &s->y,    &s->z,   &s->vx,  &s->vy,  &s->vz,
                       &s->vhx, &s->vhy,  &s->vhz, &s->u,   &s->h,   &s->m,
                       &s->rho, &s->P,    &s->c,   &s->vsig, &s->ax, &s->ay,
                       &s->az,  &s->dudt, &s->gx,  &s->gy,  &s->gz,  &s->phi,
-                      &s->kad};
+                      &s->kad, &s->e,    &s->dedt, &s->alpha, &s->divv,
+                      &s->ddivvdt};
   memcpy(out, f, sizeof(f));
   return NULL;
 }
@@ -164,6 +167,9 @@ int fo_upload(fo_sys *s, const fsph_particles *in) {
     s->P[i] = in->P ? in->P[i] : 0.0;
     s->c[i] = in->c ? in->c[i] : 0.0;
     s->vsig[i] = 0; s->ax[i] = s->ay[i] = s->az[i] = 0; s->dudt[i] = 0;
+    s->e[i] = 0; s->dedt[i] = 0;
+    s->alpha[i] = s->p.viscosity == FSPH_VISC_CULLEN ? 0.0 : s->p.visc_alpha;
+    s->divv[i] = 0; s->ddivvdt[i] = 0;
     s->gx[i] = s->gy[i] = s->gz[i] = s->phi[i] = 0;
     s->key[i] = 0; s->nngb[i] = 0; s->hngb[i] = 0; s->hiter[i] = 0;
   }
@@ -212,6 +218,10 @@ int fo_download(fo_sys *s, int field, void *dst) {
     case FSPH_F_GY: src = s->gy; break;
     case FSPH_F_GZ: src = s->gz; break;
     case FSPH_F_PHI: src = s->phi; break;
+    case FSPH_F_E: src = s->e; break;
+    case FSPH_F_DEDT: src = s->dedt; break;
+    case FSPH_F_ALPHA: src = s->alpha; break;
+    case FSPH_F_DIVV: src = s->divv; break;
     case FSPH_F_ID: memcpy(dst, s->id, (size_t)s->n * 8); return 0;
     case FSPH_F_KEY: memcpy(dst, s->key, (size_t)s->n * 8); return 0;
     case FSPH_F_NNGB:
@@ -227,6 +237,23 @@ int fo_download(fo_sys *s, int field, void *dst) {
     default: return FSPH_E_ARG;
   }
   memcpy(dst, src, b);
+  return 0;
+}
+
+/* Overwrites one field (fsph_set_field's counterpart). */
+int fo_set_field(fo_sys *s, int field, const double *src) {
+  double *dst = NULL;
+  switch (field) {
+    case FSPH_F_U: dst = s->u; break;
+    case FSPH_F_VX: dst = s->vx; break;
+    case FSPH_F_VY: dst = s->vy; break;
+    case FSPH_F_VZ: dst = s->vz; break;
+    case FSPH_F_E: dst = s->e; break;
+    case FSPH_F_ALPHA: dst = s->alpha; break;
+    case FSPH_F_DIVV: dst = s->divv; break;
+    default: return FSPH_E_ARG;
+  }
+  memcpy(dst, src, (size_t)s->n * 8);
   return 0;
 }
 
@@ -849,6 +876,158 @@ int fo_eos_init(fo_sys *s) {
   return 0;
 }
 
+/* set_total_energy (default_physics.h:328-335) and recover_internal_energy
+   (default_physics.h:127-145) with no external potential (epot = 0): e =
+   (ekin + u) + 0 and u = (e - ekin) - 0, ekin = .5 (v.v) summed in
+   component order. */
+static double kinetic(const fo_sys *s, int64_t a) {
+  double vv = 0;
+  vv += s->vx[a] * s->vx[a]; vv += s->vy[a] * s->vy[a];
+  vv += s->vz[a] * s->vz[a];
+  return .5 * vv;
+}
+
+static void set_total_energy(fo_sys *s) {
+  for (int64_t a = 0; a < s->n; ++a)
+    s->e[a] = kinetic(s, a) + s->u[a] + 0.0;
+}
+
+int fo_set_total_energy(fo_sys *s) {
+  if (!s->p.thermokinetic) return FSPH_E_ARG;
+  set_total_energy(s);
+  return 0;
+}
+
+/* The reference stops the run when the recovered energy is not positive;
+   here it is stored as it is and the state check reports it
+   (fo_check_state: u <= 0 in the thermokinetic formulation). */
+static void recover_internal_energy(fo_sys *s, int64_t a) {
+  s->u[a] = s->e[a] - kinetic(s, a) - 0.0;
+}
+
+/* compute_divv (default_physics.h:274-300), called at the end of the
+   density functor with the Cullen-Dehnen viscosity (:318-319): the
+   reference sums m_b (v_a . grad W_ab), with the particle's own velocity,
+   and keeps d(div v)/dt against the previous step's value. */
+static void compute_divv(fo_sys *s, int64_t a) {
+  const fsph_params *p = &s->p;
+  double div_v = 0.0;
+  for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+    int64_t b = s->nidx[j];
+    double pos_ab[3] = {s->x[a] - s->x[b], s->y[a] - s->y[b],
+                        s->z[a] - s->z[b]};
+    double h_ab = .5 * (s->h[a] + s->h[b]);
+    double DiW[3];
+    fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
+    double d = 0;
+    d += s->vx[a] * DiW[0]; d += s->vy[a] * DiW[1]; d += s->vz[a] * DiW[2];
+    div_v += s->m[b] * d;
+  }
+  div_v /= s->rho[a];
+  s->ddivvdt[a] = (div_v - s->divv[a]) / s->dt;
+  s->divv[a] = div_v;
+}
+
+static double signnum(double x) { /* viscosity.h:63-67 */
+  if (x > 0.0) return 1.0;
+  if (x < 0.0) return -1.0;
+  return x;
+}
+
+/* compute_alpha (viscosity.h:318-356) with A_trigger (:289-306) and
+   compute_xi (:188-274): the Cullen-Dehnen viscosity switch.  Reads only
+   the particle's own alpha, so the visiting order does not matter. */
+int fo_compute_alpha(fo_sys *s) {
+  if (!s->have_tree || s->p.viscosity != FSPH_VISC_CULLEN) return FSPH_E_ARG;
+  const fsph_params *p = &s->p;
+  int64_t n = s->n;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    const double c_a = s->c[a], h_a = s->h[a], alpha_a = s->alpha[a],
+                 rho_a = s->rho[a];
+    const double pa[3] = {s->x[a], s->y[a], s->z[a]};
+    const double va[3] = {s->vx[a], s->vy[a], s->vz[a]};
+    double vsig = 0.0;
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double d[3] = {pa[0] - s->x[b], pa[1] - s->y[b], pa[2] - s->z[b]};
+      double dd = 0;
+      dd += d[0] * d[0]; dd += d[1] * d[1]; dd += d[2] * d[2];
+      double dist = sqrt(dd);
+      double nab[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
+      double vab[3] = {va[0] - s->vx[b], va[1] - s->vy[b], va[2] - s->vz[b]};
+      double cab = SMAX(c_a, s->c[b]);
+      double dot = 0;
+      dot += vab[0] * nab[0]; dot += vab[1] * nab[1]; dot += vab[2] * nab[2];
+      vsig = SMAX(vsig, cab - SMIN(dot, 0.0));
+    }
+    const double div_v = s->divv[a];
+    /* A_trigger */
+    const double trig = SMAX(-s->ddivvdt[a], 0.0);
+    /* compute_xi: R_a and the velocity gradient */
+    double R_a = 0.0, gradV[9];
+    for (int k = 0; k < 9; ++k) gradV[k] = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int jj = 0; jj < 3; ++jj) {
+        for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+          int64_t b = s->nidx[j];
+          double pos_ab[3] = {pa[0] - s->x[b], pa[1] - s->y[b],
+                              pa[2] - s->z[b]};
+          double vab[3] = {va[0] - s->vx[b], va[1] - s->vy[b],
+                           va[2] - s->vz[b]};
+          double DiW[3];
+          fo_kernel_gradient(p->kernel, pos_ab, .5 * (h_a + s->h[b]), DiW);
+          gradV[3 * i + jj] += s->m[b] * (-vab[i]) * DiW[jj];
+        }
+        gradV[3 * i + jj] /= rho_a;
+      }
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double d[3] = {pa[0] - s->x[b], pa[1] - s->y[b], pa[2] - s->z[b]};
+      double dd = 0;
+      dd += d[0] * d[0]; dd += d[1] * d[1]; dd += d[2] * d[2];
+      double Wab = fo_kernel(p->kernel, sqrt(dd), .5 * (h_a + s->h[b]));
+      R_a += signnum(s->divv[b]) * s->m[b] * Wab;
+    }
+    R_a /= rho_a;
+    double traceSS = 0.0, SymT[9];
+    for (int i = 0; i < 3; ++i) {
+      for (int jj = 0; jj < 3; ++jj)
+        SymT[3 * i + jj] = 0.5 * (gradV[3 * i + jj] + gradV[3 * jj + i]);
+      SymT[3 * i + i] -= div_v / 3;
+      for (int jj = 0; jj < 3; ++jj)
+        traceSS += SymT[3 * i + jj] * SymT[3 * i + jj];
+    }
+    const double q = 1.0 - R_a;
+    const double t = 2.0 * (q * q * q * q) * div_v;
+    const double res = t * t;
+    const double xi = (res < 1e-16) ? 0.0 : res / (res + traceSS);
+    const double A = xi * trig;
+    const double vh = vsig / h_a;
+    const double alpha_loc =
+        p->visc_alpha_max * A / (p->visc_delta * (vh * vh) + A);
+    if (alpha_a <= alpha_loc) {
+      s->alpha[a] = alpha_loc;
+    } else {
+      const double decayt = h_a / (2.0 * p->visc_l * vsig);
+      const double dalphadt = (alpha_a - alpha_loc) / decayt;
+      s->alpha[a] = alpha_a * exp(-dalphadt * s->dt);
+    }
+  }
+  return 0;
+}
+
+/* initialize_alpha (viscosity.h:164-171) */
+static void initialize_alpha(fo_sys *s) {
+  for (int64_t a = 0; a < s->n; ++a)
+    s->alpha[a] = s->p.viscosity == FSPH_VISC_CULLEN ? 0.0 : s->p.visc_alpha;
+}
+
+int fo_initialize_alpha(fo_sys *s) {
+  initialize_alpha(s);
+  return 0;
+}
+
 /* Particles are handed to the functor in descending sorted index (groups
    and their entities are emitted by a stack DFS that pops digit 7 first,
    tree_topology.h:176-191, 222-235, 292-310), so when particle a computes
@@ -876,6 +1055,8 @@ int fo_density_eos(fo_sys *s) {
       rho_a += s->m[b] * Wab;
     }
     s->rho[a] = rho_a;
+    if (p->thermokinetic && p->evolve_internal_energy)
+      recover_internal_energy(s, a);
     eos_pressure_soundspeed(p, rho_a, s->u[a], s->kad[a], &s->P[a], &s->c[a]);
   }
 #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
@@ -899,6 +1080,7 @@ int fo_density_eos(fo_sys *s) {
       vsig = SMAX(vsig, cab - SMIN(dot, 0.0));
     }
     s->vsig[a] = vsig;
+    if (p->viscosity == FSPH_VISC_CULLEN) compute_divv(s, a);
   }
   free(c_old);
   return 0;
@@ -916,8 +1098,12 @@ static double visc_mu(double h_ab, const double v[3], const double r[3],
   return result * (dot < 0.0);
 }
 
-static double visc_pi(const fsph_params *p, double rho_ab, double c_ab,
-                      double mu_ab) {
+/* viscosity_function<visc_constant> (viscosity.h:128-137; alpha_ab is
+   ignored) or <visc_cullen> (viscosity.h:151-155) */
+static double visc_pi(const fsph_params *p, double alpha_ab, double rho_ab,
+                      double c_ab, double mu_ab) {
+  if (p->viscosity == FSPH_VISC_CULLEN)
+    return -alpha_ab * (c_ab - 2.0 * mu_ab) * mu_ab / rho_ab;
   return (-p->visc_alpha * c_ab + p->visc_beta * mu_ab) * mu_ab / rho_ab;
 }
 
@@ -943,7 +1129,8 @@ int fo_acceleration(fo_sys *s) {
       double mu_ab = visc_mu(h_ab, v12_ab, pos_ab, p->visc_eps);
       double rho_ab = .5 * (rho_a + s->rho[b]);
       double c_ab = .5 * (c_a + s->c[b]);
-      double Pi = visc_pi(p, rho_ab, c_ab, mu_ab);
+      double Pi = visc_pi(p, .5 * (s->alpha[a] + s->alpha[b]), rho_ab, c_ab,
+                          mu_ab);
       double DiW[3];
       fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
       double Prho2_b = s->P[b] / (s->rho[b] * s->rho[b]);
@@ -984,7 +1171,8 @@ int fo_dudt(fo_sys *s) {
       double mu_ab = visc_mu(h_ab, v12_ab, pos_ab, p->visc_eps);
       double rho_ab = .5 * (rho_a + s->rho[b]);
       double c_ab = .5 * (c_a + s->c[b]);
-      double Pi = visc_pi(p, rho_ab, c_ab, mu_ab);
+      double Pi = visc_pi(p, .5 * (s->alpha[a] + s->alpha[b]), rho_ab, c_ab,
+                          mu_ab);
       double DiW[3];
       fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
       double vdw = 0;
@@ -1004,6 +1192,64 @@ int fo_dudt(fo_sys *s) {
 }
 
 /* ------------------------------------------------------------------ */
+/* de/dt of the thermokinetic formulation, default_physics.h:503-566,   */
+/* then recompute_pressure_soundspeed_thermokinetic, :170-180            */
+/* ------------------------------------------------------------------ */
+int fo_dedt(fo_sys *s) {
+  if (!s->have_tree || !s->p.thermokinetic) return FSPH_E_ARG;
+  const fsph_params *p = &s->p;
+  int64_t n = s->n;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 256)
+  for (int64_t a = 0; a < n; ++a) {
+    double h_a = s->h[a], rho_a = s->rho[a], P_a = s->P[a], c_a = s->c[a];
+    double vel_a[3] = {s->vx[a], s->vy[a], s->vz[a]};
+    double ga_a[3] = {s->gx[a], s->gy[a], s->gz[a]};
+    double gv = 0;
+    gv += ga_a[0] * vel_a[0]; gv += ga_a[1] * vel_a[1];
+    gv += ga_a[2] * vel_a[2];
+    double Prho2_a = P_a / (rho_a * rho_a);
+    double dedt = 0;
+    for (int64_t j = s->noff[a]; j < s->noff[a + 1]; ++j) {
+      int64_t b = s->nidx[j];
+      double pos_ab[3] = {s->x[a] - s->x[b], s->y[a] - s->y[b],
+                          s->z[a] - s->z[b]};
+      int differ = (s->x[b] != s->x[a]) || (s->y[b] != s->y[a]) ||
+                   (s->z[b] != s->z[a]);
+      double m_b = s->m[b] * differ;
+      double v12_ab[3] = {s->vhx[a] - s->vhx[b], s->vhy[a] - s->vhy[b],
+                          s->vhz[a] - s->vhz[b]};
+      double vel_b[3] = {s->vx[b], s->vy[b], s->vz[b]};
+      double h_ab = .5 * (h_a + s->h[b]);
+      double mu_ab = visc_mu(h_ab, v12_ab, pos_ab, p->visc_eps);
+      double rho_ab = .5 * (rho_a + s->rho[b]);
+      double c_ab = .5 * (c_a + s->c[b]);
+      double Pi = visc_pi(p, .5 * (s->alpha[a] + s->alpha[b]), rho_ab, c_ab,
+                          mu_ab);
+      double DiW[3];
+      fo_kernel_gradient(p->kernel, pos_ab, h_ab, DiW);
+      double va = 0, vb = 0;
+      va += vel_a[0] * DiW[0]; va += vel_a[1] * DiW[1]; va += vel_a[2] * DiW[2];
+      vb += vel_b[0] * DiW[0]; vb += vel_b[1] * DiW[1]; vb += vel_b[2] * DiW[2];
+      double Prho2_b = s->P[b] / (s->rho[b] * s->rho[b]);
+      dedt -= m_b * (Prho2_a * vb + va * Prho2_b + .5 * Pi * (vb + va));
+    }
+    dedt += gv;
+    s->dedt[a] = dedt;
+  }
+  /* recompute_pressure_soundspeed_thermokinetic: u recovered from e and
+     the current v, then P, c at u + dt/2 (de/dt - v.a) */
+  for (int64_t a = 0; a < n; ++a) {
+    recover_internal_energy(s, a);
+    double va = 0;
+    va += s->vx[a] * s->ax[a]; va += s->vy[a] * s->ay[a];
+    va += s->vz[a] * s->az[a];
+    double uh = s->u[a] + 0.5 * s->dt * (s->dedt[a] - va);
+    eos_pressure_soundspeed(p, s->rho[a], uh, s->kad[a], &s->P[a], &s->c[a]);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* A13: leapfrog, integration.h:53-110                                  */
 /* ------------------------------------------------------------------ */
 int fo_integrate(fo_sys *s, uint32_t what) {
@@ -1011,6 +1257,8 @@ int fo_integrate(fo_sys *s, uint32_t what) {
   double dt = s->dt;
   if (what & FSPH_KICK_U)
     for (int64_t i = 0; i < n; ++i) s->u[i] = s->u[i] + 0.5 * dt * s->dudt[i];
+  if (what & FSPH_KICK_E)
+    for (int64_t i = 0; i < n; ++i) s->e[i] = s->e[i] + 0.5 * dt * s->dedt[i];
   if (what & FSPH_KICK_V)
     for (int64_t i = 0; i < n; ++i) {
       double f = 0.5 * dt;
@@ -1214,12 +1462,17 @@ int fo_check_state(fo_sys *s) {
   for (int64_t i = 0; i < s->n; ++i) {
     for (int k = 0; k < NDBL; ++k) {
       double *a = *f[k];
-      if (a == s->c || a == s->h ||
-          (a == s->kad && s->p.eos != FSPH_EOS_POLYTROPIC))
-        continue; /* the reference checks neither c nor h */
+      if (a == s->c || a == s->h || a == s->alpha || a == s->divv ||
+          a == s->ddivvdt ||
+          (a == s->kad && s->p.eos != FSPH_EOS_POLYTROPIC) ||
+          (a == s->u && !s->p.evolve_internal_energy))
+        continue; /* fields check_nans does not look at */
       if (a[i] != a[i]) nan = 1;
     }
-    if (s->rho[i] < 0 || s->P[i] < 0 || s->u[i] < 0) neg = 1;
+    const int ev = s->p.evolve_internal_energy;
+    if (s->rho[i] < 0 || s->P[i] < 0 || (ev && s->u[i] < 0)) neg = 1;
+    /* recover_internal_energy stops at u <= 0 (default_physics.h:134-143) */
+    if (ev && s->p.thermokinetic && !(s->u[i] > 0)) neg = 1;
   }
   return nan ? FSPH_E_NAN : (neg ? FSPH_E_NEG : 0);
 }
@@ -1231,32 +1484,41 @@ int fo_set_dt(fo_sys *s, double dt) {
 
 /* One loop iteration of app/drivers/hydro/main_driver.cc:98-277 (and of
    app/drivers/newtonian/main_driver.cc when gravity is on), with
-   sph_variable_h = yes, thermokinetic_formulation = no, constant
-   viscosity, no relaxation, pressure_updates_number = 1. */
+   sph_variable_h = yes, thermokinetic_formulation and sph_viscosity as the
+   parameters say, no relaxation, pressure_updates_number = 1. */
 int fo_step(fo_sys *s) {
   int rc;
+  const int tk = s->p.thermokinetic;
+  const int cd = s->p.viscosity == FSPH_VISC_CULLEN;
+  const int ev = s->p.evolve_internal_energy;
   if (!s->initial_done) {
     if ((rc = fo_update_iteration(s))) return rc;
     fo_eos_init(s);
+    if (tk) set_total_energy(s);
+    if (cd) initialize_alpha(s);
     if ((rc = fo_density_eos(s))) return rc;
     fo_integrate(s, FSPH_SAVE_VHALF);
+    if (cd && (rc = fo_compute_alpha(s))) return rc;
     if ((rc = fo_acceleration(s))) return rc;
     if (s->p.gravity && (rc = fo_gravity(s))) return rc;
     fo_timestep(s, NULL);
-    if ((rc = fo_dudt(s))) return rc;
+    if (ev && (rc = tk ? fo_dedt(s) : fo_dudt(s))) return rc;
     s->initial_done = 1;
   } else {
-    fo_integrate(s, FSPH_KICK_U);
+    if (ev) fo_integrate(s, tk ? FSPH_KICK_E : FSPH_KICK_U);
     fo_integrate(s, FSPH_KICK_V);
     fo_integrate(s, FSPH_SAVE_VHALF);
     fo_integrate(s, FSPH_DRIFT);
     if ((rc = fo_update_iteration(s))) return rc;
     if ((rc = fo_density_eos(s))) return rc;
+    if (cd && (rc = fo_compute_alpha(s))) return rc;
     if ((rc = fo_acceleration(s))) return rc;
     if (s->p.gravity && (rc = fo_gravity(s))) return rc;
     fo_integrate(s, FSPH_KICK_V);
-    if ((rc = fo_dudt(s))) return rc;
-    fo_integrate(s, FSPH_KICK_U);
+    if (ev) {
+      if ((rc = tk ? fo_dedt(s) : fo_dudt(s))) return rc;
+      fo_integrate(s, tk ? FSPH_KICK_E : FSPH_KICK_U);
+    }
   }
   fo_update_h(s);
   fo_timestep(s, NULL);

@@ -25,6 +25,7 @@ void fo_set_threads(int n);
 int fo_upload(fo_sys *s, const fsph_particles *parts);
 int fo_count(const fo_sys *s, int64_t *n_total, int64_t *n_real);
 int fo_download(fo_sys *s, int field, void *dst);
+int fo_set_field(fo_sys *s, int field, const double *src);
 int fo_update_iteration(fo_sys *s);
 int fo_eos_init(fo_sys *s);
 int fo_density_eos(fo_sys *s);
@@ -32,6 +33,10 @@ int fo_acceleration(fo_sys *s);
 int fo_gravity(fo_sys *s);
 int fo_gravity_engine(fo_sys *s);
 int fo_dudt(fo_sys *s);
+int fo_dedt(fo_sys *s);
+int fo_set_total_energy(fo_sys *s);
+int fo_compute_alpha(fo_sys *s);
+int fo_initialize_alpha(fo_sys *s);
 int fo_integrate(fo_sys *s, uint32_t what);
 int fo_update_h(fo_sys *s);
 int fo_timestep(fo_sys *s, double *dt);

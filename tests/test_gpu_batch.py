@@ -217,3 +217,104 @@ def test_graph_replay_matches_eager_every_step(precision, tuning):
             assert eager.instance_time(i) == graph.instance_time(i)
     eager.close()
     graph.close()
+
+
+def test_loguniform_64_batch_against_oracle_and_invariants(oracle_mod):
+    """The benchmark batch at its full size: 64 instances with seeded
+    log-uniform particle counts in [8k, 200k] (about 3.2M particles), each a
+    periodic Sedov blast or an open Sod tube by a seeded coin, graph replay
+    of the step tail, initial iteration + 2 steps.  The smallest blast and
+    the smallest tube are checked against the FP64 oracle run on that
+    instance alone (FP32 bar of the single-simulation tests); every instance
+    against invariants that hold at any size: ids a permutation of the
+    instance's particles and copies (no copies for a tube), sorted keys,
+    interior density 1 of a blast's unit-density lattice, and vanishing
+    total momentum change (pairwise antisymmetric forces: sum m a = 0 over
+    the instance)."""
+    sets, prm, periodic = generators.mixed_batch()
+    assert len(sets) == 64 and 0 < sum(periodic) < 64
+    n_all = sum(len(p["x"]) for p in sets)
+    assert 2_000_000 < n_all < 5_000_000
+    print(f"64-instance batch, {n_all} particles, {sum(periodic)} blasts",
+          flush=True)
+    b = BatchSystem(_abi.default_params(**prm), 64)
+    b.upload(sets)
+    for i, per in enumerate(periodic):
+        b.instance_set_periodic(i, per)
+    b.step()
+    b.enable_graph(True)
+    for _ in range(2):
+        b.step()
+    for i, parts in enumerate(sets):
+        n_tot, n_real = b.instance_count(i)
+        assert n_real == len(parts["x"])
+        if not periodic[i]:
+            assert n_tot == n_real, i
+        ids = b.instance_get(i, "id")
+        assert np.array_equal(np.sort(ids), np.arange(n_tot)), i
+        key = b.instance_get(i, "key")
+        assert np.all(key[1:] > key[:-1]), i
+        real = ids < n_real
+        if periodic[i]:
+            rho = b.instance_get(i, "rho")
+            assert abs(np.median(rho[real]) - 1.0) < 0.02, i
+        m = b.instance_get(i, "m")
+        A = np.stack([b.instance_get(i, f) for f in ("ax", "ay", "az")], 1)
+        mom = np.abs((m[:, None] * A).sum(0)).max()
+        assert mom <= 1e-5 * (m[:, None] * np.abs(A)).sum(), i
+    sizes = np.array([len(p["x"]) for p in sets])
+    per = np.array(periodic)
+    picks = [int(np.flatnonzero(per)[np.argmin(sizes[per])]),
+             int(np.flatnonzero(~per)[np.argmin(sizes[~per])])]
+    for i in picks:
+        print(f"instance {i}: {sizes[i]} particles, periodic {periodic[i]}, "
+              f"vs the oracle", flush=True)
+        q = dict(prm)
+        if not periodic[i]:
+            q["periodic"] = (0, 0, 0)
+        o = oracle_mod.OracleSystem(_abi.default_params(**q), threads=8)
+        o.upload(sets[i])
+        for _ in range(3):
+            o.step()
+        assert b.instance_count(i) == o.count()
+        dtg, dto = b.instance_time(i)[1], o.time()[1]
+        assert abs(dtg - dto) <= 1e-4 * dto
+        # real particles by id: FP32 and FP64 positions differ by ~1e-10,
+        # which can move a particle across a Morton cell boundary, so the
+        # sorted order (and the numbering of the periodic copies that
+        # follows it) need not agree
+        n_real = len(sets[i]["x"])
+        ig, io = b.instance_get(i, "id"), o.get("id")
+        rg, ro = ig < n_real, io < n_real
+        og, oo = np.argsort(ig[rg]), np.argsort(io[ro])
+        assert np.array_equal(ig[rg][og], io[ro][oo])
+        for f in ("x", "y", "z", "u", "rho"):
+            g = b.instance_get(i, f)[rg][og]
+            r = o.get(f)[ro][oo]
+            tol = 1e-9 if f in "xyz" else 1e-4 * np.abs(r).max()
+            assert np.abs(g - r).max() <= tol, (i, f)
+        o.close()
+    b.close()
+
+
+def test_open_instance_in_a_periodic_batch_equals_its_own_run():
+    """An instance switched to open boundaries in a periodic batch evolves
+    exactly as a context of its own without periodic boundaries, and the
+    periodic instance beside it as its own periodic context."""
+    tube = generators.sod_tube(j=3)
+    blast, prm = generators.sedov(n_side=12)
+    b = BatchSystem(_abi.default_params(**prm), 2)
+    b.upload([blast, tube])
+    b.instance_set_periodic(1, False)
+    for _ in range(3):
+        b.step()
+    for i, (parts, per) in enumerate(((blast, True), (tube, False))):
+        q = dict(prm) if per else dict(prm, periodic=(0, 0, 0))
+        ref = _single(parts, q, 3)
+        ids = b.instance_get(i, "id")
+        o = np.argsort(ids)
+        assert np.array_equal(ids[o], ref["id"]), i
+        for f in FIELDS:
+            assert np.array_equal(b.instance_get(i, f)[o], ref[f]), (i, f)
+        assert b.instance_time(i) == ref["time"], i
+    b.close()

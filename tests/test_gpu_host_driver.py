@@ -49,14 +49,19 @@ def sod_inputs(nl=30):
     return parts, prm
 
 
-def test_cpp_driver_equals_fsph_step():
+@pytest.mark.parametrize("thermokinetic", [1, 0])
+def test_cpp_driver_equals_fsph_step(thermokinetic):
+    """Both energy equations: the thermokinetic one (the reference's
+    default, set_total_energy / compute_dedt / leapfrog_kick_e) and the
+    internal-energy one (compute_dudt / leapfrog_kick_u)."""
     steps = 4
-    print("running", DRIVER, flush=True)
-    out = subprocess.run([DRIVER, str(steps), "1"], capture_output=True,
-                         text=True, timeout=120)
+    print("running", DRIVER, "thermokinetic", thermokinetic, flush=True)
+    out = subprocess.run([DRIVER, str(steps), "1", str(thermokinetic)],
+                         capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     d = json.loads(out.stdout.strip().splitlines()[-1])
     parts, prm = sod_inputs()
+    prm["thermokinetic"] = thermokinetic
     g = BodySystem(_abi.default_params(**prm))
     g.upload(parts)
     for _ in range(steps + 1):
@@ -73,3 +78,36 @@ def test_cpp_driver_equals_fsph_step():
     assert d["mass"] == pytest.approx(parts["m"].sum(), rel=1e-14)
     v = g.get("vx")
     assert abs(d["px"]) < 1e-10 * (g.get("m") * np.abs(v)).sum() + 1e-15
+
+
+FUNCTOR_CHECK = os.path.join(ROOT, "flecsph_amd", "fsph_functor_check")
+
+
+def test_host_functor_through_apply_in_smoothinglength():
+    """A functor with no device form (the reference's compute_divv,
+    default_physics.h:274-300, written against fsph::body) runs through
+    body_system::apply_in_smoothinglength on host copies over the device's
+    neighbour lists, and its result is written back to the device.  With
+    one smoothing length for all particles the lists hold exactly the pairs
+    within h, so an all-pairs numpy sum must give the same divergence."""
+    print("running", FUNCTOR_CHECK, flush=True)
+    out = subprocess.run([FUNCTOR_CHECK, "10"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    d = {k: np.array(v) for k, v in json.loads(out.stdout).items()}
+    X = np.stack([d["x"], d["y"], d["z"]], 1)
+    V = np.stack([d["vx"], d["vy"], d["vz"]], 1)
+    h, m, rho = d["h"], d["m"], d["rho"]
+    want = np.zeros(len(h))
+    for a in range(len(h)):
+        rab = X[a] - X
+        r = np.sqrt((rab ** 2).sum(1))
+        hab = 0.5 * (h[a] + h)
+        q = 2 * r / hab
+        dW = np.where(q < 1, -3 * q + 2.25 * q * q,
+                      np.where(q < 2, -0.75 * (2 - q) ** 2, 0.0))
+        F = np.where(r > 0, 2 * (8 / math.pi) / hab ** 4 * dW /
+                     np.maximum(r, 1e-300), 0.0)
+        want[a] = (m * (rab @ V[a]) * F).sum() / rho[a]
+    assert np.abs(want).max() > 0
+    assert np.abs(d["divv"] - want).max() <= 1e-12 * np.abs(want).max()

@@ -43,8 +43,9 @@ def _oracle_newton(oracle_mod, prm, state, h_old, ids):
 @pytest.mark.parametrize("case", ["collapse", "sedov"])
 def test_learned_h_is_the_newton_root(oracle_mod, case):
     if case == "collapse":
+        # the collapse with its self-gravity (FMM, theta 0.5, quadrupole)
         parts, prm = generators.collapse(n=40000)
-        prm.update(gravity=_abi.GRAV_NONE)
+        assert prm["gravity"] == _abi.GRAV_FMM
     else:
         parts, prm = generators.sedov(n_side=24)
     prm.update(h_mode=_abi.H_LEARNED, learn_steps=5)
@@ -80,8 +81,40 @@ def test_learned_h_is_the_newton_root(oracle_mod, case):
         it_plain.append(ref["hiter"].mean())
         checked += 1
     assert checked >= 3
-    # the learned start needs no more Newton steps than the plain one
+    print("%s: mean Newton iterations learned %.3f plain %.3f" % (
+        case, np.mean(it_learned), np.mean(it_plain)))
+    # the learned start never needs more Newton steps than the plain one
     assert np.mean(it_learned) <= np.mean(it_plain) + 0.05
+
+
+def test_learned_h_gathers_fewer_candidates():
+    """What the learned form gains over plain Newton-h: the same steps of
+    the 40k collapse with self-gravity, run once with Newton-h and once with
+    learned-h (5 training updates).  After training, each learned update
+    solves inside a gather radius fitted to the predictor's error instead
+    of h_search h, so its gather tiles hold far fewer candidate words (the
+    pair tests of the solve) for the same roots (the test above checks the
+    roots against the oracle)."""
+    parts, prm = generators.collapse(n=40000)
+    out = {}
+    for mode in (_abi.H_NEWTON, _abi.H_LEARNED):
+        g = BodySystem(_abi.default_params(**dict(prm, h_mode=mode,
+                                                   learn_steps=5)))
+        g.upload(parts)
+        rec = []
+        for k in range(10):
+            g.step()
+            rec.append(g.h_gather())
+        out[mode] = rec
+        g.close()
+    hs = _abi.default_params(**prm).h_search
+    for k in range(7, 10):
+        (kn, wn), (kl, wl) = out[_abi.H_NEWTON][k], out[_abi.H_LEARNED][k]
+        print("step %d: gather radius %.3f -> %.3f h, mask words %d -> %d "
+              "(%.2fx)" % (k, kn, kl, wn, wl, wn / max(wl, 1)), flush=True)
+        assert kn == pytest.approx(hs)
+        assert kl < kn
+        assert wl < 0.7 * wn
 
 
 def test_learned_h_statistics_before_and_after_training():
