@@ -162,6 +162,7 @@ PROTOTYPES = {
     "time": (C.c_int, [_P, _PI64, _PD, _PD]),
     "set_dt": (C.c_int, [_P, C.c_double]),
     "reductions": (C.c_int, [_P, _PD]),
+    "reductions_noalloc": (C.c_int, [_P, _PD]),
     "dedt": (C.c_int, [_P]),
     "h_gather": (C.c_int, [_P, _PD, C.POINTER(C.c_int64)]),
     "instance_set_periodic": (C.c_int, [_P, C.c_int, C.c_int]),

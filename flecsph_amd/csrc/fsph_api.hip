@@ -380,6 +380,7 @@ int fsph_create(const fsph_params *p, int device, fsph_ctx **out) {
   c->device = device;
   c->rsize = p->precision == FSPH_PREC_FP64 ? 8 : 4;
   if (hipSetDevice(device) != hipSuccess ||
+      fsph::scratch_setup(device) != FSPH_OK ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) !=
           hipSuccess) {
     delete c;
@@ -808,6 +809,13 @@ int fsph_reductions(fsph_ctx *ctx, double out[10]) {
   NOT_BATCH;
   if (!out) return FSPH_E_ARG;
   return fsph::stage_reductions(c, out);
+}
+
+int fsph_reductions_noalloc(fsph_ctx *ctx, double out[10]) {
+  CTX_OR_FAIL;
+  NOT_BATCH;
+  if (!out) return FSPH_E_ARG;
+  return fsph::stage_reductions_noalloc(c, out);
 }
 
 int fsph_get_range(fsph_ctx *ctx, double lo[3], double hi[3]) {

@@ -371,6 +371,11 @@ int check_result(Ctx *c);
 int stage_reductions(Ctx *c, double out[10]);  // k_step.hip
 // set_total_energy of the thermokinetic formulation (k_step.hip)
 int stage_set_total_energy(Ctx *c);
+// Sets up, once per device and with that device current, the device address
+// of the allocation-free reductions' scratch array (k_step.hip).
+int scratch_setup(int device);
+// fsph_reductions with its partial sums in that scratch array.
+int stage_reductions_noalloc(Ctx *c, double out[10]);
 // device pointer to 2.5 max h (the periodic halo), no host wait
 int stage_halo(Ctx *c, const double **halo);  // k_step.hip
 int read_mail(Ctx *c, int words);             // fsph_api.hip

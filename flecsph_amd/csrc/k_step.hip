@@ -6,6 +6,7 @@
 // the reference's arithmetic operation by operation.
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 
 #include "fsph_internal.h"
 
@@ -514,6 +515,53 @@ template <class T> int red_impl(Ctx *c, double out[10]) {
 }
 
 }  // namespace
+
+// ---- allocation-free reductions --------------------------------------------
+// fsph_reductions_noalloc keeps its partial sums in this file-scope array
+// instead of the context's growable buffer: every device the library runs
+// on holds one copy (kRedRows + 1 rows of 10 doubles, 41,040 bytes) until
+// the process ends.  In host code the name `scratch` is a stand-in for the
+// symbol, not a device address; the address of each device's copy comes
+// from hipGetSymbolAddress in scratch_setup and is what the kernels get.
+static __device__ __align__(256) double scratch[(kRedRows + 1) * 10];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+double *scratch_addr[kScratchDevices];  // per-device address of `scratch`
+}  // namespace
+
+int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) return FSPH_E_NODEV;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_addr[device]) return FSPH_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return FSPH_E_HIP;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+    return FSPH_E_HIP;
+  scratch_addr[device] = static_cast<double *>(p);
+  return FSPH_OK;
+}
+
+int stage_reductions_noalloc(Ctx *c, double out[10]) {
+  double *part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (c->device >= 0 && c->device < kScratchDevices)
+      part = scratch_addr[c->device];
+  }
+  if (!part)
+    return fail(c, FSPH_E_HIP, "scratch of device %d was not set up",
+                c->device);
+  const int nb = red_blocks(c->n);
+  if (nb > kRedRows)
+    return fail(c, FSPH_E_CAP,
+                "reduction needs %d partial rows; the scratch holds %d", nb,
+                kRedRows);
+  return c->rsize == 4 ? red_run<float>(c, part, nb, out)
+                       : red_run<double>(c, part, nb, out);
+}
 
 int stage_integrate(Ctx *c, uint32_t what) {
   return c->rsize == 4 ? integrate_impl<float>(c, what)

@@ -203,6 +203,11 @@ class _System:
     def set_fmm_capacity(self, cap_q, cap_m2l, cap_p2p):
         self._call("set_fmm_capacity", int(cap_q), int(cap_m2l), int(cap_p2p))
 
+    def reductions_noalloc(self):
+        out = np.zeros(10)
+        self._call("reductions_noalloc", _dptr(out))
+        return out
+
     def range(self):
         lo, hi = np.zeros(3), np.zeros(3)
         self._call("get_range", _dptr(lo), _dptr(hi))

@@ -349,6 +349,18 @@ int fsph_set_dt(fsph_ctx *ctx, double dt);
    out[7..9] angular momentum. */
 int fsph_reductions(fsph_ctx *ctx, double out[10]);
 
+/* fsph_reductions that allocates nothing.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The partial sums live in a static device array of the library, one copy
+   per device, set up by fsph_create: at most 512 partial rows of 10
+   doubles (41,040 bytes), which serves any particle count (the reduction
+   never launches more than 512 workgroups); a call that would need more
+   returns FSPH_E_CAP.  The result equals fsph_reductions bit for bit.
+   Every call on one device shares that array: calls that overlap on one
+   GPU (contexts of one device driven from several host threads) race. */
+int fsph_reductions_noalloc(fsph_ctx *ctx, double out[10]);
+
 /* ---- parity/introspection ---------------------------------------------- */
 
 /* Range of the last update_iteration: lo[3], hi[3]. */
