@@ -94,6 +94,7 @@ VALU_FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
 # commands.  A run with another tuning reports no traffic.
 TRAFFIC = {
     ("sedov64", "accel"): ((64, 64), 603_936_237),
+    ("polytrope1m", "p2p"): ((64, 64), 984_486_505),
 }
 
 # Side workloads measured after the headline one.
