@@ -31,6 +31,8 @@ PREC_FP32 = 0
 PREC_FP64 = 1
 GRAV_NONE = 0
 GRAV_FMM = 1
+FMM_WALK = 0
+FMM_PAIRS = 1
 
 H_CLOSED = 0
 H_NEWTON = 1
@@ -174,6 +176,7 @@ PROTOTYPES = {
     "set_total_energy": (C.c_int, [_P]),
     "gravity_stats": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "set_fmm_capacity": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int64]),
+    "set_fmm_builder": (C.c_int, [_P, C.c_int, C.c_int]),
     "get_range": (C.c_int, [_P, _PD, _PD]),
     "get_tree": (C.c_int, [_P, _PI64, _PU64, _PI32, _PI32, _PI32, _PI64,
                            _PI32, _PI32]),

@@ -626,10 +626,15 @@ int fsph_gravity(fsph_ctx *ctx) {
   return FSPH_OK;
 }
 
-int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[6]) {
+int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[8]) {
   CTX_OR_FAIL;
   if (!out) return FSPH_E_ARG;
   return fsph::gravity_stats(c, out);
+}
+
+int fsph_set_fmm_builder(fsph_ctx *ctx, int builder, int stack_limit) {
+  CTX_OR_FAIL;
+  return fsph::set_fmm_builder(c, builder, stack_limit);
 }
 
 int fsph_set_fmm_capacity(fsph_ctx *ctx, int64_t cap_q, int64_t cap_m2l,

@@ -63,6 +63,8 @@ enum : int {
 // (sticky: every read-back that depends on a scan checks it).
 constexpr int kCheckWord = 4;
 constexpr int kScanWord = 6;
+// [kWalkWord]: overflow bits of the FMM sink-major walk
+constexpr int kWalkWord = 8;
 // mail_h slot that receives the state check's two words, and the one that
 // receives the scan word at the end of a step / with the group count
 constexpr int kMailCheck = kMail - 3;
@@ -239,10 +241,17 @@ struct Ctx {
     DBuf flag, pos, nodecell, partcell;
     DBuf cnode, cstart, ccount, cbucket, cchild, ccom, cmass, crad, cQ, cL;
     int64_t cap_q = 0, cap_m2l = 0, cap_p2p = 0;
-    int64_t n_m2l = 0, n_p2p = 0;
-    int levels = 0;  // levels of the last pair recursion
+    int64_t n_m2l = 0, n_p2p = 0;  // n_m2l < 0: per-cell counts in mcnt
+    int levels = 0;  // levels of the last pair recursion / depths walked
     DBuf qa, qb, m2l, p2p, ctr;
     DBuf m2l_off, m2l_src, p2p_off, p2p_src, cursor;
+    // sink-major walk: internal cells by depth, chunked list pool, per-cell
+    // chain heads and counts
+    int builder = FSPH_FMM_WALK;       // asked for
+    int used_builder = FSPH_FMM_WALK;  // of the last evaluation
+    int stack_limit = 0;               // 0: the full LDS stack
+    int64_t cap_chunks = 0, chunks_used = 0;
+    DBuf iflag, ipos, parents, lpool, lnext, llen, dhead, phead, pcnt, mcnt;
   } grav;
 
   // time stepping
@@ -355,7 +364,8 @@ int stage_compute_alpha(Ctx *c);       // k_sph.hip
 int stage_initialize_alpha(Ctx *c);    // k_step.hip
 int stage_gravity(Ctx *c);             // k_grav.hip
 void free_gravity(Ctx *c);             // k_grav.hip
-int gravity_stats(Ctx *c, int64_t out[6]);  // k_grav.hip
+int gravity_stats(Ctx *c, int64_t out[8]);  // k_grav.hip
+int set_fmm_builder(Ctx *c, int builder, int stack_limit);  // k_grav.hip
 int set_fmm_capacity(Ctx *c, int64_t cap_q, int64_t cap_m2l,
                      int64_t cap_p2p);      // k_grav.hip
 int stage_integrate(Ctx *c, uint32_t what);  // k_step.hip

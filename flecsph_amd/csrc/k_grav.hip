@@ -580,6 +580,306 @@ __global__ __launch_bounds__(kT) void k_l2l_single(
   l2l_one<T>(c, L, cell, nodecell[leaf_parent[p]]);
 }
 
+// ---------------------------------------------------------------------------
+// Sink-major walk: the pair recursion above, reorganised by sink.  Every
+// pair (A, B) of the recursion has A's parent pair (P, B) split on the sink
+// side, or (A, B') split on the source side, or is a sibling pair from
+// (P, P).  So a sink's pairs are: its siblings (itself included) and the
+// sources its parent split on the sink side (D(P)), refined on the source
+// side until each is accepted (M2L), summed directly (P2P), or split on
+// the sink side (into D(A), for A's children).  One wavefront walks one
+// sink, depth by depth (parents before children): a lane per candidate,
+// source-side splits on an LDS stack, M2L summed in registers and added to
+// the parent's shifted local expansion, so the sink's local expansion is
+// complete when its walk ends and no list is bucketed by sink.  P2P and
+// D(A) plus A's children go to chained 64-entry chunks of a list pool; the
+// P2P chains are then copied into a CSR list.  The pair set is the
+// recursion's, so the result is the same up to FP32 summation order.
+// ---------------------------------------------------------------------------
+constexpr int kWalkStack = 1024;  // LDS stack entries of a sink's walk
+constexpr int kChunk = 64;        // entries per list chunk
+constexpr int kWalkGrid = 8192;   // workgroups of a depth (grid-stride)
+// The pool is cut into kStripes stripes with a counter each (128 bytes
+// apart); workgroup b takes its chunks from stripe b % kStripes, so no
+// counter sees more than 1/kStripes of the allocations.
+constexpr int kStripes = 64, kStripeWords = 16;
+
+// flags word of the walk (ctx flags [kWalkWord]): kWalkPoolFull: a list
+// chunk beyond the pool's capacity was asked for (the walk is redone with
+// a larger pool); kWalkStackFull: a sink's stack would pass its limit (the
+// evaluation falls back to the pair recursion).
+constexpr int kWalkPoolFull = 1, kWalkStackFull = 2;
+
+struct ListPool {
+  int32_t *pool, *next, *len;  // chunk ch: pool[kChunk * ch + i], i < len
+  int64_t scap;                // chunks per stripe
+  unsigned long long *used;    // chunks handed out: used[kStripeWords * s]
+};
+
+template <class T> struct WalkArgs {
+  Cells c;
+  const T *Q;
+  T *L;
+  T G;
+  int order;
+  double theta;
+  const int32_t *parents;          // internal cells in depth order
+  const int64_t *plo, *phi;        // this depth's parents: [*plo, *phi)
+  int32_t root;                    // >= 0: the launch walks the root alone
+  ListPool l;
+  int32_t *dhead, *phead, *pcnt, *mcnt;
+  int stack_limit;
+  int *flags;
+  const int *scan_word;  // the parents' scan stalled: nothing is walked
+};
+
+__device__ inline uint64_t lanes_below() {
+  const int lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Appends v of every lane with want to buf (n entries, wave-uniform).
+__device__ inline void buf_append(bool want, int32_t v, int32_t *buf,
+                                  int &n) {
+  const uint64_t m = __ballot(want);
+  if (want) buf[n + __popcll(m & lanes_below())] = v;
+  n += __popcll(m);
+  __syncthreads();
+}
+
+// Moves the first `take` entries of buf (n entries) into a new chunk at
+// the head of the chain `head`, and the rest to the front of buf.
+__device__ inline void buf_flush(const ListPool &l, int32_t *buf, int &n,
+                                 int32_t &head, int take, int *flags) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stripe = blockIdx.x % kStripes;
+  unsigned long long k = 0;
+  if (lane == 0) k = atomicAdd(l.used + kStripeWords * stripe, 1ull);
+  k = __shfl(k, 0, 64);
+  const int64_t ch = stripe * l.scap + (int64_t)k;
+  if (k < (unsigned long long)l.scap) {
+    if (lane < take) l.pool[kChunk * ch + lane] = buf[lane];
+    if (lane == 0) {
+      l.next[ch] = head;
+      l.len[ch] = take;
+    }
+    head = (int32_t)ch;
+  } else if (lane == 0) {
+    atomicOr(flags, kWalkPoolFull);
+  }
+  const int rest = n - take;
+  const int32_t v = lane < rest ? buf[take + lane] : 0;
+  __syncthreads();
+  if (lane < rest) buf[lane] = v;
+  __syncthreads();
+  n = rest;
+}
+
+template <class T>
+__device__ void walk_sink(const WalkArgs<T> &a, int32_t A, int32_t P,
+                          int32_t *stk, int32_t *pbuf, int32_t *dbuf) {
+  const int lane = threadIdx.x & 63;
+  const Cells &c = a.c;
+  const double ax = c.com[3 * (int64_t)A], ay = c.com[3 * (int64_t)A + 1],
+               az = c.com[3 * (int64_t)A + 2];
+  const double ra = c.rad[A];
+  const bool ba = c.bucket[A] != 0;
+  const int limit = a.stack_limit < kWalkStack ? a.stack_limit : kWalkStack;
+  T Lk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int m2l = 0, np = 0, nd = 0, sp = 0, pc = 0;
+  int32_t ph = -1, dh = -1;
+  bool over = false;
+  // one candidate per lane (B < 0: none)
+  auto visit = [&](int32_t B) {
+    int kind = 0;  // 1 M2L, 2 P2P, 3 split the sink, 4 split the source
+    if (B == A) {
+      kind = ba ? 2 : 0;  // a cell meets itself: its children's siblings
+    } else if (B >= 0) {
+#pragma clang fp contract(off)
+      const double d0 = ax - c.com[3 * (int64_t)B],
+                   d1 = ay - c.com[3 * (int64_t)B + 1],
+                   d2 = az - c.com[3 * (int64_t)B + 2];
+      double s = 0;
+      s += d0 * d0;
+      s += d1 * d1;
+      s += d2 * d2;
+      const double rb = c.rad[B];
+      const bool bb = c.bucket[B] != 0;
+      if (ra + rb < a.theta * sqrt(s)) {
+        kind = 1;
+        T Qb[6] = {0, 0, 0, 0, 0, 0};
+        if (a.order >= 2)
+          for (int k = 0; k < 6; ++k) Qb[k] = a.Q[6 * (int64_t)B + k];
+        m2l_add<T>(a.G, a.order, (T)d0, (T)d1, (T)d2, (T)c.mass[B], Qb, Lk);
+        ++m2l;
+      } else if (ba && bb) {
+        kind = 2;
+      } else if (!ba && (bb || ra > rb)) {
+        kind = 3;
+      } else {
+        kind = 4;
+      }
+    }
+    const int np0 = np;
+    buf_append(kind == 2, B, pbuf, np);
+    pc += np - np0;
+    if (np >= kChunk) buf_flush(a.l, pbuf, np, ph, kChunk, a.flags);
+    buf_append(kind == 3, B, dbuf, nd);
+    if (nd >= kChunk) buf_flush(a.l, dbuf, nd, dh, kChunk, a.flags);
+    const uint64_t sm = __ballot(kind == 4);
+    const int ns = __popcll(sm);
+    if (sp + ns > limit) {
+      over = true;
+    } else {
+      if (kind == 4) stk[sp + __popcll(sm & lanes_below())] = B;
+      sp += ns;
+    }
+    __syncthreads();
+  };
+  // children of up to 8 stacked cells per round, the latest first
+  auto drain = [&]() {
+    while (sp > 0 && !over) {
+      const int k = sp < 8 ? sp : 8;
+      const int e = lane >> 3;
+      int32_t B = -1;
+      if (e < k) B = c.child[8 * (int64_t)stk[sp - 1 - e] + (lane & 7)];
+      __syncthreads();
+      sp -= k;
+      visit(B);
+    }
+  };
+  if (P < 0) {
+    visit(lane == 0 ? A : -1);
+    drain();
+  } else {
+    for (int32_t ch = a.dhead[P]; ch >= 0 && !over; ch = a.l.next[ch]) {
+      const int len = a.l.len[ch];
+      visit(lane < len ? a.l.pool[kChunk * (int64_t)ch + lane] : -1);
+      drain();
+    }
+  }
+  if (over) {
+    if (lane == 0) atomicOr(a.flags, kWalkStackFull);
+    return;
+  }
+  if (!ba) {  // A's children meet their siblings
+    const int32_t cc = lane < 8 ? c.child[8 * (int64_t)A + lane] : -1;
+    buf_append(cc >= 0, cc, dbuf, nd);
+    if (nd >= kChunk) buf_flush(a.l, dbuf, nd, dh, kChunk, a.flags);
+  }
+  if (np) buf_flush(a.l, pbuf, np, ph, np, a.flags);
+  if (nd) buf_flush(a.l, dbuf, nd, dh, nd, a.flags);
+  for (int k = 0; k < 10; ++k) Lk[k] = wave_sum(Lk[k]);
+  m2l = wave_sum(m2l);
+  if (P >= 0) {  // L2L: the parent's expansion shifted to A's centre
+    const T *Lp = a.L + 10 * (int64_t)P;
+    T phi, g0, g1, g2;
+    local_eval(Lp, (T)(ax - c.com[3 * (int64_t)P]),
+               (T)(ay - c.com[3 * (int64_t)P + 1]),
+               (T)(az - c.com[3 * (int64_t)P + 2]), &phi, &g0, &g1, &g2);
+    Lk[0] += phi;
+    Lk[1] += g0;
+    Lk[2] += g1;
+    Lk[3] += g2;
+    for (int k = 4; k < 10; ++k) Lk[k] += Lp[k];
+  }
+  if (lane < 10) {
+    T v = Lk[0];
+    for (int k = 1; k < 10; ++k)
+      if (lane == k) v = Lk[k];
+    a.L[10 * (int64_t)A + lane] = v;
+  }
+  if (lane == 0) {
+    a.dhead[A] = dh;
+    a.phead[A] = ph;
+    a.pcnt[A] = pc;
+    a.mcnt[A] = m2l;
+  }
+}
+
+// Sinks of one depth: the children of that depth's parents (8 slots per
+// parent), or the root alone.
+template <class T>
+__global__ __launch_bounds__(64) void k_fmm_walk(WalkArgs<T> a) {
+  __shared__ int32_t stk[kWalkStack];
+  __shared__ int32_t pbuf[2 * kChunk], dbuf[2 * kChunk];
+  if (*a.scan_word & kFlagScanStall) return;
+  if (a.root >= 0) {
+    if (blockIdx.x == 0 && !(*a.flags & kWalkStackFull))
+      walk_sink<T>(a, a.root, -1, stk, pbuf, dbuf);
+    return;
+  }
+  const int64_t lo = *a.plo, items = 8 * (*a.phi - lo);
+  const int64_t G = gridDim.x;
+  for (int64_t it = xcd_item(blockIdx.x, G); it < items; it += G) {
+    if (*(volatile int *)a.flags & kWalkStackFull) return;
+    const int32_t P = a.parents[lo + (it >> 3)];
+    const int32_t A = a.c.child[8 * (int64_t)P + (it & 7)];
+    if (A >= 0) walk_sink<T>(a, A, P, stk, pbuf, dbuf);
+  }
+}
+
+// internal cells in depth order: flags over the depth-ordered node list,
+// then the scatter (nothing after a stalled scan)
+__global__ __launch_bounds__(kT) void k_internal_flags(
+    const int32_t *__restrict__ nlevel, int64_t nn,
+    const int32_t *__restrict__ nodecell, const int32_t *__restrict__ cbucket,
+    int32_t *__restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nn) return;
+  const int32_t cell = nodecell[nlevel[i]];
+  flag[i] = cell >= 0 && !cbucket[cell];
+}
+__global__ __launch_bounds__(kT) void k_internal_scatter(
+    const int32_t *__restrict__ nlevel, int64_t nn,
+    const int32_t *__restrict__ nodecell, const int32_t *__restrict__ flag,
+    const int64_t *__restrict__ pos, const int *__restrict__ scan_word,
+    int32_t *__restrict__ parents) {
+  if (*scan_word & kFlagScanStall) return;
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nn && flag[i]) parents[pos[i]] = nodecell[nlevel[i]];
+}
+
+// P2P chains into CSR (one wavefront per cell, grid-stride)
+__global__ __launch_bounds__(64) void k_chains_to_csr(
+    const int32_t *__restrict__ head, const int64_t *__restrict__ off,
+    int64_t nc, ListPool l, int32_t *__restrict__ src) {
+  for (int64_t cell = blockIdx.x; cell < nc; cell += gridDim.x) {
+    int64_t o = off[cell];
+    for (int32_t ch = head[cell]; ch >= 0; ch = l.next[ch]) {
+      const int len = l.len[ch];
+      if ((int)threadIdx.x < len)
+        src[o + threadIdx.x] = l.pool[kChunk * (int64_t)ch + threadIdx.x];
+      o += len;
+    }
+  }
+}
+
+// [0] walk flags, [1] most chunks asked of a stripe, [2] P2P entries,
+// [3] scan word, [4] chunks used (one wavefront)
+__global__ void k_walk_mail(const int *__restrict__ flags,
+                            const unsigned long long *__restrict__ used,
+                            int64_t scap,
+                            const int64_t *__restrict__ p2p_total,
+                            const int *__restrict__ scan_word,
+                            int64_t *__restrict__ mail) {
+  const int lane = threadIdx.x;
+  const int64_t u = lane < kStripes ? (int64_t)used[kStripeWords * lane] : 0;
+  int64_t mx = u, sum = u < scap ? u : scap;
+  for (int o = 32; o > 0; o >>= 1) {
+    const int64_t m2 = __shfl_xor(mx, o, 64);
+    mx = mx > m2 ? mx : m2;
+    sum += __shfl_xor(sum, o, 64);
+  }
+  if (lane == 0) {
+    mail[0] = *flags;
+    mail[1] = mx;
+    mail[2] = *p2p_total;
+    mail[3] = *scan_word;
+    mail[4] = sum;
+  }
+}
+
 template <class T> struct KS {
   T s = 0, c = 0;
   __device__ inline void add(T v) {
@@ -615,8 +915,9 @@ __device__ inline float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double fast_rsq(double x) { return 1.0 / sqrt(x); }
 
 constexpr int kEvalB = 64;            // threads per bucket workgroup
-constexpr int kStageBuckets = 16;     // at most this many source buckets
+constexpr int kStageBuckets = 32;     // at most this many source buckets
 constexpr int kStage = 256;           // and particles staged per round
+static_assert(kMaxLeaf <= kStage && kStageBuckets <= kEvalB, "staging");
 
 // Positions in the P2P sums: FP32 evaluations stage them relative to the
 // sink bucket's centre of mass (one FP64 difference per staged source and
@@ -625,21 +926,60 @@ constexpr int kStage = 256;           // and particles staged per round
 template <class T> using EPos = typename std::conditional<sizeof(T) == 4,
                                                           float, double>::type;
 
+// FP32 direct sums of one lane over its staged sources [q, q1): terms are
+// summed eight at a time in FP32 and each group of eight goes into the FP64
+// accumulators.  SELF rounds (the sink's own bucket is staged) skip the
+// pairs at distance zero: the particle itself, and any source that
+// coincides with it in the FP32 relative coordinates; other rounds skip the
+// test.  There a pair at FP32 distance zero needs two particles of
+// different buckets closer than about 1e-7 of their distance to the sink
+// bucket's centre; its non-finite acceleration is what the state check
+// after the step reports.
+template <bool SELF>
+__device__ inline void p2p_terms(const float4 *s4, int q, int q1, float px,
+                                 float py, float pz, KS<float> &f0,
+                                 KS<float> &f1, KS<float> &f2,
+                                 KS<float> &pp) {
+  auto term = [&](const float4 s, float &g0, float &g1, float &g2,
+                  float &ph) {
+    const float d0 = px - s.x, d1 = py - s.y, d2 = pz - s.z;
+    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    float ir = fast_rsq(r2);
+    if (SELF) ir = r2 > 0.f ? ir : 0.f;
+    const float gm = s.w * ir;
+    const float fr = gm * (ir * ir);
+    ph -= gm;
+    g0 -= fr * d0;
+    g1 -= fr * d1;
+    g2 -= fr * d2;
+  };
+  constexpr int U = 8;
+  for (; q + U <= q1; q += U) {
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) term(s4[q + u], g0, g1, g2, ph);
+    f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
+  }
+  if (q < q1) {
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
+    for (; q < q1; ++q) term(s4[q], g0, g1, g2, ph);
+    f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
+  }
+}
+
 // One wavefront per bucket: its local expansion, then direct sums over its
 // P2P source buckets (unsoftened, skipping the particle itself, fmm.h:37-46,
-// 124-143).  Sources are staged in LDS kStageBuckets buckets at a time; a
+// 124-143).  Sources are staged in LDS kStageBuckets buckets at a time (the
+// lanes read the next list entries and their sizes in parallel); a
 // bucket of c particles gets LP = floor(64/c) lanes per particle, which
 // split the staged sources; the lanes' partial sums are added in lane order
 // through LDS.
 //
 // FP32 evaluations stage each source as one float4 (position relative to
-// the sink bucket's centre of mass, and G m) read with one LDS load, and a
-// lane sums its terms four at a time in FP32 (fused multiply-adds) before
-// adding each group of four to FP64 accumulators: one conversion and one
-// FP64 add per four terms instead of per term, at the cost of at most three
-// more FP32 roundings per term.  The particle itself (and any coincident
-// source) is the pair at distance zero, which contributes nothing.  FP64
-// evaluations keep the per-term form.
+// the sink bucket's centre of mass, and G m) read with one LDS load; the LP
+// lanes of a particle take contiguous, odd-length runs of the staged
+// sources (runs of odd length start in different LDS banks), summed by
+// p2p_terms.  FP64 evaluations keep the per-term form.
 template <class T>
 __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   using P = EPos<T>;
@@ -649,7 +989,7 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
       sz[F32 ? 1 : kStage];
   __shared__ T sm[F32 ? 1 : kStage];
   __shared__ int32_t sidx[F32 ? 1 : kStage];
-  __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1], snb;
+  __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1];
   __shared__ T part[4][kEvalB];
   const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
   if (!a.c.bucket[cell]) return;
@@ -680,27 +1020,38 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   }
   const int64_t j1 = a.off[cell + 1];
   for (int64_t j0 = a.off[cell]; j0 < j1;) {
-    if (t == 0) {  // whole buckets while they fit (a bucket has <= kStage)
-      int acc = 0, i = 0;
-      while (i < kStageBuckets && j0 + i < j1) {
-        const int32_t s = a.src[j0 + i];
-        const int sc = a.c.count[s];
-        if (acc + sc > kStage) break;
-        bst[i] = a.c.start[s];
-        boff[i] = acc;
-        acc += sc;
-        ++i;
-      }
-      boff[i] = acc;
-      snb = i;
+    // the next kStageBuckets list entries, one per lane: whole buckets
+    // while they fit (a bucket has at most kMaxLeaf <= kStage particles,
+    // so the first one always does; the fitting lanes are a prefix)
+    int32_t s = -1;
+    int sc = 0, sst = 0;
+    if (t < kStageBuckets && j0 + t < j1) {
+      s = a.src[j0 + t];
+      sc = a.c.count[s];
+      sst = a.c.start[s];
     }
+    int tot;
+    const int ex = wave_excl_scan(sc, &tot);
+    const bool fit = s >= 0 && ex + sc <= kStage;
+    const int nb = __popcll(__ballot(fit));
+    const bool self = __ballot(fit && s == (int32_t)cell) != 0;
+    if (fit) {
+      bst[t] = sst;
+      boff[t] = ex;
+    }
+    if (t == nb - 1) boff[nb] = ex + sc;
     __syncthreads();
-    const int nb = snb;
     const int nst = boff[nb];
     j0 += nb;
     for (int q = t; q < nst; q += kEvalB) {
-      int i = 0;
-      while (boff[i + 1] <= q) ++i;
+      int i = 0, hi = nb - 1;  // the last bucket starting at or before q
+      while (i < hi) {
+        const int mid = (i + hi + 1) >> 1;
+        if (boff[mid] <= q)
+          i = mid;
+        else
+          hi = mid - 1;
+      }
       const int64_t pi = bst[i] + (q - boff[i]);
       if constexpr (F32) {
         s4[q] = make_float4((float)(a.x[pi] - ox), (float)(a.y[pi] - oy),
@@ -715,39 +1066,12 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
     __syncthreads();
     if (act) {
       if constexpr (F32) {
-        // four terms per group: partial sums in FP32, then into FP64
-        int q = r;
-        for (; q + 3 * LP < nst; q += 4 * LP) {
-          float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const float4 s = s4[q + u * LP];
-            const float d0 = px - s.x, d1 = py - s.y, d2 = pz - s.z;
-            const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            const float ir = r2 > 0.f ? fast_rsq(r2) : 0.f;
-            const float gm = s.w * ir;
-            const float fr = gm * (ir * ir);
-            ph -= gm;
-            g0 -= fr * d0;
-            g1 -= fr * d1;
-            g2 -= fr * d2;
-          }
-          f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
-        }
-        float g0 = 0.f, g1 = 0.f, g2 = 0.f, ph = 0.f;
-        for (; q < nst; q += LP) {
-          const float4 s = s4[q];
-          const float d0 = px - s.x, d1 = py - s.y, d2 = pz - s.z;
-          const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          const float ir = r2 > 0.f ? fast_rsq(r2) : 0.f;
-          const float gm = s.w * ir;
-          const float fr = gm * (ir * ir);
-          ph -= gm;
-          g0 -= fr * d0;
-          g1 -= fr * d1;
-          g2 -= fr * d2;
-        }
-        f0.add(g0); f1.add(g1); f2.add(g2); pp.add(ph);
+        const int per = ((nst + LP - 1) / LP) | 1;
+        const int q0 = r * per, q1 = min(nst, q0 + per);
+        if (self)
+          p2p_terms<true>(s4, q0, q1, px, py, pz, f0, f1, f2, pp);
+        else
+          p2p_terms<false>(s4, q0, q1, px, py, pz, f0, f1, f2, pp);
       } else {
         for (int q = r; q < nst; q += LP) {
           if (sidx[q] == (int32_t)p) continue;
@@ -786,22 +1110,36 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   }
 }
 
-// Particle-particle interactions of a P2P list of (sink, source) bucket
-// pairs: count(sink) * count(source), less the self pairs of a bucket with
+// Sizes of the last evaluation: [0] buckets, [1] M2L pairs (from per-cell
+// counts, when given), [2] particle-particle interactions of the CSR P2P
+// list: count(sink) * count(source), less the self pairs of a bucket with
 // itself.
-__global__ __launch_bounds__(kT) void k_p2p_work(
-    const int2 *__restrict__ l, int64_t n, const int32_t *__restrict__ count,
+__global__ __launch_bounds__(kT) void k_grav_counts(
+    const int32_t *__restrict__ bucket, const int32_t *__restrict__ count,
+    int64_t nc, const int32_t *__restrict__ mcnt,
+    const int64_t *__restrict__ off, const int32_t *__restrict__ src,
     unsigned long long *__restrict__ out) {
-  unsigned long long s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kT) {
-    const int2 pr = l[i];
-    const unsigned long long a = (unsigned)count[pr.x],
-                             b = (unsigned)count[pr.y];
-    s += a * b - (pr.x == pr.y ? a : 0ull);
+  unsigned long long nb = 0, nm = 0, ni = 0;
+  for (int64_t a = (int64_t)blockIdx.x * kT + threadIdx.x; a < nc;
+       a += (int64_t)gridDim.x * kT) {
+    nb += bucket[a] != 0;
+    if (mcnt) nm += (unsigned)mcnt[a];
+    const unsigned long long ca = (unsigned)count[a];
+    for (int64_t j = off[a]; j < off[a + 1]; ++j) {
+      const int32_t b = src[j];
+      ni += ca * (unsigned)count[b] - (b == a ? ca : 0ull);
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+  for (int o = 32; o > 0; o >>= 1) {
+    nb += __shfl_down(nb, o, 64);
+    nm += __shfl_down(nm, o, 64);
+    ni += __shfl_down(ni, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (nb) atomicAdd(out, nb);
+    if (nm) atomicAdd(out + 1, nm);
+    if (ni) atomicAdd(out + 2, ni);
+  }
 }
 
 int bucket_sort(Ctx *c, const int2 *list, int64_t n, int64_t nc, DBuf &off,
@@ -824,73 +1162,14 @@ int bucket_sort(Ctx *c, const int2 *list, int64_t n, int64_t nc, DBuf &off,
   return hip_check(c, hipGetLastError(), "k_scatter_sinks");
 }
 
-template <class T> int gravity_impl(Ctx *c) {
+// The level-synchronous pair recursion: lists bucketed by sink, M2L one
+// wavefront per sink, L2L level by level; leaves the P2P list in CSR form
+// (p2p_off, p2p_src).
+template <class T>
+int pairs_path(Ctx *c, const Cells &cells, const T *Q, T *Lc) {
   auto &G = c->grav;
-  const int64_t n = c->n, nn = c->nn;
-  const int L = c->p.leaf_size;
+  const int64_t n = c->n, nc = G.nc;
   const int order = c->p.fmm_order >= 2 ? 2 : 1;
-  // ---- cells
-  const int64_t tot = nn + n;
-  FSPH_TRY(ensure(c, G.flag, (size_t)(tot + 1) * 4));
-  FSPH_TRY(ensure(c, G.pos, (size_t)(tot + 1) * 8));
-  FSPH_TRY(ensure(c, G.nodecell, (size_t)(nn + 1) * 4));
-  FSPH_TRY(ensure(c, G.partcell, (size_t)(n + 1) * 4));
-  k_cell_flags<<<blocks_for(tot), kT, 0, c->stream>>>(
-      c->ncount.as<int32_t>(), c->nparent.as<int32_t>(), nn,
-      c->leaf_parent.as<int32_t>(), n, L, G.flag.as<int32_t>());
-  FSPH_HIP(c, hipGetLastError());
-  FSPH_TRY(scan_i32_to_i64(c, G.flag.as<int32_t>(), G.pos.as<int64_t>(), tot));
-  int64_t nc = 0;
-  FSPH_TRY(read_scan_total(c, G.pos.as<int64_t>() + tot, &nc));
-  G.nc = nc;
-  if (nc > G.cap_cells) {
-    const int64_t cap = nc + nc / 4 + 1024;
-    FSPH_TRY(ensure(c, G.cnode, cap * 4));
-    FSPH_TRY(ensure(c, G.cstart, cap * 4));
-    FSPH_TRY(ensure(c, G.ccount, cap * 4));
-    FSPH_TRY(ensure(c, G.cbucket, cap * 4));
-    FSPH_TRY(ensure(c, G.cchild, cap * 32));
-    FSPH_TRY(ensure(c, G.ccom, cap * 24));
-    FSPH_TRY(ensure(c, G.cmass, cap * 8));
-    FSPH_TRY(ensure(c, G.crad, cap * 8));
-    FSPH_TRY(ensure(c, G.cQ, cap * 6 * sizeof(T)));
-    FSPH_TRY(ensure(c, G.cL, cap * 10 * sizeof(T)));
-    G.cap_cells = cap;
-  }
-  CellOut co{G.cnode.as<int32_t>(),   G.cstart.as<int32_t>(),
-             G.ccount.as<int32_t>(),  G.cbucket.as<int32_t>(),
-             G.cchild.as<int32_t>(),  G.nodecell.as<int32_t>(),
-             G.partcell.as<int32_t>(), G.ccom.as<double>(),
-             G.cmass.as<double>(),    G.crad.as<double>()};
-  k_cell_write<<<blocks_for(tot), kT, 0, c->stream>>>(
-      G.flag.as<int32_t>(), G.pos.as<int64_t>(), nn, n, L,
-      c->nstart.as<int32_t>(), c->ncount.as<int32_t>(), c->ncom.as<double>(),
-      c->nmass.as<double>(), c->nrad.as<double>(), c->f[PF_X].as<double>(),
-      c->f[PF_Y].as<double>(), c->f[PF_Z].as<double>(),
-      c->f[PF_M].as<double>(), co);
-  k_cell_children<<<blocks_for(nc), kT, 0, c->stream>>>(
-      G.cnode.as<int32_t>(), G.cbucket.as<int32_t>(), nc,
-      c->nchild.as<int32_t>(), G.nodecell.as<int32_t>(),
-      G.partcell.as<int32_t>(), G.cchild.as<int32_t>());
-  FSPH_HIP(c, hipGetLastError());
-  Cells cells{G.cnode.as<int32_t>(), G.cstart.as<int32_t>(),
-              G.ccount.as<int32_t>(), G.cbucket.as<int32_t>(),
-              G.cchild.as<int32_t>(), G.ccom.as<double>(),
-              G.cmass.as<double>(),  G.crad.as<double>()};
-  T *Q = G.cQ.as<T>();
-  T *Lc = G.cL.as<T>();
-  // ---- multipoles: P2M on buckets, M2M up the node levels
-  k_p2m<T><<<blocks_for(nc), kT, 0, c->stream>>>(
-      cells, nc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(), Q);
-  for (int d = c->max_depth; d >= 0; --d) {
-    const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
-    if (!cnt) continue;
-    k_m2m<T><<<blocks_for(cnt), kT, 0, c->stream>>>(
-        cells, c->nlevel.as<int32_t>() + c->level_off[d], cnt,
-        G.nodecell.as<int32_t>(), Q);
-  }
-  FSPH_HIP(c, hipGetLastError());
   // ---- pair recursion
   if (G.cap_q == 0) {
     G.cap_q = std::max<int64_t>(1 << 16, 64 * nc);
@@ -982,6 +1261,197 @@ template <class T> int gravity_impl(Ctx *c) {
       cells, G.partcell.as<int32_t>(), n, c->leaf_parent.as<int32_t>(),
       G.nodecell.as<int32_t>(), Lc);
   FSPH_HIP(c, hipGetLastError());
+  return FSPH_OK;
+}
+
+// The sink-major walk (k_fmm_walk), one launch per depth.  *fallback = 1
+// when a sink's stack would pass its limit: the pair recursion then builds
+// the lists.  Leaves the local expansions and the P2P list in CSR form.
+template <class T>
+int walk_path(Ctx *c, const Cells &cells, const T *Q, T *Lc, int *fallback) {
+  auto &G = c->grav;
+  const int64_t nn = c->nn, nc = G.nc;
+  *fallback = 0;
+  FSPH_TRY(ensure(c, G.iflag, (size_t)(nn + 1) * 4));
+  FSPH_TRY(ensure(c, G.ipos, (size_t)(nn + 1) * 8));
+  FSPH_TRY(ensure(c, G.parents, (size_t)(nc + 1) * 4));
+  FSPH_TRY(ensure(c, G.dhead, (size_t)(nc + 1) * 4));
+  FSPH_TRY(ensure(c, G.phead, (size_t)(nc + 1) * 4));
+  FSPH_TRY(ensure(c, G.pcnt, (size_t)(nc + 1) * 4));
+  FSPH_TRY(ensure(c, G.mcnt, (size_t)(nc + 1) * 4));
+  FSPH_TRY(ensure(c, G.p2p_off, (size_t)(nc + 1) * 8));
+  FSPH_TRY(ensure(c, G.ctr, (size_t)kStripes * kStripeWords * 8));
+  PhaseTimer t_walk(c, FSPH_T_GRAV_PAIRS);
+  // internal cells in depth order
+  k_internal_flags<<<blocks_for(nn), kT, 0, c->stream>>>(
+      c->nlevel.as<int32_t>(), nn, G.nodecell.as<int32_t>(),
+      G.cbucket.as<int32_t>(), G.iflag.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i32_to_i64(c, G.iflag.as<int32_t>(), G.ipos.as<int64_t>(),
+                           nn));
+  k_internal_scatter<<<blocks_for(nn), kT, 0, c->stream>>>(
+      c->nlevel.as<int32_t>(), nn, G.nodecell.as<int32_t>(),
+      G.iflag.as<int32_t>(), G.ipos.as<int64_t>(),
+      c->flags.as<int>() + kScanWord, G.parents.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  if (G.cap_chunks == 0) G.cap_chunks = std::max<int64_t>(4096, 4 * nc);
+  const int64_t scap0 = (G.cap_chunks + kStripes - 1) / kStripes;
+  int *wflag = c->flags.as<int>() + kWalkWord;
+  unsigned long long *used = G.ctr.as<unsigned long long>();
+  int64_t n_p2p = 0;
+  int64_t scap = scap0;
+  for (int attempt = 0;; ++attempt) {
+    G.cap_chunks = scap * kStripes;
+    FSPH_TRY(ensure(c, G.lpool, (size_t)G.cap_chunks * kChunk * 4));
+    FSPH_TRY(ensure(c, G.lnext, (size_t)G.cap_chunks * 4));
+    FSPH_TRY(ensure(c, G.llen, (size_t)G.cap_chunks * 4));
+    FSPH_HIP(c, hipMemsetAsync(used, 0, (size_t)kStripes * kStripeWords * 8,
+                               c->stream));
+    FSPH_HIP(c, hipMemsetAsync(wflag, 0, sizeof(int), c->stream));
+    WalkArgs<T> wa{cells,
+                   Q,
+                   Lc,
+                   (T)c->p.G,
+                   c->p.fmm_order >= 2 ? 2 : 1,
+                   c->p.theta,
+                   G.parents.as<int32_t>(),
+                   nullptr,
+                   nullptr,
+                   0,  // the root is cell 0 (node 0, first in node order)
+                   ListPool{G.lpool.as<int32_t>(), G.lnext.as<int32_t>(),
+                            G.llen.as<int32_t>(), scap, used},
+                   G.dhead.as<int32_t>(),
+                   G.phead.as<int32_t>(),
+                   G.pcnt.as<int32_t>(),
+                   G.mcnt.as<int32_t>(),
+                   G.stack_limit > 0 ? G.stack_limit : kWalkStack,
+                   wflag,
+                   c->flags.as<int>() + kScanWord};
+    k_fmm_walk<T><<<1, 64, 0, c->stream>>>(wa);
+    int depths = 1;
+    wa.root = -1;
+    const int64_t *ipos = G.ipos.as<int64_t>();
+    for (int d = 1; d <= c->max_depth + 1; ++d) {
+      const int64_t nodes = c->level_off[d] - c->level_off[d - 1];
+      if (!nodes) continue;
+      wa.plo = ipos + c->level_off[d - 1];
+      wa.phi = ipos + c->level_off[d];
+      const unsigned grid =
+          (unsigned)std::min<int64_t>(8 * nodes, kWalkGrid);
+      k_fmm_walk<T><<<grid, 64, 0, c->stream>>>(wa);
+      ++depths;
+    }
+    FSPH_HIP(c, hipGetLastError());
+    FSPH_TRY(scan_i32_to_i64(c, G.pcnt.as<int32_t>(),
+                             G.p2p_off.as<int64_t>(), nc));
+    k_walk_mail<<<1, 64, 0, c->stream>>>(wflag, used, scap,
+                                         G.p2p_off.as<int64_t>() + nc,
+                                         c->flags.as<int>() + kScanWord,
+                                         c->mail.as<int64_t>());
+    FSPH_HIP(c, hipGetLastError());
+    FSPH_TRY(read_mail(c, 5));
+    if (c->mail_h[3] & kFlagScanStall)
+      return fail(c, FSPH_E_HIP, "scan: a tile's look-back did not complete");
+    const int fl = (int)c->mail_h[0];
+    if (fl & kWalkStackFull) {
+      *fallback = 1;
+      return FSPH_OK;
+    }
+    if (!(fl & kWalkPoolFull)) {
+      n_p2p = c->mail_h[2];
+      G.levels = depths;
+      G.chunks_used = c->mail_h[4];
+      break;
+    }
+    if (attempt >= 30) return fail(c, FSPH_E_CAP, "FMM list pool overflow");
+    const int64_t want = c->mail_h[1];  // of the fullest stripe
+    scap = std::max<int64_t>(2 * scap, want + want / 4 + 16);
+  }
+  FSPH_TRY(ensure(c, G.p2p_src, (size_t)(n_p2p > 0 ? n_p2p : 1) * 4));
+  k_chains_to_csr<<<(unsigned)std::min<int64_t>(nc, kWalkGrid), 64, 0,
+                    c->stream>>>(
+      G.phead.as<int32_t>(), G.p2p_off.as<int64_t>(), nc,
+      ListPool{G.lpool.as<int32_t>(), G.lnext.as<int32_t>(),
+               G.llen.as<int32_t>(), scap, used},
+      G.p2p_src.as<int32_t>());
+  FSPH_TRY(hip_check(c, hipGetLastError(), "k_chains_to_csr"));
+  G.n_p2p = n_p2p;
+  G.n_m2l = -1;  // per-cell counts in mcnt, summed on request
+  return FSPH_OK;
+}
+
+template <class T> int gravity_impl(Ctx *c) {
+  auto &G = c->grav;
+  const int64_t n = c->n, nn = c->nn;
+  const int L = c->p.leaf_size;
+  const int order = c->p.fmm_order >= 2 ? 2 : 1;
+  // ---- cells
+  const int64_t tot = nn + n;
+  FSPH_TRY(ensure(c, G.flag, (size_t)(tot + 1) * 4));
+  FSPH_TRY(ensure(c, G.pos, (size_t)(tot + 1) * 8));
+  FSPH_TRY(ensure(c, G.nodecell, (size_t)(nn + 1) * 4));
+  FSPH_TRY(ensure(c, G.partcell, (size_t)(n + 1) * 4));
+  k_cell_flags<<<blocks_for(tot), kT, 0, c->stream>>>(
+      c->ncount.as<int32_t>(), c->nparent.as<int32_t>(), nn,
+      c->leaf_parent.as<int32_t>(), n, L, G.flag.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  FSPH_TRY(scan_i32_to_i64(c, G.flag.as<int32_t>(), G.pos.as<int64_t>(), tot));
+  int64_t nc = 0;
+  FSPH_TRY(read_scan_total(c, G.pos.as<int64_t>() + tot, &nc));
+  G.nc = nc;
+  if (nc > G.cap_cells) {
+    const int64_t cap = nc + nc / 4 + 1024;
+    FSPH_TRY(ensure(c, G.cnode, cap * 4));
+    FSPH_TRY(ensure(c, G.cstart, cap * 4));
+    FSPH_TRY(ensure(c, G.ccount, cap * 4));
+    FSPH_TRY(ensure(c, G.cbucket, cap * 4));
+    FSPH_TRY(ensure(c, G.cchild, cap * 32));
+    FSPH_TRY(ensure(c, G.ccom, cap * 24));
+    FSPH_TRY(ensure(c, G.cmass, cap * 8));
+    FSPH_TRY(ensure(c, G.crad, cap * 8));
+    FSPH_TRY(ensure(c, G.cQ, cap * 6 * sizeof(T)));
+    FSPH_TRY(ensure(c, G.cL, cap * 10 * sizeof(T)));
+    G.cap_cells = cap;
+  }
+  CellOut co{G.cnode.as<int32_t>(),   G.cstart.as<int32_t>(),
+             G.ccount.as<int32_t>(),  G.cbucket.as<int32_t>(),
+             G.cchild.as<int32_t>(),  G.nodecell.as<int32_t>(),
+             G.partcell.as<int32_t>(), G.ccom.as<double>(),
+             G.cmass.as<double>(),    G.crad.as<double>()};
+  k_cell_write<<<blocks_for(tot), kT, 0, c->stream>>>(
+      G.flag.as<int32_t>(), G.pos.as<int64_t>(), nn, n, L,
+      c->nstart.as<int32_t>(), c->ncount.as<int32_t>(), c->ncom.as<double>(),
+      c->nmass.as<double>(), c->nrad.as<double>(), c->f[PF_X].as<double>(),
+      c->f[PF_Y].as<double>(), c->f[PF_Z].as<double>(),
+      c->f[PF_M].as<double>(), co);
+  k_cell_children<<<blocks_for(nc), kT, 0, c->stream>>>(
+      G.cnode.as<int32_t>(), G.cbucket.as<int32_t>(), nc,
+      c->nchild.as<int32_t>(), G.nodecell.as<int32_t>(),
+      G.partcell.as<int32_t>(), G.cchild.as<int32_t>());
+  FSPH_HIP(c, hipGetLastError());
+  Cells cells{G.cnode.as<int32_t>(), G.cstart.as<int32_t>(),
+              G.ccount.as<int32_t>(), G.cbucket.as<int32_t>(),
+              G.cchild.as<int32_t>(), G.ccom.as<double>(),
+              G.cmass.as<double>(),  G.crad.as<double>()};
+  T *Q = G.cQ.as<T>();
+  T *Lc = G.cL.as<T>();
+  // ---- multipoles: P2M on buckets, M2M up the node levels
+  k_p2m<T><<<blocks_for(nc), kT, 0, c->stream>>>(
+      cells, nc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+      c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(), Q);
+  for (int d = c->max_depth; d >= 0; --d) {
+    const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
+    if (!cnt) continue;
+    k_m2m<T><<<blocks_for(cnt), kT, 0, c->stream>>>(
+        cells, c->nlevel.as<int32_t>() + c->level_off[d], cnt,
+        G.nodecell.as<int32_t>(), Q);
+  }
+  FSPH_HIP(c, hipGetLastError());
+  // ---- interaction lists, M2L and L2L
+  int fallback = G.builder == FSPH_FMM_PAIRS;
+  if (!fallback) FSPH_TRY(walk_path<T>(c, cells, Q, Lc, &fallback));
+  G.used_builder = fallback ? FSPH_FMM_PAIRS : FSPH_FMM_WALK;
+  if (fallback) FSPH_TRY(pairs_path<T>(c, cells, Q, Lc));
   // ---- L2P + P2P
   EvalArgs<T> ea{cells, nc, G.p2p_off.as<int64_t>(), G.p2p_src.as<int32_t>(),
                  Lc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
@@ -1005,27 +1475,41 @@ int stage_gravity(Ctx *c) {
   return c->rsize == 4 ? gravity_impl<float>(c) : gravity_impl<double>(c);
 }
 
-int gravity_stats(Ctx *c, int64_t out[6]) {
+int gravity_stats(Ctx *c, int64_t out[8]) {
   auto &G = c->grav;
-  for (int i = 0; i < 6; ++i) out[i] = 0;
+  for (int i = 0; i < 8; ++i) out[i] = 0;
   if (G.nc == 0) return FSPH_OK;  // no gravity evaluation yet
-  out[0] = G.nc;
-  out[1] = G.nc - G.n_node_cells;
-  out[2] = G.n_m2l;
-  out[3] = G.n_p2p;
-  out[5] = G.levels;
-  if (G.n_p2p == 0) return FSPH_OK;
-  FSPH_TRY(ensure(c, c->red_tmp, 8));
+  FSPH_TRY(ensure(c, c->red_tmp, 4 * 8));
   unsigned long long *w = c->red_tmp.as<unsigned long long>();
-  FSPH_HIP(c, hipMemsetAsync(w, 0, 8, c->stream));
-  const unsigned nb = (unsigned)std::min<int64_t>(2048, blocks_for(G.n_p2p));
-  k_p2p_work<<<nb, kT, 0, c->stream>>>(G.p2p.as<int2>(), G.n_p2p,
-                                       G.ccount.as<int32_t>(), w);
+  FSPH_HIP(c, hipMemsetAsync(w, 0, 4 * 8, c->stream));
+  const unsigned nb = (unsigned)std::min<int64_t>(2048, blocks_for(G.nc));
+  k_grav_counts<<<nb, kT, 0, c->stream>>>(
+      G.cbucket.as<int32_t>(), G.ccount.as<int32_t>(), G.nc,
+      G.n_m2l < 0 ? G.mcnt.as<int32_t>() : nullptr,
+      G.p2p_off.as<int64_t>(), G.p2p_src.as<int32_t>(), w);
   FSPH_HIP(c, hipGetLastError());
-  unsigned long long v = 0;
-  FSPH_HIP(c, hipMemcpyAsync(&v, w, 8, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long v[3] = {0, 0, 0};
+  FSPH_HIP(c, hipMemcpyAsync(v, w, 3 * 8, hipMemcpyDeviceToHost, c->stream));
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
-  out[4] = (int64_t)v;
+  out[0] = G.nc;
+  out[1] = (int64_t)v[0];
+  out[2] = G.n_m2l >= 0 ? G.n_m2l : (int64_t)v[1];
+  out[3] = G.n_p2p;
+  out[4] = (int64_t)v[2];
+  out[5] = G.levels;
+  out[6] = G.used_builder;
+  out[7] = G.used_builder == FSPH_FMM_WALK ? G.chunks_used : 0;
+  return FSPH_OK;
+}
+
+int set_fmm_builder(Ctx *c, int builder, int stack_limit) {
+  if (builder != FSPH_FMM_WALK && builder != FSPH_FMM_PAIRS)
+    return fail(c, FSPH_E_ARG, "unknown FMM list builder %d", builder);
+  if (stack_limit < 0 || stack_limit > kWalkStack)
+    return fail(c, FSPH_E_ARG, "walk stack limit must be in [0, %d]",
+                kWalkStack);
+  c->grav.builder = builder;
+  c->grav.stack_limit = stack_limit;
   return FSPH_OK;
 }
 
@@ -1036,6 +1520,7 @@ int set_fmm_capacity(Ctx *c, int64_t cap_q, int64_t cap_m2l,
   c->grav.cap_q = cap_q;
   c->grav.cap_m2l = cap_m2l;
   c->grav.cap_p2p = cap_p2p;
+  c->grav.cap_chunks = (cap_q + kChunk - 1) / kChunk;
   return FSPH_OK;
 }
 
@@ -1045,7 +1530,9 @@ void free_gravity(Ctx *c) {
                  &G.cstart,  &G.ccount,  &G.cbucket,  &G.cchild,   &G.ccom,
                  &G.cmass,   &G.crad,    &G.cQ,       &G.cL,       &G.qa,
                  &G.qb,      &G.m2l,     &G.p2p,      &G.ctr,      &G.m2l_off,
-                 &G.m2l_src, &G.p2p_off, &G.p2p_src,  &G.cursor};
+                 &G.m2l_src, &G.p2p_off, &G.p2p_src,  &G.cursor,   &G.iflag,
+                 &G.ipos,    &G.parents, &G.lpool,    &G.lnext,    &G.llen,
+                 &G.dhead,   &G.phead,   &G.pcnt,     &G.mcnt};
   for (DBuf *b : all)
     if (b->p) {
       hipFree(b->p);

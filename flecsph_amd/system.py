@@ -193,12 +193,17 @@ class _System:
         return out
 
     def gravity_stats(self):
-        out = np.zeros(6, np.int64)
+        out = np.zeros(8, np.int64)
         self._call("gravity_stats",
                    out.ctypes.data_as(C.POINTER(C.c_int64)))
         keys = ("cells", "buckets", "m2l_pairs", "p2p_pairs",
-                "p2p_interactions", "pair_levels")
+                "p2p_interactions", "pair_levels", "builder",
+                "pool_chunks")
         return dict(zip(keys, (int(v) for v in out)))
+
+    def set_fmm_builder(self, builder, stack_limit=0):
+        """builder: _abi.FMM_WALK or _abi.FMM_PAIRS."""
+        self._call("set_fmm_builder", int(builder), int(stack_limit))
 
     def set_fmm_capacity(self, cap_q, cap_m2l, cap_p2p):
         self._call("set_fmm_capacity", int(cap_q), int(cap_m2l), int(cap_p2p))

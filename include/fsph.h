@@ -258,15 +258,33 @@ int fsph_gravity(fsph_ctx *ctx);
    reference counterpart): out[0] FMM cells (internal nodes with more than
    leaf_size particles, and buckets), out[1] buckets, out[2] M2L cell
    pairs, out[3] P2P bucket pairs, out[4] particle-particle interactions of
-   those pairs (self pairs excluded), out[5] levels of the dual-tree pair
-   recursion.  All zero before the first gravity evaluation. */
-int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[6]);
+   those pairs (self pairs excluded), out[5] levels of the pair recursion
+   or depths of the sink-major walk, out[6] the list builder that ran
+   (FSPH_FMM_WALK or FSPH_FMM_PAIRS), out[7] list-pool chunks the walk
+   used.  All zero before the first gravity evaluation. */
+int fsph_gravity_stats(fsph_ctx *ctx, int64_t out[8]);
 
-/* Capacities (entries) of the FMM pair queue, M2L list and P2P list that
-   the next fsph_gravity starts from (no reference counterpart).  The lists
-   grow by doubling whenever a level overflows them, so any positive value
-   gives the same result; small values make the overflow-and-regrow path
-   run, for testing. */
+/* FMM interaction-list builders (fsph_set_fmm_builder).  Both build the
+   pair set of the dual-tree recursion (tree_topology.h:449-795):
+   FSPH_FMM_WALK walks it sink by sink, depth by depth, and sums M2L and
+   L2L in the walk; FSPH_FMM_PAIRS runs it level by level over (sink,
+   source) pairs and buckets the lists by sink. */
+#define FSPH_FMM_WALK 0
+#define FSPH_FMM_PAIRS 1
+
+/* List builder of the next fsph_gravity calls (no reference counterpart;
+   testing and measurement), FSPH_FMM_WALK by default.  stack_limit in
+   [1, 1024] caps the walk's per-sink LDS stack of cells still to split
+   (0: 1024); an evaluation in which a sink would pass it is redone by the
+   pair recursion (gravity_stats out[6] says which ran). */
+int fsph_set_fmm_builder(fsph_ctx *ctx, int builder, int stack_limit);
+
+/* Capacities (entries) that the next fsph_gravity starts from (no
+   reference counterpart): cap_q sizes the walk's list pool (in 64-entry
+   chunks) and the pair recursion's queue, cap_m2l and cap_p2p the pair
+   recursion's M2L and P2P lists.  They grow whenever an evaluation
+   overflows them, so any positive value gives the same result; small
+   values make the overflow-and-regrow path run, for testing. */
 int fsph_set_fmm_capacity(fsph_ctx *ctx, int64_t cap_q, int64_t cap_m2l,
                           int64_t cap_p2p);
 

@@ -156,5 +156,43 @@ def test_fmm_list_overflow_regrows(oracle_mod):
             g.set_fmm_capacity(64 << k, 64, 64 << k)
         gg, pg = run(g, parts)
         st = g.gravity_stats()
+        assert st["builder"] == _abi.FMM_WALK and st["pool_chunks"] > 1
         assert st["m2l_pairs"] > 64 and st["p2p_pairs"] > 64
         assert rel(gg, go) <= 1e-10 and rel(pg, po) <= 1e-10, k
+    # the pair recursion's own queue and lists overflow and regrow too
+    g.set_fmm_builder(_abi.FMM_PAIRS)
+    g.set_fmm_capacity(64, 64, 64)
+    gg, pg = run(g, parts)
+    assert g.gravity_stats()["builder"] == _abi.FMM_PAIRS
+    assert rel(gg, go) <= 1e-10 and rel(pg, po) <= 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [(_abi.PREC_FP64, 1e-12),
+                                           (_abi.PREC_FP32, 1e-6)])
+def test_walk_and_pair_recursion_agree(precision, tol):
+    """The sink-major walk and the level-synchronous pair recursion build
+    the same interaction set (equal M2L and P2P pair counts and
+    particle-particle interactions), so their fields differ by summation
+    order only; a stack limit of one cell makes the walk hand the
+    evaluation to the pair recursion, with the same result."""
+    parts, prm = generators.polytrope(n=50_000)
+    prm["precision"] = precision
+    p = _abi.default_params(**prm)
+    print(f"walk vs pairs, polytrope 50k, precision {precision}", flush=True)
+    out = {}
+    for name, builder, limit in (("walk", _abi.FMM_WALK, 0),
+                                 ("pairs", _abi.FMM_PAIRS, 0),
+                                 ("fallback", _abi.FMM_WALK, 1)):
+        g = BodySystem(p)
+        g.set_fmm_builder(builder, limit)
+        out[name] = run(g, parts) + (g.gravity_stats(),)
+    sw, sp, sf = (out[k][2] for k in ("walk", "pairs", "fallback"))
+    assert sw["builder"] == _abi.FMM_WALK
+    assert sp["builder"] == _abi.FMM_PAIRS
+    assert sf["builder"] == _abi.FMM_PAIRS
+    for k in ("cells", "buckets", "m2l_pairs", "p2p_pairs",
+              "p2p_interactions"):
+        assert sw[k] == sp[k] == sf[k], k
+    gw, pw = out["walk"][:2]
+    for k in ("pairs", "fallback"):
+        assert rel(out[k][0], gw) <= tol and rel(out[k][1], pw) <= tol, k
