@@ -5,18 +5,22 @@
 // (sink, source) pairs; a pair is accepted when r_A + r_B < theta d
 // (tree_geometry.h:343-350), two leaves interact directly, a cell meets
 // itself by splitting into all child pairs, and otherwise the larger radius
-// is split.  Here the same recursion runs level-synchronously over FMM cells
-// (octree nodes with more than L particles, plus buckets: maximal nodes with
-// at most L particles and single particles under larger nodes), one thread
-// per pair with wave-aggregated appends.  Accepted pairs become M2L
-// interactions (order 1: monopole source and first-order local field, which
-// is the reference's scheme at L = 1; order 2: traceless quadrupole source
-// and second-order local expansion), bucket pairs become P2P interactions.
-// Lists are bucketed by sink; M2L runs one wavefront per sink cell with a
-// shuffle reduction, L2L goes down level by level, and one workgroup per
-// bucket evaluates its local expansion and sums its P2P sources from LDS.
-// Centres of mass and radii are the FP64 octree aggregates; multipole and
-// pair arithmetic is FP32 with FP64 position differences.
+// is split.  Here the same pair set is built over FMM cells (octree nodes
+// with more than L particles, plus buckets: maximal nodes with at most L
+// particles and single particles under larger nodes).  Accepted pairs are
+// M2L interactions (order 1: monopole source and first-order local field,
+// which is the reference's scheme at L = 1; order 2: traceless quadrupole
+// source and second-order local expansion), bucket pairs P2P interactions.
+//
+// Two builders of that set: the sink-major walk (default; one wavefront per
+// sink, depth by depth, M2L and L2L summed in the walk, P2P lists chained
+// in a chunk pool) and the level-synchronous pair recursion (one thread per
+// pair; lists bucketed by sink, M2L one wavefront per sink cell, L2L level
+// by level), which also takes over when a walk's stack would overflow.
+// Then one workgroup per bucket evaluates its local expansion and sums its
+// P2P sources from LDS.  Centres of mass and radii are the FP64 octree
+// aggregates; multipole and pair arithmetic is FP32 with FP64 position
+// differences.
 #include <cfloat>
 #include <type_traits>
 
@@ -696,21 +700,23 @@ __device__ void walk_sink(const WalkArgs<T> &a, int32_t A, int32_t P,
       kind = ba ? 2 : 0;  // a cell meets itself: its children's siblings
     } else if (B >= 0) {
 #pragma clang fp contract(off)
-      const double d0 = ax - c.com[3 * (int64_t)B],
-                   d1 = ay - c.com[3 * (int64_t)B + 1],
-                   d2 = az - c.com[3 * (int64_t)B + 2];
+      // everything the pair may need, loaded at once (one memory latency
+      // per round instead of two)
+      const double bx = c.com[3 * (int64_t)B], by = c.com[3 * (int64_t)B + 1],
+                   bz = c.com[3 * (int64_t)B + 2];
+      const double rb = c.rad[B], mb = c.mass[B];
+      const bool bb = c.bucket[B] != 0;
+      T Qb[6] = {0, 0, 0, 0, 0, 0};
+      if (a.order >= 2)
+        for (int k = 0; k < 6; ++k) Qb[k] = a.Q[6 * (int64_t)B + k];
+      const double d0 = ax - bx, d1 = ay - by, d2 = az - bz;
       double s = 0;
       s += d0 * d0;
       s += d1 * d1;
       s += d2 * d2;
-      const double rb = c.rad[B];
-      const bool bb = c.bucket[B] != 0;
       if (ra + rb < a.theta * sqrt(s)) {
         kind = 1;
-        T Qb[6] = {0, 0, 0, 0, 0, 0};
-        if (a.order >= 2)
-          for (int k = 0; k < 6; ++k) Qb[k] = a.Q[6 * (int64_t)B + k];
-        m2l_add<T>(a.G, a.order, (T)d0, (T)d1, (T)d2, (T)c.mass[B], Qb, Lk);
+        m2l_add<T>(a.G, a.order, (T)d0, (T)d1, (T)d2, (T)mb, Qb, Lk);
         ++m2l;
       } else if (ba && bb) {
         kind = 2;
@@ -752,10 +758,30 @@ __device__ void walk_sink(const WalkArgs<T> &a, int32_t A, int32_t P,
     visit(lane == 0 ? A : -1);
     drain();
   } else {
-    for (int32_t ch = a.dhead[P]; ch >= 0 && !over; ch = a.l.next[ch]) {
+    // D'(P) chunk by chunk, the next chunk's header and entries loaded
+    // while the current one is walked (a chunk's 64 slots are all in the
+    // pool; the ones past its length are ignored)
+    int32_t ch = a.dhead[P], nch = -1;
+    int32_t B = -1;
+    if (ch >= 0) {
       const int len = a.l.len[ch];
-      visit(lane < len ? a.l.pool[kChunk * (int64_t)ch + lane] : -1);
+      nch = a.l.next[ch];
+      B = a.l.pool[kChunk * (int64_t)ch + lane];
+      B = lane < len ? B : -1;
+    }
+    while (ch >= 0 && !over) {
+      int nlen = 0;
+      int32_t nnch = -1, nB = -1;
+      if (nch >= 0) {
+        nlen = a.l.len[nch];
+        nnch = a.l.next[nch];
+        nB = a.l.pool[kChunk * (int64_t)nch + lane];
+      }
+      visit(B);
       drain();
+      ch = nch;
+      nch = nnch;
+      B = lane < nlen ? nB : -1;
     }
   }
   if (over) {

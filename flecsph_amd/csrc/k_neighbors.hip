@@ -94,10 +94,20 @@ struct TravArgs {
 
 // MAXG: the largest group (64 when group_max <= 64, else kMaxGroup), which
 // sizes the group's LDS arrays.
+//
+// A node passes the sphere test if some particle's sphere meets its box.
+// The particles are taken eight at a time: a node box that misses the
+// hull box of a chunk's spheres (each sphere's bounding cube, widened by
+// 1e-9 of |centre| + radius, well beyond the rounding of the sphere test)
+// fails the sphere test of every particle of the chunk, so only chunks
+// whose hull it meets are tested particle by particle.  The accepted nodes
+// are the same.
+constexpr int kHullChunk = 8;
 template <int kStack, int MAXG, bool GF>
 __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
 #pragma clang fp contract(off)
   __shared__ double sx[MAXG], sy[MAXG], sz[MAXG], sh[MAXG];
+  __shared__ double hull[MAXG / kHullChunk][6];
   __shared__ int32_t stk[kStack];
   const int lane = threadIdx.x;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
@@ -117,16 +127,17 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     for (int d = 0; d < 6; ++d) gb[d] = a.box[6 * (int64_t)gn + d];
   } else {
     // Without a group node (gather mode, single-particle groups): the box
-    // hull of the group's spheres, slightly widened.  A node that meets a
-    // sphere meets the hull, so testing it first only saves sphere tests.
+    // hull of the group's spheres, widened as the chunk hulls below.  A
+    // node that meets a sphere meets the hull, so testing it first only
+    // saves sphere tests.
     for (int d = 0; d < 3; ++d) {
       gb[d] = DBL_MAX;
       gb[d + 3] = -DBL_MAX;
     }
     for (int k = lane; k < gc; k += kW) {
-      const double r = sh[k] * (1.0 + 1e-9);
       const double c[3] = {sx[k], sy[k], sz[k]};
       for (int d = 0; d < 3; ++d) {
+        const double r = sh[k] + 1e-9 * (fabs(c[d]) + sh[k]);
         gb[d] = fmin(gb[d], c[d] - r);
         gb[d + 3] = fmax(gb[d + 3], c[d] + r);
       }
@@ -155,6 +166,25 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     for (int o = 32; o > 0; o >>= 1) frad = fmax(frad, __shfl_xor(frad, o, 64));
     frad *= 1.0 + 1e-9;
   }
+  __syncthreads();
+  const int nhull = (gc + kHullChunk - 1) / kHullChunk;
+  if (lane < nhull) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX},
+           hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    const int k1 = min(gc, (lane + 1) * kHullChunk);
+    for (int k = lane * kHullChunk; k < k1; ++k) {
+      const double c[3] = {sx[k], sy[k], sz[k]};
+      for (int d = 0; d < 3; ++d) {
+        const double pad = 1e-9 * (fabs(c[d]) + sh[k]);
+        lo[d] = fmin(lo[d], c[d] - sh[k] - pad);
+        hi[d] = fmax(hi[d], c[d] + sh[k] + pad);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      hull[lane][d] = lo[d];
+      hull[lane][d + 3] = hi[d];
+    }
+  }
   if (lane == 0) stk[0] = a.root ? a.root[a.inst[gs]] : 0;  // root
   __syncthreads();
   int sp = 1;
@@ -172,11 +202,15 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
       double bx[6];
       for (int d = 0; d < 6; ++d) bx[d] = a.box[6 * (int64_t)node + d];
       if (box_box(bx, gb)) {
-        for (int k = 0; k < gc; ++k)
-          if (sphere_box(bx, sx[k], sy[k], sz[k], sh[k])) {
-            acc = true;
-            break;
-          }
+        for (int q = 0; q < nhull && !acc; ++q) {
+          if (!box_box(bx, hull[q])) continue;
+          const int k1 = min(gc, (q + 1) * kHullChunk);
+          for (int k = q * kHullChunk; k < k1; ++k)
+            if (sphere_box(bx, sx[k], sy[k], sz[k], sh[k])) {
+              acc = true;
+              break;
+            }
+        }
       }
       if (acc)
         for (int j = 0; j < 8; ++j) ch[j] = a.child[8 * (int64_t)node + j];
