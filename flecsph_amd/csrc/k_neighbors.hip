@@ -199,8 +199,11 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
     bool acc = false;
     int32_t ch[8];
     if (node >= 0) {
+      // the children are loaded with the box (one memory latency per step;
+      // a rejected node's are not used)
       double bx[6];
       for (int d = 0; d < 6; ++d) bx[d] = a.box[6 * (int64_t)node + d];
+      for (int j = 0; j < 8; ++j) ch[j] = a.child[8 * (int64_t)node + j];
       if (box_box(bx, gb)) {
         for (int q = 0; q < nhull && !acc; ++q) {
           if (!box_box(bx, hull[q])) continue;
@@ -212,8 +215,6 @@ __global__ __launch_bounds__(kW) void k_traverse(TravArgs a) {
             }
         }
       }
-      if (acc)
-        for (int j = 0; j < 8; ++j) ch[j] = a.child[8 * (int64_t)node + j];
     }
     int nnode = 0, nleaf = 0;
     uint32_t keep = 0;  // leaf children that pass the leaf filter
