@@ -525,7 +525,10 @@ int fsph_enable_graph(fsph_ctx *ctx, int on);
    more than 64 particles is processed as two tiles of its rows, each with
    the group's candidates that neighbour one of its rows) and candidate
    chunk (64, 128, 256 candidates staged in LDS per round), keeps the
-   fastest and returns it.  group_max, the reference's grouping
+   fastest and returns it.  Candidates are compared on the time of the
+   phases the tuning changes (FSPH_T_MASKS, _DENSITY, _VSIG, _ACCEL,
+   _DUDT, _H and _GRAPH, by HIP events), not of whole steps.  group_max,
+   the reference's grouping
    (tree_topology::sub_entities_, tree_topology.h:222-235,2121), is not
    searched: the traversal's node test uses the whole group
    (tree_topology.h:323-355), so the neighbour lists depend on it.  The
