@@ -252,6 +252,7 @@ struct Ctx {
     int stack_limit = 0;               // 0: the full LDS stack
     int64_t cap_chunks = 0, chunks_used = 0;
     DBuf iflag, ipos, parents, lpool, lnext, llen, dhead, phead, pcnt, mcnt;
+    DBuf crec;  // packed per-cell records the walk reads
   } grav;
 
   // time stepping

@@ -600,6 +600,30 @@ __global__ __launch_bounds__(kT) void k_l2l_single(
 // P2P chains are then copied into a CSR list.  The pair set is the
 // recursion's, so the result is the same up to FP32 summation order.
 // ---------------------------------------------------------------------------
+// What the walk reads of a candidate cell, in one record (one 64-byte
+// line for FP32 multipoles) instead of five arrays: centre, radius, mass
+// with the bucket flag as its sign (cells have positive mass), quadrupole.
+template <class T> struct alignas(sizeof(T) == 4 ? 64 : 128) CellRec {
+  double cx, cy, cz, rad, smass;
+  T Q[6];
+};
+
+template <class T>
+__global__ __launch_bounds__(kT) void k_cell_rec(Cells c, int64_t nc,
+                                                 const T *__restrict__ Q,
+                                                 CellRec<T> *__restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nc) return;
+  CellRec<T> v;
+  v.cx = c.com[3 * i];
+  v.cy = c.com[3 * i + 1];
+  v.cz = c.com[3 * i + 2];
+  v.rad = c.rad[i];
+  v.smass = c.bucket[i] ? -c.mass[i] : c.mass[i];
+  for (int k = 0; k < 6; ++k) v.Q[k] = Q[6 * i + k];
+  r[i] = v;
+}
+
 constexpr int kWalkStack = 1024;  // LDS stack entries of a sink's walk
 constexpr int kChunk = 64;        // entries per list chunk
 constexpr int kWalkGrid = 8192;   // workgroups of a depth (grid-stride)
@@ -622,6 +646,7 @@ struct ListPool {
 
 template <class T> struct WalkArgs {
   Cells c;
+  const CellRec<T> *rec;
   const T *Q;
   T *L;
   T G;
@@ -700,16 +725,14 @@ __device__ void walk_sink(const WalkArgs<T> &a, int32_t A, int32_t P,
       kind = ba ? 2 : 0;  // a cell meets itself: its children's siblings
     } else if (B >= 0) {
 #pragma clang fp contract(off)
-      // everything the pair may need, loaded at once (one memory latency
-      // per round instead of two)
-      const double bx = c.com[3 * (int64_t)B], by = c.com[3 * (int64_t)B + 1],
-                   bz = c.com[3 * (int64_t)B + 2];
-      const double rb = c.rad[B], mb = c.mass[B];
-      const bool bb = c.bucket[B] != 0;
-      T Qb[6] = {0, 0, 0, 0, 0, 0};
-      if (a.order >= 2)
-        for (int k = 0; k < 6; ++k) Qb[k] = a.Q[6 * (int64_t)B + k];
-      const double d0 = ax - bx, d1 = ay - by, d2 = az - bz;
+      // everything the pair may need, in one record (one memory latency
+      // per round, one cache line per candidate)
+      const CellRec<T> rb_ = a.rec[B];
+      const double rb = rb_.rad, mb = fabs(rb_.smass);
+      const bool bb = rb_.smass < 0;
+      T Qb[6];
+      for (int k = 0; k < 6; ++k) Qb[k] = a.order >= 2 ? rb_.Q[k] : T(0);
+      const double d0 = ax - rb_.cx, d1 = ay - rb_.cy, d2 = az - rb_.cz;
       double s = 0;
       s += d0 * d0;
       s += d1 * d1;
@@ -1308,6 +1331,9 @@ int walk_path(Ctx *c, const Cells &cells, const T *Q, T *Lc, int *fallback) {
   FSPH_TRY(ensure(c, G.p2p_off, (size_t)(nc + 1) * 8));
   FSPH_TRY(ensure(c, G.ctr, (size_t)kStripes * kStripeWords * 8));
   PhaseTimer t_walk(c, FSPH_T_GRAV_PAIRS);
+  FSPH_TRY(ensure(c, G.crec, (size_t)(nc + 1) * sizeof(CellRec<T>)));
+  k_cell_rec<T><<<blocks_for(nc), kT, 0, c->stream>>>(
+      cells, nc, Q, G.crec.as<CellRec<T>>());
   // internal cells in depth order
   k_internal_flags<<<blocks_for(nn), kT, 0, c->stream>>>(
       c->nlevel.as<int32_t>(), nn, G.nodecell.as<int32_t>(),
@@ -1335,6 +1361,7 @@ int walk_path(Ctx *c, const Cells &cells, const T *Q, T *Lc, int *fallback) {
                                c->stream));
     FSPH_HIP(c, hipMemsetAsync(wflag, 0, sizeof(int), c->stream));
     WalkArgs<T> wa{cells,
+                   G.crec.as<CellRec<T>>(),
                    Q,
                    Lc,
                    (T)c->p.G,
@@ -1558,7 +1585,7 @@ void free_gravity(Ctx *c) {
                  &G.qb,      &G.m2l,     &G.p2p,      &G.ctr,      &G.m2l_off,
                  &G.m2l_src, &G.p2p_off, &G.p2p_src,  &G.cursor,   &G.iflag,
                  &G.ipos,    &G.parents, &G.lpool,    &G.lnext,    &G.llen,
-                 &G.dhead,   &G.phead,   &G.pcnt,     &G.mcnt};
+                 &G.dhead,   &G.phead,   &G.pcnt,     &G.mcnt,     &G.crec};
   for (DBuf *b : all)
     if (b->p) {
       hipFree(b->p);
