@@ -1,15 +1,18 @@
 // k_mlp.hip -- the learned smoothing-length predictor: a 6 -> 64 -> 64 -> 1
-// tanh MLP with BF16 weights and FP32 accumulation on MFMA
-// (v_mfma_f32_32x32x8_bf16), its backward pass and Adam.
+// tanh MLP with BF16 weights and FP32 accumulation on MFMA, its backward
+// pass and Adam.
 //
-// Operand layout of v_mfma_f32_32x32x8_bf16 (wave64): lane l supplies
-// A[l%32][4*(l/32)+i] and B[4*(l/32)+i][l%32], i = 0..3; result register r
-// of lane l is D[8*(r/4) + 4*(l/32) + r%4][l%32].
+// MFMA forms (wave64): the K = 64 products use gfx950's
+// v_mfma_f32_32x32x16_bf16, where lane l supplies A[l%32][8*(l/32)+i] and
+// B[8*(l/32)+i][l%32], i = 0..7; layer 1 (K = 8) uses
+// v_mfma_f32_32x32x8_bf16, lane l supplying A[l%32][4*(l/32)+i] and
+// B[4*(l/32)+i][l%32], i = 0..3.  In both, result register r of lane l is
+// D[8*(r/4) + 4*(l/32) + r%4][l%32].
 //
 // Forward: one wavefront per tile of 32 samples.  Layer 1 (K = 8, inputs
-// padded from 6) and layer 2 (K = 64) are MFMA products; activations go
-// through LDS to change from the result layout to the operand layout; the
-// 64 -> 1 output layer is a VALU dot product.
+// padded from 6) and layer 2 (K = 64, four K = 16 products) are MFMA
+// products; activations go through LDS to change from the result layout to
+// the operand layout; the 64 -> 1 output layer is a VALU dot product.
 // Backward: persistent workgroups of two wavefronts (64 samples) recompute
 // the forward pass, then dW2 = H1^T dZ2 (K = 64 samples) and dH1 = dZ2 W2^T
 // run on MFMA, the rest on VALU; partial gradients stay in registers/LDS
@@ -19,6 +22,7 @@
 namespace fsph {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
@@ -40,6 +44,12 @@ __device__ inline float bf2f(unsigned short b) {
 
 __device__ inline f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline __bf16 as_bf(unsigned short u) {
+  return __builtin_bit_cast(__bf16, u);
 }
 
 __device__ inline int c_row(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
@@ -90,14 +100,14 @@ __device__ inline void forward_tile(const MlpW &w, const float *X, int64_t B,
   __syncthreads();
   for (int nt = 0; nt < 2; ++nt) {
     f32x16 c = {0};
-    for (int ks = 0; ks < kH / 8; ++ks) {
-      bf16x4 aa, b;
-      for (int i = 0; i < 4; ++i) {
-        const int k = ks * 8 + 4 * (l >> 5) + i;
-        aa[i] = (short)sH1[(l & 31) * kH + k];
-        b[i] = (short)w.W2b[k * kH + nt * 32 + (l & 31)];
+    for (int ks = 0; ks < kH / 16; ++ks) {
+      bf16x8 aa, b;
+      for (int i = 0; i < 8; ++i) {
+        const int k = ks * 16 + 8 * (l >> 5) + i;
+        aa[i] = as_bf(sH1[(l & 31) * kH + k]);
+        b[i] = as_bf(w.W2b[k * kH + nt * 32 + (l & 31)]);
       }
-      c = mfma(aa, b, c);
+      c = mfma16(aa, b, c);
     }
     const int col = nt * 32 + (l & 31);
     const float bias = w.p[kOffB2 + col];
@@ -185,27 +195,27 @@ __global__ __launch_bounds__(128) void k_mlp_bwd(MlpW w, const float *X,
     __syncthreads();
     // dW2[k][n] += sum_s H1[s][k] dZ2[s][n]: wave wv owns k in wv*32..+31
     for (int nt = 0; nt < 2; ++nt) {
-      for (int ks = 0; ks < 64 / 8; ++ks) {
-        bf16x4 a, b;
-        for (int i = 0; i < 4; ++i) {
-          const int s = ks * 8 + 4 * (l >> 5) + i;
-          a[i] = (short)sH1[s >> 5][(s & 31) * kH + wv * 32 + (l & 31)];
-          b[i] = (short)sdZ2[s * kH + nt * 32 + (l & 31)];
+      for (int ks = 0; ks < 64 / 16; ++ks) {
+        bf16x8 a, b;
+        for (int i = 0; i < 8; ++i) {
+          const int s = ks * 16 + 8 * (l >> 5) + i;
+          a[i] = as_bf(sH1[s >> 5][(s & 31) * kH + wv * 32 + (l & 31)]);
+          b[i] = as_bf(sdZ2[s * kH + nt * 32 + (l & 31)]);
         }
-        acc[nt] = mfma(a, b, acc[nt]);
+        acc[nt] = mfma16(a, b, acc[nt]);
       }
     }
     // dH1 = dZ2 W2^T for the wave's 32 samples; dZ1 = dH1 (1 - H1^2)
     for (int nt = 0; nt < 2; ++nt) {
       f32x16 c = {0};
-      for (int ks = 0; ks < kH / 8; ++ks) {
-        bf16x4 a, b;
-        for (int i = 0; i < 4; ++i) {
-          const int n = ks * 8 + 4 * (l >> 5) + i;
-          a[i] = (short)sdZ2[(wv * 32 + (l & 31)) * kH + n];
-          b[i] = (short)w.W2bT[n * kH + nt * 32 + (l & 31)];
+      for (int ks = 0; ks < kH / 16; ++ks) {
+        bf16x8 a, b;
+        for (int i = 0; i < 8; ++i) {
+          const int n = ks * 16 + 8 * (l >> 5) + i;
+          a[i] = as_bf(sdZ2[(wv * 32 + (l & 31)) * kH + n]);
+          b[i] = as_bf(w.W2bT[n * kH + nt * 32 + (l & 31)]);
         }
-        c = mfma(a, b, c);
+        c = mfma16(a, b, c);
       }
       const int col = nt * 32 + (l & 31);
       for (int r = 0; r < 16; ++r) {

@@ -484,9 +484,14 @@ __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
         if (in) word |= 1u << jj;
       }
       rw[w][k] = word;
-      if (a.nngb) atomicAdd(&cnt[k], __popc(word));
     }
     __syncthreads();
+    if (a.nngb)  // each row's count, by the lane that owns the row
+      for (int k = lane; k < gc; k += kW) {
+        int nk = 0;
+        for (int w = 0; w < nwr; ++w) nk += __popc(rw[w][k]);
+        cnt[k] += nk;
+      }
     if (r0 + kR < cc) prefetch(r0 + kR);
     {
       // candidates of the round that neighbour some row of each tile
