@@ -253,6 +253,7 @@ struct Ctx {
     int64_t cap_chunks = 0, chunks_used = 0;
     DBuf iflag, ipos, parents, lpool, lnext, llen, dhead, phead, pcnt, mcnt;
     DBuf crec;  // packed per-cell records the walk reads
+    DBuf prel;  // FP32: per particle, position - bucket centre and G m
   } grav;
 
   // time stepping

@@ -133,18 +133,25 @@ __device__ inline void add_quad(double Q[6], double m, double y0, double y1,
 }
 
 template <class T>
+// (FP32 runs: also each particle's position relative to its bucket's
+// centre of mass and G m, as one float4 that the P2P staging reads)
 __global__ __launch_bounds__(kT) void k_p2m(Cells c, int64_t nc,
                                             const double *__restrict__ x,
                                             const double *__restrict__ y,
                                             const double *__restrict__ z,
                                             const double *__restrict__ m,
-                                            T *__restrict__ Q) {
+                                            T *__restrict__ Q, T G,
+                                            float4 *__restrict__ prel) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= nc || !c.bucket[i]) return;
   double q[6] = {0, 0, 0, 0, 0, 0};
   const double cx = c.com[3 * i], cy = c.com[3 * i + 1], cz = c.com[3 * i + 2];
-  for (int64_t p = c.start[i]; p < c.start[i] + c.count[i]; ++p)
+  for (int64_t p = c.start[i]; p < c.start[i] + c.count[i]; ++p) {
     add_quad(q, m[p], x[p] - cx, y[p] - cy, z[p] - cz);
+    if (prel)
+      prel[p] = make_float4((float)(x[p] - cx), (float)(y[p] - cy),
+                            (float)(z[p] - cz), (float)(G * m[p]));
+  }
   for (int k = 0; k < 6; ++k) Q[6 * i + k] = (T)q[k];
 }
 
@@ -955,6 +962,7 @@ template <class T> struct EvalArgs {
   const int64_t *off;
   const int32_t *src;
   const T *L;
+  const float4 *prel;  // FP32 runs: position - bucket centre, G m
   const double *x, *y, *z, *m;
   T *gx, *gy, *gz, *phi;
   T G;
@@ -966,7 +974,9 @@ __device__ inline double fast_rsq(double x) { return 1.0 / sqrt(x); }
 constexpr int kEvalB = 64;            // threads per bucket workgroup
 constexpr int kStageBuckets = 32;     // at most this many source buckets
 constexpr int kStage = 256;           // and particles staged per round
-static_assert(kMaxLeaf <= kStage && kStageBuckets <= kEvalB, "staging");
+static_assert(kMaxLeaf <= kStage && kStageBuckets <= kEvalB &&
+                  kStage % kEvalB == 0,
+              "staging");
 
 // Positions in the P2P sums: FP32 evaluations stage them relative to the
 // sink bucket's centre of mass (one FP64 difference per staged source and
@@ -1029,8 +1039,11 @@ __device__ inline void p2p_terms(const float4 *s4, int q, int q1, float px,
 // lanes of a particle take contiguous, odd-length runs of the staged
 // sources (runs of odd length start in different LDS banks), summed by
 // p2p_terms.  FP64 evaluations keep the per-term form.
+// (the bound keeps the FP32 kernel at 79 VGPRs without spills: six
+// wavefronts per SIMD, which measured 1.24 -> 1.12 ms on the 1M polytrope
+// against five at 115 VGPRs)
 template <class T>
-__global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
+__global__ __launch_bounds__(kEvalB, 6) void k_eval(EvalArgs<T> a) {
   using P = EPos<T>;
   constexpr bool F32 = sizeof(T) == 4;
   __shared__ float4 s4[F32 ? kStage : 1];
@@ -1039,6 +1052,7 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   __shared__ T sm[F32 ? 1 : kStage];
   __shared__ int32_t sidx[F32 ? 1 : kStage];
   __shared__ int32_t bst[kStageBuckets], boff[kStageBuckets + 1];
+  __shared__ float4 bof[F32 ? kStageBuckets : 1];  // source - sink centre
   __shared__ T part[4][kEvalB];
   const int64_t cell = xcd_item(blockIdx.x, gridDim.x);
   if (!a.c.bucket[cell]) return;
@@ -1056,7 +1070,12 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
   KS<T> f0, f1, f2, pp;
   if (act) {
     const double X = a.x[p], Y = a.y[p], Z = a.z[p];
-    px = (P)(X - ox); py = (P)(Y - oy); pz = (P)(Z - oz);
+    if constexpr (F32) {
+      const float4 o = a.prel[p];
+      px = o.x; py = o.y; pz = o.z;
+    } else {
+      px = (P)(X - ox); py = (P)(Y - oy); pz = (P)(Z - oz);
+    }
     if (r == 0) {
       T phi, g0, g1, g2;
       local_eval(a.L + 10 * cell, (T)(X - cx), (T)(Y - cy), (T)(Z - cz),
@@ -1074,10 +1093,15 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
     // so the first one always does; the fitting lanes are a prefix)
     int32_t s = -1;
     int sc = 0, sst = 0;
+    float4 of = make_float4(0.f, 0.f, 0.f, 0.f);
     if (t < kStageBuckets && j0 + t < j1) {
       s = a.src[j0 + t];
       sc = a.c.count[s];
       sst = a.c.start[s];
+      if constexpr (F32)
+        of = make_float4((float)(a.c.com[3 * (int64_t)s] - cx),
+                         (float)(a.c.com[3 * (int64_t)s + 1] - cy),
+                         (float)(a.c.com[3 * (int64_t)s + 2] - cz), 0.f);
     }
     int tot;
     const int ex = wave_excl_scan(sc, &tot);
@@ -1087,13 +1111,15 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
     if (fit) {
       bst[t] = sst;
       boff[t] = ex;
+      if constexpr (F32) bof[t] = of;
     }
     if (t == nb - 1) boff[nb] = ex + sc;
     __syncthreads();
     const int nst = boff[nb];
     j0 += nb;
-    for (int q = t; q < nst; q += kEvalB) {
-      int i = 0, hi = nb - 1;  // the last bucket starting at or before q
+    // the bucket of staged slot q: the last one starting at or before q
+    auto bucket_of = [&](int q) {
+      int i = 0, hi = nb - 1;
       while (i < hi) {
         const int mid = (i + hi + 1) >> 1;
         if (boff[mid] <= q)
@@ -1101,11 +1127,35 @@ __global__ __launch_bounds__(kEvalB) void k_eval(EvalArgs<T> a) {
         else
           hi = mid - 1;
       }
-      const int64_t pi = bst[i] + (q - boff[i]);
-      if constexpr (F32) {
-        s4[q] = make_float4((float)(a.x[pi] - ox), (float)(a.y[pi] - oy),
-                            (float)(a.z[pi] - oz), (float)(a.G * a.m[pi]));
-      } else {
+      return i;
+    };
+    if constexpr (F32) {
+      // a source is its float4 relative to its own bucket's centre plus
+      // that bucket's offset from the sink's; the lane's loads are all
+      // issued before any is used
+      constexpr int kPer = kStage / kEvalB;
+      float4 v[kPer];
+      int bi[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int q = t + u * kEvalB;
+        bi[u] = -1;
+        if (q < nst) {
+          bi[u] = bucket_of(q);
+          v[u] = a.prel[bst[bi[u]] + (q - boff[bi[u]])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u)
+        if (bi[u] >= 0) {
+          const float4 o = bof[bi[u]];
+          s4[t + u * kEvalB] =
+              make_float4(v[u].x + o.x, v[u].y + o.y, v[u].z + o.z, v[u].w);
+        }
+    } else {
+      for (int q = t; q < nst; q += kEvalB) {
+        const int i = bucket_of(q);
+        const int64_t pi = bst[i] + (q - boff[i]);
         sx[q] = (P)(a.x[pi] - ox); sy[q] = (P)(a.y[pi] - oy);
         sz[q] = (P)(a.z[pi] - oz);
         sm[q] = (T)a.m[pi];
@@ -1489,9 +1539,14 @@ template <class T> int gravity_impl(Ctx *c) {
   T *Q = G.cQ.as<T>();
   T *Lc = G.cL.as<T>();
   // ---- multipoles: P2M on buckets, M2M up the node levels
+  float4 *prel = nullptr;
+  if (sizeof(T) == 4) {
+    FSPH_TRY(ensure(c, G.prel, (size_t)(n + 1) * sizeof(float4)));
+    prel = G.prel.as<float4>();
+  }
   k_p2m<T><<<blocks_for(nc), kT, 0, c->stream>>>(
       cells, nc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
-      c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(), Q);
+      c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(), Q, (T)c->p.G, prel);
   for (int d = c->max_depth; d >= 0; --d) {
     const int64_t cnt = c->level_off[d + 1] - c->level_off[d];
     if (!cnt) continue;
@@ -1507,7 +1562,8 @@ template <class T> int gravity_impl(Ctx *c) {
   if (fallback) FSPH_TRY(pairs_path<T>(c, cells, Q, Lc));
   // ---- L2P + P2P
   EvalArgs<T> ea{cells, nc, G.p2p_off.as<int64_t>(), G.p2p_src.as<int32_t>(),
-                 Lc, c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
+                 Lc, sizeof(T) == 4 ? G.prel.as<float4>() : nullptr,
+                 c->f[PF_X].as<double>(), c->f[PF_Y].as<double>(),
                  c->f[PF_Z].as<double>(), c->f[PF_M].as<double>(),
                  c->f[PF_GX].as<T>(), c->f[PF_GY].as<T>(), c->f[PF_GZ].as<T>(),
                  c->f[PF_PHI].as<T>(), (T)c->p.G};
