@@ -366,7 +366,7 @@ __device__ inline uint32_t compress(uint32_t x, uint32_t m, const Compress &c) {
 template <bool GATHER, int MAXG, int NH>
 __global__ __launch_bounds__(kW) void k_masks(MaskArgs a) {
 #pragma clang fp contract(off)
-  constexpr int kR = 128;  // candidates per round (4 mask words)
+  constexpr int kR = 64;  // candidates per round (2 mask words)
   constexpr int kRW = kR / 32;
   __shared__ double cx[kR], cy[kR], cz[kR], ch2[kR];
   __shared__ double px[MAXG], py[MAXG], pz[MAXG], ph2[MAXG];
