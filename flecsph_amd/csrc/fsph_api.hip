@@ -421,7 +421,8 @@ void fsph_destroy(fsph_ctx *ctx) {
                        &c->ccount, &c->moff, &c->mask, &c->gcand,
                        &c->gccount, &c->gmoff, &c->gmask, &c->h_prop,
                        &c->h_prev, &c->hflag, &c->glist, &c->hgflag,
-                       &c->hgscan, &c->hcnt};
+                       &c->hgscan, &c->hcnt, &c->hflag2, &c->hgflag2,
+                       &c->glist2};
   for (auto *b : all)
     if (b->p) hipFree(b->p);
   fsph::free_gravity(c);

@@ -177,6 +177,8 @@ struct Ctx {
   DBuf glist;                // compacted group list (int32)
   DBuf hgflag, hgscan;       // per-group fallback flags and their scan
   DBuf hcnt;                 // int32 counters: [0] flagged, [1] clamped
+  DBuf hflag2, hgflag2, glist2;  // the same for the learned path's second
+                                 // solve (gather tiles around the proposal)
 
   // learned smoothing-length predictor (k_mlp.hip)
   struct Mlp {

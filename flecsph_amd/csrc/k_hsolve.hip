@@ -87,6 +87,13 @@ template <class T> struct NewtonArgs {
   int32_t *hngb, *hiter;
   int kernel, maxit;
   double kappa, c3, tol, ntol;
+  // Tiles of the SPH passes instead of gather tiles: slot[t] is tile t's
+  // candidate slot, and the bracket's upper end is capped at the current h
+  // (the SPH masks hold every candidate within h of a row, tree_topology.h:
+  // 365-379), so a root beyond it is clamped and re-solved, and a particle
+  // whose starting h (the proposal) lies beyond it is flagged unsolved.
+  const int32_t *slot;
+  int cap_h;
 };
 
 constexpr int kBand = 4;  // band entries kept per particle
@@ -108,11 +115,13 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
   const int gc = a.gcount[g], cc = a.ccount[gi];
-  const int64_t mo = a.moff[gi], cb = gi * a.ccap;
+  const int64_t mo = a.moff[gi],
+                cb = (int64_t)(a.slot ? a.slot[gi] : gi) * a.ccap;
   const int t = threadIdx.x;
   const bool active = t < gc;  // thread t owns particle t
   const int64_t ia = gs + t;
   double px = 0, py = 0, pz = 0, ma = 0, H = 1, h = 1, lo = 0, hi = 1;
+  bool defer = false;  // a proposal beyond the capped bracket: not solved
   if (active) {
     px = a.x[ia]; py = a.y[ia]; pz = a.z[ia];
     ma = a.m[ia];
@@ -123,15 +132,21 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
       if (a.h0_bracket) {
         H = a.kappa * p;
         h = p;
+        if (a.cap_h) {
+          H = fmin(H, a.h[ia]);
+          defer = p > H;
+        }
       } else if (p > 0 && p < H) {
         h = p;
       }
     }
     hi = H;
   }
-  const bool solve = active && !(a.only && !a.only[ia]);
-  bool done = !solve, conv = false, need_count = false;
-  double f0 = 0;
+  // only[i] != 0 selects the particles to solve; in a pass that flags,
+  // 2 (a proposal already found off the target) is handed on unsolved
+  const int onl = a.only && active ? a.only[ia] : 1;
+  const bool solve = active && !defer && (a.flag ? onl == 1 : onl != 0);
+  bool done = !solve, conv = false, need_count = false, off = false;
   int it = 0, nb = 0;
   // The hard count #{b : r_ab < h} is wanted at the final h, one Newton
   // update after the last evaluation.  Each evaluation therefore also keeps
@@ -216,21 +231,41 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
       if (eval) nb = cnt;
       break;
     }
+    if (!done && it == 0 && a.flag) {
+      // With flags, a proposal whose weighted count (4pi/3) h^3 rho~ / m + 1
+      // is off the target by more than ntol is given up at once: its h is
+      // restored and re-solved with the full radius.
+      const double f0 = (double)rho * h * h * h - ma * a.c3;
+      off = (4.0 * M_PI / 3.0) * fabs(f0) / ma > a.ntol;
+      if (off) {
+        ++it;
+        done = true;
+      }
+    }
     if (!done) {
       const double h3 = h * h * h;
       const double f = (double)rho * h3 - ma * a.c3;
       const double fp = 3.0 * h * h * (double)rho + h3 * (double)drho;
-      if (it == 0) f0 = f;
       if (f < 0)
         lo = h;
       else
         hi = h;
       double hn = fp > 0 ? h - f / fp : -1.0;
+      // In a capped bracket with no evaluation at or above the root yet, a
+      // step from below that is no Newton step inside the bracket (it
+      // leaves it, or the slope is not positive) marks a root taken to lie
+      // beyond the cap: given up, and re-solved with the wider radius.  So
+      // every estimate kept there is a Newton step, or lies in a bracket
+      // [lo, hi] with f(hi) >= 0 evaluated.
+      const bool beyond =
+          a.cap_h && f < 0 && hi == H && !(hn > lo && hn <= H);
       if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
-      const bool cv = fabs(hn - h) < a.tol * hn;
-      const bool fin = cv || it + 1 >= a.maxit;
+      const bool cv = !beyond && fabs(hn - h) < a.tol * hn;
+      const bool fin = cv || beyond || it + 1 >= a.maxit;
       ++it;
-      if (fin) {
+      if (beyond) {
+        done = true;
+      } else if (fin) {
         const double hn2 = hn * hn;
         const int nbd = bandn[t];
         int adj = 0;
@@ -244,7 +279,7 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
         else
           nb = cnt + adj;
       }
-      h = hn;
+      if (!beyond) h = hn;
     }
     if (!__syncthreads_or(!done)) {
       if (!__syncthreads_or(need_count)) break;
@@ -253,24 +288,30 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
   }
   if (a.flag) {
     // A root at the bracket's upper end lies beyond the gather radius; a
-    // proposal whose weighted count (4pi/3) h^3 rho~ / m + 1 is off the
-    // target by more than ntol is not trusted either.
-    const bool clamped = solve && (!conv || h >= H * (1.0 - 2.0 * a.tol));
-    const bool far = solve && (4.0 * M_PI / 3.0) * fabs(f0) / ma > a.ntol;
-    const bool fl = clamped || far;
+    // proposal off the target is not trusted either (flag 2: handed on
+    // to the full-radius solve unsolved).  Under the cap at the current h
+    // every candidate within the final h <= H was seen, so only a final h
+    // beyond the cap is re-solved.
+    const bool at_end =
+        a.cap_h ? h > H : h >= H * (1.0 - 2.0 * a.tol);
+    const bool clamped = solve && !off && (!conv || at_end);
+    const bool far = off || (active && onl == 2);
+    const bool fl = clamped || far || defer;
     const int nf = __syncthreads_count(fl);
-    const int ncl = __syncthreads_count(clamped);
+    const int ncl = __syncthreads_count(clamped || defer);
     if (t == 0) {
       a.gflag[gi] = nf > 0;
       if (nf) atomicAdd(&a.cnt[0], nf);
       if (ncl) atomicAdd(&a.cnt[1], ncl);
     }
-    if (active) a.flag[ia] = fl;
+    if (active) a.flag[ia] = far ? 2 : (int32_t)fl;
   }
   if (solve) {
     a.h[ia] = h;
     a.hngb[ia] = nb;
     a.hiter[ia] = a.only ? a.hiter[ia] + it : it;
+  } else if (defer) {
+    a.hiter[ia] = 0;  // the re-solve adds its iterations
   }
 }
 
@@ -300,6 +341,22 @@ NewtonArgs<T> newton_args(Ctx *c, double kappa, const int32_t *glist) {
   a.c3 = 3.0 / (4.0 * M_PI) * (c->p.n_target - 1);
   a.tol = c->p.newton_tol > 0 ? c->p.newton_tol : 1e-4;
   a.ntol = c->p.n_tol > 0 ? c->p.n_tol : 5;
+  return a;
+}
+
+// The SPH passes' tiles of this step (built for the current positions and
+// h, the rows' candidates compacted to those within max(h_a, h_b) of some
+// row): a Newton solve whose bracket stays at or below the current h finds
+// every gather neighbour there.
+template <class T> NewtonArgs<T> newton_args_sph(Ctx *c, double kappa) {
+  NewtonArgs<T> a = newton_args<T>(c, kappa, nullptr);
+  a.cand = c->cand.as<int32_t>();
+  a.ccount = c->tccount.as<int32_t>();
+  a.moff = c->tmoff.as<int64_t>();
+  a.mask = c->mask.as<uint32_t>();
+  a.ccap = c->ccap;
+  a.slot = c->tslot.as<int32_t>();
+  a.cap_h = 1;
   return a;
 }
 
@@ -439,15 +496,35 @@ __global__ __launch_bounds__(256) void k_dlh(const double *h,
   if (i < n) dlh[i] = (T)log(h[i] / h_old[i]);
 }
 
+// The flagged slots of a solve, as tile ids: map[s] for slot s (or s).
 __global__ __launch_bounds__(256) void k_group_list(const int32_t *gflag,
                                                     const int64_t *gscan,
                                                     int64_t ng,
+                                                    const int32_t *map,
                                                     int32_t *glist) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g < ng && gflag[g]) glist[gscan[g]] = (int32_t)g;
+  if (g < ng && gflag[g]) glist[gscan[g]] = map ? map[g] : (int32_t)g;
 }
 
 inline unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// counters of the last solve: [0] flagged particles, [1] clamped
+int read_h_counts(Ctx *c, int32_t cnt[2]) {
+  FSPH_HIP(c, hipMemcpyAsync(cnt, c->hcnt.p, 8, hipMemcpyDeviceToHost,
+                             c->stream));
+  FSPH_HIP(c, hipStreamSynchronize(c->stream));
+  return FSPH_OK;
+}
+
+// The tiles of the nslots slots flagged in gflag, into list (*nl of them).
+int flagged_tiles(Ctx *c, const int32_t *gflag, int64_t nslots,
+                  const int32_t *map, int32_t *list, int64_t *nl) {
+  FSPH_TRY(scan_i32_to_i64(c, gflag, c->hgscan.as<int64_t>(), nslots));
+  FSPH_TRY(read_scan_total(c, c->hgscan.as<int64_t>() + nslots, nl));
+  k_group_list<<<blocks256(nslots), 256, 0, c->stream>>>(
+      gflag, c->hgscan.as<int64_t>(), nslots, map, list);
+  return hip_check(c, hipGetLastError(), "k_group_list");
+}
 
 template <class T> FeatArgs<T> feat_args(Ctx *c) {
   FeatArgs<T> a{};
@@ -578,6 +655,9 @@ template <class T> int learned_impl(Ctx *c) {
     FSPH_TRY(ensure(c, c->hgscan, (size_t)(c->nt + 1) * 8));
     FSPH_TRY(ensure(c, c->glist, (size_t)c->nt * 4));
     FSPH_TRY(ensure(c, c->hcnt, 2 * 4));
+    FSPH_TRY(ensure(c, c->hflag2, (size_t)n * 4));
+    FSPH_TRY(ensure(c, c->hgflag2, (size_t)c->nt * 4));
+    FSPH_TRY(ensure(c, c->glist2, (size_t)c->nt * 4));
     {
       PhaseTimer tm(c, FSPH_T_H);
       if (M.use_model) {
@@ -605,43 +685,63 @@ template <class T> int learned_impl(Ctx *c) {
     const double kl = std::exp(std::min(std::log(kappa),
                                         std::max(0.03, 6.0 * rms)));
     M.kappa = (float)kl;
-    FSPH_TRY(stage_gather_tiles(c, kl, c->h_prop.as<float>()));
-    c->h_gather_words += c->gmask_words;
     c->h_kappa = kl;
+    // The bracket (0, min(kl h_p, h)] lies within the SPH tiles of this
+    // step, so the solve runs on them and no gather tiles are built; a
+    // particle whose root lies beyond its current h is clamped there and
+    // re-solved below with the full gather radius.
     FSPH_HIP(c, hipMemsetAsync(c->hcnt.p, 0, 8, c->stream));
-    NewtonArgs<T> a = newton_args<T>(c, kl, nullptr);
+    NewtonArgs<T> a = newton_args_sph<T>(c, kl);
     a.h0 = c->h_prop.as<float>();
     a.h0_bracket = 1;
     a.flag = c->hflag.as<int32_t>();
     a.gflag = c->hgflag.as<int32_t>();
     a.cnt = c->hcnt.as<int32_t>();
     FSPH_TRY(launch_newton(c, a, c->nt));
+    // Then, for the particles flagged there (root beyond the current h, or
+    // a proposal off the target), the solve on gather tiles of radius
+    // kl h_p around the proposal, built for their tiles only; what that
+    // still flags falls back to plain Newton-h with the full radius.
     int32_t cnt[2] = {0, 0};
-    FSPH_HIP(c, hipMemcpyAsync(cnt, c->hcnt.p, 8, hipMemcpyDeviceToHost,
-                               c->stream));
-    FSPH_HIP(c, hipStreamSynchronize(c->stream));
-    c->h_fallback = cnt[0];
-    c->h_clamped = cnt[1];
+    FSPH_TRY(read_h_counts(c, cnt));
+    c->h_fallback = c->h_clamped = 0;
     if (cnt[0] > 0) {
-      FSPH_TRY(scan_i32_to_i64(c, c->hgflag.as<int32_t>(),
-                               c->hgscan.as<int64_t>(), c->nt));
       int64_t ngl = 0;
-      FSPH_TRY(read_scan_total(c, c->hgscan.as<int64_t>() + c->nt, &ngl));
-      k_group_list<<<blocks256(c->nt), 256, 0, c->stream>>>(
-          c->hgflag.as<int32_t>(), c->hgscan.as<int64_t>(), c->nt,
-          c->glist.as<int32_t>());
-      FSPH_HIP(c, hipGetLastError());
-      k_restore<<<blocks256(n), 256, 0, c->stream>>>(
-          c->hflag.as<int32_t>(), c->h_prev.as<double>(), n,
-          c->f[PF_H].as<double>());
-      FSPH_HIP(c, hipGetLastError());
-      FSPH_HIP(c, hipStreamSynchronize(c->stream));
-      FSPH_TRY(stage_gather_tiles(c, kappa, nullptr, c->glist.as<int32_t>(),
-                                  ngl));
+      FSPH_TRY(flagged_tiles(c, c->hgflag.as<int32_t>(), c->nt, nullptr,
+                             c->glist.as<int32_t>(), &ngl));
+      FSPH_TRY(stage_gather_tiles(c, kl, c->h_prop.as<float>(),
+                                  c->glist.as<int32_t>(), ngl));
       c->h_gather_words += c->gmask_words;
-      NewtonArgs<T> b = newton_args<T>(c, kappa, c->glist.as<int32_t>());
+      // (the second solve writes flags only for its tiles' particles)
+      FSPH_HIP(c, hipMemsetAsync(c->hflag2.p, 0, (size_t)n * 4, c->stream));
+      FSPH_HIP(c, hipMemsetAsync(c->hcnt.p, 0, 8, c->stream));
+      NewtonArgs<T> b = newton_args<T>(c, kl, c->glist.as<int32_t>());
+      b.h0 = c->h_prop.as<float>();
+      b.h0_bracket = 1;
       b.only = c->hflag.as<int32_t>();
+      b.flag = c->hflag2.as<int32_t>();
+      b.gflag = c->hgflag2.as<int32_t>();
+      b.cnt = c->hcnt.as<int32_t>();
       FSPH_TRY(launch_newton(c, b, ngl));
+      FSPH_TRY(read_h_counts(c, cnt));
+      c->h_fallback = cnt[0];
+      c->h_clamped = cnt[1];
+      if (cnt[0] > 0) {
+        int64_t ngl2 = 0;
+        FSPH_TRY(flagged_tiles(c, c->hgflag2.as<int32_t>(), ngl,
+                               c->glist.as<int32_t>(),
+                               c->glist2.as<int32_t>(), &ngl2));
+        k_restore<<<blocks256(n), 256, 0, c->stream>>>(
+            c->hflag2.as<int32_t>(), c->h_prev.as<double>(), n,
+            c->f[PF_H].as<double>());
+        FSPH_HIP(c, hipGetLastError());
+        FSPH_TRY(stage_gather_tiles(c, kappa, nullptr,
+                                    c->glist2.as<int32_t>(), ngl2));
+        c->h_gather_words += c->gmask_words;
+        NewtonArgs<T> f = newton_args<T>(c, kappa, c->glist2.as<int32_t>());
+        f.only = c->hflag2.as<int32_t>();
+        FSPH_TRY(launch_newton(c, f, ngl2));
+      }
     }
   }
   {

@@ -436,11 +436,14 @@ int fsph_h_stats(fsph_ctx *ctx, int64_t *fallback, int64_t *clamped,
                  int32_t *trained_steps, float *last_loss);
 
 /* Gather set of the last smoothing-length update (Newton-h or learned-h):
-   *kappa is the gather radius of its first pass in units of the particle's
-   h (h_search for Newton-h, the narrowed radius around the proposal for
-   learned-h), *mask_words the neighbour-mask words of every gather tile
-   built in the update (rows times 32-candidate words: the pair tests the
-   solve walks).  Both 0 after a closed-form update. */
+   *kappa is the bracket radius of its first solve in units of the
+   particle's h (h_search for Newton-h, the narrowed radius around the
+   proposal for learned-h), *mask_words the neighbour-mask words of the
+   gather tiles the update built (rows times 32-candidate words: the pair
+   tests of those tiles).  Learned-h solves first on the step's SPH tiles,
+   which it reuses (its bracket there stays within the current h), and
+   builds gather tiles only for the tiles it could not settle there.  Both
+   0 after a closed-form update. */
 int fsph_h_gather(fsph_ctx *ctx, double *kappa, int64_t *mask_words);
 
 /* Milliseconds spent in each phase of the last fsph_step, measured with

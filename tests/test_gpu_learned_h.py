@@ -91,10 +91,12 @@ def test_learned_h_gathers_fewer_candidates():
     """What the learned form gains over plain Newton-h: the same steps of
     the 40k collapse with self-gravity, run once with Newton-h and once with
     learned-h (5 training updates).  After training, each learned update
-    solves inside a gather radius fitted to the predictor's error instead
-    of h_search h, so its gather tiles hold far fewer candidate words (the
-    pair tests of the solve) for the same roots (the test above checks the
-    roots against the oracle)."""
+    solves first on the step's SPH tiles, inside a bracket fitted to the
+    predictor's error instead of h_search h, and builds gather tiles only
+    for the tiles it could not settle there (those narrowed the same way),
+    so it builds far fewer candidate words (the pair tests of the gather
+    tiles) for the same roots (the test above checks the roots against the
+    oracle)."""
     parts, prm = generators.collapse(n=40000)
     out = {}
     for mode in (_abi.H_NEWTON, _abi.H_LEARNED):
