@@ -492,6 +492,12 @@ def run_single(name, args, tuning_arg="", autotune_on=True):
                                   "leaf_size": prm["leaf_size"],
                                   "order": "quadrupole"})
         rec["gravity_stats"] = gstats
+        # M2L runs on FP32 VALU inside the list walk: each pair's operator
+        # depends on its own offset (centre-of-mass expansions on an
+        # adaptive octree), so a batch of pairs is a set of matrix-vector
+        # products with no operand reuse, which matrix cores cannot speed up
+        rec["m2l"] = {"engine": "fp32_valu", "pairs": gstats["m2l_pairs"],
+                      "mfma_utilisation": 0.0}
     return rec
 
 
