@@ -157,7 +157,7 @@ struct Ctx {
   DBuf tslot;                // per tile: candidate slot (int32)
   DBuf tccount, tmoff;       // per tile: candidates (int32), mask offset
   int64_t ccap = 2048;       // candidate capacity per slot
-  bool big_stack = false;    // traversal with the large LDS stack
+  int stack_tier[2] = {0, 0};  // traversal LDS stack size (SPH, gather)
   int chunk = 128;           // candidates per LDS round of the pair kernels
   // cand: (ng + nsplit) slots of ccap int32 (the traversal fills slot g of
   // group g; the masks kernel compacts it in place for the group's first
