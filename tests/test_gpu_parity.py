@@ -123,6 +123,32 @@ def test_neighbor_lists_bit_exact(oracle_mod, name, tile_rows):
     assert np.array_equal(g.get("nngb"), np.diff(oo).astype(float))
 
 
+def test_neighbor_lists_with_one_box_wide_sphere(oracle_mod):
+    """20,000 random particles, one of them with an h that spans the box:
+    its group's traversal accepts every node, so its pending nodes outgrow
+    the first LDS stack (512 entries; the same walk replayed on the CPU
+    peaks at 532) and the traversal is run again with the next one; its
+    candidate tile holds every particle, so the tile capacity (2048 per
+    slot at first) regrows four times.  The lists still equal the
+    oracle's."""
+    from flecsph_amd import set_tuning
+    parts, prm = generators.random_box(n=20000, seed=7, h_spread=1.0)
+    parts["h"][123] = 2.0
+    prm = dict(prm, precision=_abi.PREC_FP32)
+    g, o = both(oracle_mod, parts, prm)
+    set_tuning(g, 64, 128)
+    g.update_iteration()
+    o.update_iteration()
+    og, ig = g.neighbor_lists()
+    oo, io = o.neighbor_lists()
+    assert np.array_equal(og, oo)
+    assert np.diff(oo).max() == 20000  # the wide particle meets everyone
+    n = len(og) - 1
+    io_sorted = np.concatenate([np.sort(io[oo[i]:oo[i + 1]])
+                                for i in range(n)])
+    assert np.array_equal(ig, io_sorted)
+
+
 def rel_err(a, b, scale=None):
     scale = np.abs(b).max() if scale is None else scale
     return np.abs(a - b).max() / max(scale, 1e-300)
