@@ -98,23 +98,28 @@ template <class T> struct NewtonArgs {
 
 constexpr int kBand = 4;  // band entries kept per particle
 
-template <class T>
-__global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
-  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
-  __shared__ T sm[kChunk];
-  __shared__ int32_t sid[kChunk];
-  __shared__ uint32_t smask[kChunkW * kB];
+// KB threads per tile, thread t owning row t: 64 when every tile has at
+// most 64 rows (one wavefront, half the LDS), else 128.
+template <class T, int KB>
+__global__ __launch_bounds__(KB) void k_newton(NewtonArgs<T> a) {
+  // candidates staged per chunk: as many as threads with one wavefront
+  constexpr int CH = KB == 64 ? 64 : kChunk, CHW = CH / 32;
+  __shared__ double sx[CH], sy[CH], sz[CH];
+  __shared__ T sm[CH];
+  __shared__ int32_t sid[CH];
+  __shared__ uint32_t smask[CHW * KB];
   // row k: particle k's position, h^2, band bounds and 1/h for the walk
-  __shared__ double qx[kB], qy[kB], qz[kB], qh2[kB];
-  __shared__ T qih[kB];
-  __shared__ int32_t rcnt[kB], bandn[kB];
-  __shared__ double bandv[kB][kBand];
-  __shared__ Sched<kB> S;
-  __shared__ Parts<T, 2, kB> R;
+  __shared__ double qx[KB], qy[KB], qz[KB], qh2[KB];
+  __shared__ T qih[KB];
+  __shared__ int32_t rcnt[KB], bandn[KB];
+  __shared__ double bandv[KB][kBand];
+  __shared__ Sched<KB> S;
+  __shared__ Parts<T, 2, KB> R;
   const int64_t gi = xcd_item(blockIdx.x, gridDim.x);
   const int64_t g = a.glist ? a.glist[gi] : gi;
   const int64_t gs = a.gstart[g];
-  const int gc = a.gcount[g], cc = a.ccount[gi];
+  // (rows beyond KB cannot occur: the host picks KB from the tile width)
+  const int gc = min(a.gcount[g], KB), cc = a.ccount[gi];
   const int64_t mo = a.moff[gi],
                 cb = (int64_t)(a.slot ? a.slot[gi] : gi) * a.ccap;
   const int t = threadIdx.x;
@@ -166,16 +171,16 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
     rcnt[t] = 0;
     bandn[t] = 0;
     KSum<T> s0, s1;  // particle t's sums over the chunks
-    for (int c0 = 0; c0 < cc; c0 += kChunk) {
-      const int nch = min(kChunk, cc - c0), nw = (nch + 31) >> 5;
-      for (int j = t; j < nch; j += kB) {
+    for (int c0 = 0; c0 < cc; c0 += CH) {
+      const int nch = min(CH, cc - c0), nw = (nch + 31) >> 5;
+      for (int j = t; j < nch; j += KB) {
         const int32_t b = a.cand[cb + c0 + j];
         sx[j] = a.x[b]; sy[j] = a.y[b]; sz[j] = a.z[b];
         sm[j] = (T)a.m[b];
         sid[j] = b;
       }
       const uint32_t *src = a.mask + mo + (int64_t)(c0 >> 5) * gc;
-      for (int q = t; q < nw * gc; q += kB) smask[q] = src[q];
+      for (int q = t; q < nw * gc; q += KB) smask[q] = src[q];
       __syncthreads();
       const int np = schedule(S, smask, nw, gc, eval);
       double wx = 0, wy = 0, wz = 0, wh2 = 0, wlo2 = 0, whi2 = 0;
@@ -204,7 +209,15 @@ __global__ __launch_bounds__(kB) void k_newton(NewtonArgs<T> a) {
             if (!(rr2 < wh2)) return;
             ++rc;
             if (final_pass || sid[j] == wia) return;
-            const T rr = (T)sqrt(rr2);
+            // r in the sum's precision: an FP32 sum takes the hardware
+            // square root of the rounded r^2 (within an ulp of the rounded
+            // FP64 root, which cost about 17 instructions per pair); the
+            // counts above use the exact FP64 r^2
+            T rr;
+            if constexpr (sizeof(T) == 4)
+              rr = __builtin_amdgcn_sqrtf((float)rr2);
+            else
+              rr = sqrt(rr2);
             T W, rdW;
             kernel_w_rdw<T>(a.kernel, rr, wih, &W, &rdW);
             r0.add(sm[j] * W);
@@ -365,7 +378,10 @@ int launch_newton(Ctx *c, const NewtonArgs<T> &a, int64_t nslots) {
   if (nslots <= 0) return 0;
   {
     PhaseTimer tm(c, FSPH_T_H);
-    k_newton<T><<<(unsigned)nslots, kB, 0, c->stream>>>(a);
+    if (c->tiles_w <= 64)
+      k_newton<T, 64><<<(unsigned)nslots, 64, 0, c->stream>>>(a);
+    else
+      k_newton<T, kB><<<(unsigned)nslots, kB, 0, c->stream>>>(a);
   }
   return hip_check(c, hipGetLastError(), "k_newton");
 }
