@@ -724,6 +724,8 @@ int fsph_integrate(fsph_ctx *ctx, uint32_t what) {
     return fsph::fail(c, FSPH_E_ARG,
                       "kick_e needs the thermokinetic formulation");
   FSPH_TRY(fsph::stage_integrate(c, what));
+  // moved particles: the tree and tiles no longer describe them
+  if (what & FSPH_DRIFT) c->have_tree = false;
   FSPH_HIP(c, hipStreamSynchronize(c->stream));
   return FSPH_OK;
 }

@@ -318,7 +318,11 @@ int fsph_initialize_alpha(fsph_ctx *ctx);
 int fsph_set_total_energy(fsph_ctx *ctx);
 
 /* Leapfrog pieces (include/physics/integration.h:53-110), with the
-   context's current dt.  `what` is a bitmask of FSPH_KICK_*. */
+   context's current dt.  `what` is a bitmask of FSPH_KICK_*.  A drift
+   moves the particles away from the tree and neighbour tiles, so the
+   calls that use them (density, forces, gravity, Newton-h and learned-h
+   updates, neighbour lists) then need fsph_update_iteration first, as in
+   hydro/main_driver.cc:187-193. */
 #define FSPH_KICK_V 1u      /* leapfrog_kick_v   integration.h:66-70     */
 #define FSPH_KICK_U 2u      /* leapfrog_kick_u   integration.h:81-84     */
 #define FSPH_SAVE_VHALF 4u  /* save_velocityhalf integration.h:53-55     */

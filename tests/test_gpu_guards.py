@@ -64,3 +64,26 @@ def test_clean_state_passes_and_checks_can_be_turned_off():
     with pytest.raises(FsphError):
         g.check_state()
     g.close()
+
+
+def test_a_drift_needs_a_new_tree_before_the_neighbour_passes():
+    """After fsph_integrate drifts the particles, the tree and tiles no
+    longer describe them: the passes that use them (density, the Newton-h
+    and learned-h updates) fail with FSPH_E_ARG until
+    fsph_update_iteration, the order of hydro/main_driver.cc:187-193; a
+    kick alone leaves them valid."""
+    parts, prm = _case()
+    g = BodySystem(_abi.default_params(**dict(prm, h_mode=_abi.H_NEWTON)))
+    g.upload(parts)
+    g.step()
+    g.integrate(_abi.KICK_V)
+    g.density_eos()
+    g.integrate(_abi.DRIFT)
+    for call in (g.density_eos, g.update_h):
+        with pytest.raises(FsphError) as e:
+            call()
+        assert e.value.code == _abi.FSPH_E_ARG
+    g.update_iteration()
+    g.density_eos()
+    g.update_h()
+    g.close()
