@@ -163,9 +163,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# Tuning of both runs of the learned-h comparison (the autotuner is not used
-# there: the predictor would keep learning during its steps); the fastest of
-# the autotuner's candidates on this workload.
+# Tuning (tile rows, chunk) of both runs of the learned-h comparison (the
+# autotuner is not used there: the predictor would keep learning during its
+# steps); the fastest of the autotuner's candidates on this workload.  The
+# grouping stays the reference's (group_max 128).
 LH_TUNING = (64, 64)
 
 
@@ -211,7 +212,8 @@ def learned_h_comparison(n, steps, learn):
         "workload": f"collapse_fcc_sphere_{n_real}_fmm_theta0.5_quad_leaf32",
         "steps": steps, "learn_steps": learn,
         "timed_steps": steps - learn,
-        "tuning": {"group_max": LH_TUNING[0], "chunk": LH_TUNING[1]},
+        "tuning": {"group_max": _abi.default_params(**prm).group_max,
+                   "tile_rows": LH_TUNING[0], "chunk": LH_TUNING[1]},
         "newton_h_ms": nw["h_ms"], "learned_h_ms": le["h_ms"],
         "h_phase_speedup": nw["h_ms"] / le["h_ms"],
         "newton_step_ms": nw["step_ms"], "learned_step_ms": le["step_ms"],
@@ -256,7 +258,8 @@ def batched_run(n_inst, steps):
         "ms_per_step": float(lat.mean()),
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),
-        "group_max": gm, "chunk": ch, "graph": True,
+        "group_max": _abi.default_params(**prm).group_max, "tile_rows": gm,
+        "chunk": ch, "graph": True,
     }
 
 
