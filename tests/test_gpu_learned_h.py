@@ -40,7 +40,7 @@ def _oracle_newton(oracle_mod, prm, state, h_old, ids):
     return {k: v[real] for k, v in out.items()}
 
 
-@pytest.mark.parametrize("case", ["collapse", "sedov"])
+@pytest.mark.parametrize("case", ["collapse", "sedov", "sedov_fp64"])
 def test_learned_h_is_the_newton_root(oracle_mod, case):
     if case == "collapse":
         # the collapse with its self-gravity (FMM, theta 0.5, quadrupole)
@@ -48,6 +48,8 @@ def test_learned_h_is_the_newton_root(oracle_mod, case):
         assert prm["gravity"] == _abi.GRAV_FMM
     else:
         parts, prm = generators.sedov(n_side=24)
+        if case == "sedov_fp64":  # the FP64 solver kernels
+            prm = dict(prm, precision=_abi.PREC_FP64)
     prm.update(h_mode=_abi.H_LEARNED, learn_steps=5)
     g = BodySystem(_abi.default_params(**prm))
     g.upload(parts)
